@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X particle-swarm backend (libpps).
+
+Headline workload (one step): the plain ParallelSyncPSOKernel path at the
+single-GPU benchmark shape — 10-D Rastrigin and 10-D Rosenbrock (p = [1, 100]),
+bounds [-5.12, 5.12], 2^20 particles each, 1000 iterations each, FP32 SoA state
+resident in HBM, the iteration loop replayed as a captured hipGraph.  `value`
+is particle-iterations per second over the timed steps (init excluded, as in
+the reference's benchmarks/CPU_vs_GPU/benchmark.jl:107-121).
+
+Also reported on the same line:
+  roofline      HBM roofline of the dominant kernel (the fused sync step):
+                algorithmic bytes per launch / average launch time (hipEvents
+                on the swarm's stream over the timed region).
+  async         ParallelPSOKernel(global_update=false) at the same shape.
+  c0            3-D Rosenbrock, 1000 particles, 500 iterations (one
+                resident workgroup): time per solve and per iteration.
+  cpu_baseline  the CPU oracle (reference-faithful AoS restatement, OpenMP
+                over particles + serial argmin per iteration) on a bounded
+                sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "parallelparticleswarms.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "particle-iters/sec plain vs batched; served solves/sec + p99 latency"
+HBM_PEAK = 8.0e12
+N = 1 << 20
+D = 10
+T = 1000
+BYTES_PER_PARTICLE_ITER = 4 * (5 * D + 1)  # read x,v,best (3d) + best_cost, write x,v (2d)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_swarm(P, obj, n, d, seed, mode, launch=None):
+    lb = np.full(d, -5.12, np.float32)
+    ub = np.full(d, 5.12, np.float32)
+    f = P.Rastrigin() if obj == "rastrigin" else P.Rosenbrock()
+    params = None if obj == "rastrigin" else [1.0, 100.0]
+    prob = P.OptimizationProblem(f, np.zeros(d, np.float32), params, lb=lb, ub=ub)
+    from ppswarm import _lib as L
+
+    sp = P.swarm_params(n, mode, seed=seed,
+                        launch=L.LAUNCH_GRAPH if launch is None else launch)
+    return P.Swarm(P.context(), prob, sp)
+
+
+def traffic_from_profiles():
+    """HBM bytes per sync-step launch measured with rocprofv3 --pmc, if the
+    summary is committed (profiles/sync_step_c1_traffic.md)."""
+    path = os.path.join(ROOT, "profiles", "sync_step_c1_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def cpu_baseline(sample_iters=6):
+    import pps_oracle as O
+
+    O.lib()
+    threads = O.lib().ora_num_threads()
+    total_pi, total_s = 0, 0.0
+    for obj, params in ((O.RASTRIGIN, ()), (O.ROSENBROCK, (1.0, 100.0))):
+        prob = O.Problem(obj, D, np.full(D, -5.12, np.float32), np.full(D, 5.12, np.float32),
+                         np.asarray(params, np.float32))
+        s = O.Swarm(prob, N, seed=7, nthreads=threads)
+        s.run_sync(1)  # warm-up
+        t0 = time.perf_counter()
+        s.run_sync(sample_iters)
+        total_s += time.perf_counter() - t0
+        total_pi += N * sample_iters
+    return {"value": total_pi / total_s, "unit": "particle-iters/s", "cores": threads,
+            "kind": "port",
+            "sample": f"sync PSO, 10-D Rastrigin + 10-D Rosenbrock, 2^20 particles each, "
+                      f"{sample_iters} timed iterations each (after 1 warm-up), "
+                      f"{total_s:.1f} s of CPU time"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the swarm path has no multi-GPU launch", file=sys.stderr)
+        sys.exit(2)
+
+    import ppswarm as P
+    from ppswarm import _lib as L
+
+    ctx = P.context()
+    info = ctx.info()
+    log(f"[bench] device: {info}")
+    swarms = [make_swarm(P, "rastrigin", N, D, 101, L.MODE_SYNC),
+              make_swarm(P, "rosenbrock", N, D, 202, L.MODE_SYNC)]
+
+    def step():
+        ms = 0.0
+        for s in swarms:
+            s.run(T)
+            ms += s.elapsed_ms()
+        return ms
+
+    for i in range(args.warmup):
+        step()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        dev_ms += step()
+        log(f"[bench] step {i + 1}/{args.steps} done")
+    wall = time.perf_counter() - t0
+    pi_per_step = 2 * N * T
+    value = pi_per_step * args.steps / wall
+    launches = 2 * T * args.steps
+    avg_launch_s = dev_ms / 1e3 / launches
+    bytes_per_launch = N * BYTES_PER_PARTICLE_ITER
+    achieved = bytes_per_launch / avg_launch_s
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain ParallelSyncPSOKernel, 10-D Rastrigin + 10-D Rosenbrock "
+                               "(p=[1,100]), bounds [-5.12,5.12], 2^20 particles each, "
+                               "1000 iterations each per step, FP32 SoA, hipGraph loop",
+                   "particles": N, "dim": D, "iterations": T, "problems_per_step": 2},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": traffic_from_profiles(), "kernel": "k_sync_step",
+                     "bytes_per_launch": bytes_per_launch,
+                     "avg_launch_us": avg_launch_s * 1e6},
+    }
+    for s in swarms:
+        s.free()
+
+    if not args.no_extras:
+        # async (ParallelPSOKernel(global_update=false)) at the same shape
+        a = [make_swarm(P, "rastrigin", N, D, 303, L.MODE_ASYNC),
+             make_swarm(P, "rosenbrock", N, D, 404, L.MODE_ASYNC)]
+        for s in a:
+            s.run(50)
+        t0 = time.perf_counter()
+        ams = 0.0
+        for s in a:
+            s.run(T)
+            ams += s.elapsed_ms()
+        aw = time.perf_counter() - t0
+        out["async"] = {"value": 2 * N * T / aw, "unit": "particle-iters/s",
+                        "device_ms": ams, "workload": "same shape, one launch per problem"}
+        for s in a:
+            s.free()
+        log("[bench] async done")
+        # C0: 3-D Rosenbrock, lb=-1, ub=10, 1000 particles, 500 iterations
+        prob = P.OptimizationProblem(P.Rosenbrock(), np.zeros(3, np.float32), [1.0, 100.0],
+                                     lb=np.full(3, -1.0), ub=np.full(3, 10.0))
+        times, dms = [], []
+        for rep in range(6):
+            s = P.Swarm(ctx, prob, P.swarm_params(1000, L.MODE_SYNC, seed=rep))
+            t0 = time.perf_counter()
+            s.run(500)
+            times.append(time.perf_counter() - t0)
+            dms.append(s.elapsed_ms())
+            s.free()
+        out["c0"] = {"solve_ms_median": float(np.median(times[1:]) * 1e3),
+                     "device_us_per_iter": float(np.median(dms[1:]) * 1e3 / 500),
+                     "particle_iters_per_s": 1000 * 500 / float(np.median(times[1:]))}
+        log("[bench] c0 done")
+
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        out["cpu_baseline"] = cpu_baseline()
+        log("[bench] cpu baseline done")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

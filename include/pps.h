@@ -1,0 +1,196 @@
+/*
+ * pps.h — C ABI of libpps, the MI355X (gfx950) particle-swarm backend.
+ *
+ * Drop-in boundary for the particle-swarm hot path of ParallelParticleSwarms.jl
+ * (v1.4.0).  A Julia package extension `ccall`s these entry points in place of
+ * the KernelAbstractions kernels and host drivers the reference launches; the
+ * Python package `ppswarm` binds the same symbols with ctypes.
+ *
+ * Conventions
+ *  - Every function returns int: PPS_OK (0) or a negative PPS_E_* code.  The
+ *    message of the last failure on the calling thread is pps_last_error().
+ *    No C++ exception crosses this boundary.
+ *  - Handles (pps_ctx, pps_swarm, pps_server) are library-owned; free them with
+ *    the matching _free / _destroy call.  Host buffers are caller-owned and are
+ *    only read or written for the duration of the call.
+ *  - A pps_ctx (one HIP device + one stream) is single-thread-affine.  A
+ *    pps_server is thread-safe for submit/poll/wait.
+ *  - Device arithmetic is FP32.  pps_eval_f64 evaluates objectives in FP64 on
+ *    the host.
+ *  - Positions exchanged as "AoS" are particle-major, N x d floats, the layout
+ *    of a Vector{SVector{d,Float32}} (reference src/ParallelParticleSwarms.jl:34-40).
+ *    State exchanged as "SoA" is dimension-major, d x N floats (row j holds
+ *    coordinate j of every particle), the layout the kernels keep in HBM.
+ */
+#ifndef PPS_H
+#define PPS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PPS_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  PPS_OK = 0,
+  PPS_E_INVALID = -1,     /* bad argument / unsupported combination         */
+  PPS_E_HIP = -2,         /* HIP runtime error                              */
+  PPS_E_NOMEM = -3,       /* device allocation failed or admission refused  */
+  PPS_E_NOTFOUND = -4,    /* unknown ticket                                 */
+  PPS_E_PENDING = -5,     /* request not finished yet (poll)                */
+  PPS_E_CANCELLED = -6,   /* request cancelled                              */
+  PPS_E_TIMEOUT = -7,     /* wait timed out                                 */
+  PPS_E_STATE = -8        /* wrong lifecycle state                          */
+};
+
+/* ---- objectives (closed set; the reference accepts any Julia closure,
+ *      src/kernels.jl:31-33, which cannot cross a C ABI) ------------------ */
+enum {
+  /* Σ_{i<d} p2*(x_{i+1}-x_i^2)^2 + (p1-x_i)^2, p=[p1,p2]
+   * (reference test/gpu.jl:14-20, README.md:20-22). */
+  PPS_OBJ_ROSENBROCK = 1,
+  /* 10 d + Σ (x_i^2 - 10 cos(2π x_i)). */
+  PPS_OBJ_RASTRIGIN = 2,
+  /* Σ x_i^2. */
+  PPS_OBJ_SPHERE = 3,
+  /* Two-dimensional known-answer functions of reference test/lbfgs.jl:29-35. */
+  PPS_OBJ_BEALE = 4,
+  PPS_OBJ_BOOTH = 5,
+  PPS_OBJ_HIMMELBLAU = 6,
+  PPS_OBJ_QUADRATIC = 7,          /* p1 (x1-1)^2 + (x2+2)^2              */
+  PPS_OBJ_BOUNDARY_QUADRATIC = 8, /* (x1+1)^2                            */
+  PPS_OBJ_ROSEN2 = 9,             /* (p1-x1)^2 + p2 (x2-x1^2)^2          */
+  /* Lotka-Volterra parameter estimation: sum of squared errors of an FP32
+   * RK4 trajectory against observations (reference src/ode_pso.jl:47-52). */
+  PPS_OBJ_LOTKA_VOLTERRA = 16
+};
+
+/* Swarm update semantics. */
+enum {
+  PPS_MODE_SYNC = 0,  /* ParallelSyncPSOKernel: Jacobi step, argmin after each
+                         iteration (src/kernels.jl:127-194,
+                         src/lowerlevel_solve.jl:1-65)                        */
+  PPS_MODE_ASYNC = 1, /* ParallelPSOKernel(global_update=false): one launch,
+                         per-particle running best (src/kernels.jl:196-225)   */
+  PPS_MODE_QLOCK = 2  /* ParallelPSOKernel(global_update=true): live global
+                         best, block queue + lock (src/kernels.jl:38-125)     */
+};
+
+/* Which position becomes the global best after a sync iteration. */
+enum {
+  PPS_GBEST_PBEST = 0,   /* best_position of the argmin particle (GPU backend,
+                            src/kernels.jl:153,171)                          */
+  PPS_GBEST_CURRENT = 1  /* position of the argmin particle (CPU backend,
+                            src/lowerlevel_solve.jl:59-60)                   */
+};
+
+/* Bound handling after x <- x + v. */
+enum {
+  PPS_CLAMP_COMPONENT = 0,    /* min.(max.(x, lb), ub)  src/kernels.jl:15-17 */
+  PPS_CLAMP_LEXICOGRAPHIC = 1 /* min(max(x, lb), ub) on whole vectors,
+                                 src/ode_pso.jl:18-19                        */
+};
+
+/* Launch strategy for PPS_MODE_SYNC. */
+enum {
+  PPS_LAUNCH_AUTO = 0,
+  PPS_LAUNCH_EAGER = 1,      /* one launch per iteration, no graph            */
+  PPS_LAUNCH_GRAPH = 2,      /* iteration loop captured in a hipGraph         */
+  PPS_LAUNCH_WORKGROUP = 3   /* whole loop in one workgroup (small swarms)    */
+};
+
+/* Problem: objective, bounds and parameters (OptimizationProblem fields
+ * f, lb, ub, p; reference src/init.jl:88-115, src/utils.jl:310-318). */
+typedef struct pps_problem {
+  int32_t objective;     /* PPS_OBJ_*                                        */
+  int32_t d;             /* dimension, 1..64 (LOTKA_VOLTERRA: 4)             */
+  const float* lb;       /* d floats, finite                                 */
+  const float* ub;       /* d floats, finite, lb <= ub                       */
+  const float* params;   /* n_params floats (Rosenbrock: [p1, p2])           */
+  int32_t n_params;
+  const float* shift;    /* d floats or NULL: f is evaluated at x - shift    */
+  /* PPS_OBJ_LOTKA_VOLTERRA only: observations u(t_k), k = 0..n_obs-1,
+   * interleaved [u1(t0), u2(t0), u1(t1), ...]; the trajectory starts at
+   * params[0..1] = u0, steps dt = params[2], and is sampled every
+   * (int)params[3] steps. */
+  const float* obs;
+  int32_t n_obs;
+} pps_problem;
+
+/* Swarm/algorithm parameters (ParallelSyncPSOKernel / ParallelPSOKernel
+ * fields, src/algorithms.jl:78-121,192-204). */
+typedef struct pps_swarm_params {
+  int64_t n;                 /* num_particles                                */
+  int32_t mode;              /* PPS_MODE_*                                   */
+  int32_t gbest_from;        /* PPS_GBEST_* (sync only)                      */
+  int32_t clamp;             /* PPS_CLAMP_*                                  */
+  int32_t launch;            /* PPS_LAUNCH_* (sync only)                     */
+  uint64_t seed;             /* Philox key of this swarm's random stream     */
+  float c1, c2;              /* 1.4962f0 in the reference                    */
+  int32_t qlock_min_improvers; /* QLOCK: block publishes only when at least
+                                  this many particles beat the global best;
+                                  the reference uses 2 (src/kernels.jl:97)   */
+  int32_t block;             /* threads per workgroup, 0 = tuned             */
+} pps_swarm_params;
+
+/* Fill p with the reference defaults (c1 = c2 = 1.4962, SYNC, PBEST,
+ * per-component clamp, qlock_min_improvers = 2, auto launch). */
+void pps_swarm_params_default(pps_swarm_params* p);
+
+typedef struct pps_ctx pps_ctx;
+typedef struct pps_swarm pps_swarm;
+typedef struct pps_server pps_server;
+
+/* ---- errors / context --------------------------------------------------- */
+const char* pps_last_error(void);
+int pps_abi_version(void);
+/* Create a context on HIP device `device` with its own stream. */
+int pps_ctx_create(int device, pps_ctx** out);
+int pps_ctx_destroy(pps_ctx* ctx);
+/* Device facts read from hipDeviceProp: CU count, clock (kHz), memory. */
+int pps_ctx_info(pps_ctx* ctx, int32_t* num_cus, int32_t* clock_khz,
+                 int64_t* total_mem, int64_t* free_mem);
+
+/* ---- swarm lifecycle (init / solve! / reinit!, src/init.jl:56-141) ----- */
+/* Allocate SoA state in HBM and run the uniform initialiser
+ * (src/utils.jl:57-84, src/init.jl:38-54): x_j = r*(ub_j-lb_j)+lb_j, v = 0,
+ * cost = f(x), best = x; gbest = argmin (ties: lowest index). */
+int pps_swarm_init(pps_ctx* ctx, const pps_problem* prob,
+                   const pps_swarm_params* params, pps_swarm** out);
+/* Draw a fresh initial swarm (next epoch of the random stream). */
+int pps_swarm_reinit(pps_swarm* s);
+/* Run `maxiters` iterations; w_k = w * wdamp^k as an iterated FP32 product
+ * (src/lowerlevel_solve.jl:23-33).  Resumable: the random stream continues. */
+int pps_swarm_run(pps_swarm* s, int32_t maxiters, float w, float wdamp);
+/* Global best after the last run: position (d floats), FP32 cost, particle
+ * index of the argmin (PPS_MODE_QLOCK: index of the last published winner). */
+int pps_swarm_get_gbest(pps_swarm* s, float* pos, float* cost, int64_t* idx);
+/* Set the attractor used by the next iteration (the reference's
+ * cache.gbest, src/init.jl:88-115; ParallelSyncPSOKernel does not write the
+ * result back into it, src/solve.jl:36-59). */
+int pps_swarm_set_gbest(pps_swarm* s, const float* pos, float cost);
+/* Current positions, AoS (N x d): the solution's `original` field
+ * (src/solve.jl:53). */
+int pps_swarm_get_positions(pps_swarm* s, float* out_aos);
+/* Full SoA state (any pointer may be NULL): x, v, best (d x N each), best_cost
+ * (N), and the iteration counter of the random stream. */
+int pps_swarm_get_state(pps_swarm* s, float* x, float* v, float* best,
+                        float* best_cost, uint32_t* iter);
+int pps_swarm_set_state(pps_swarm* s, const float* x, const float* v,
+                        const float* best, const float* best_cost,
+                        uint32_t iter);
+/* Device time of the last run (hipEvents on the swarm's stream). */
+int pps_swarm_elapsed_ms(pps_swarm* s, float* ms);
+int pps_swarm_free(pps_swarm* s);
+
+/* ---- host FP64 re-evaluation -------------------------------------------- */
+int pps_eval_f64(const pps_problem* prob, int64_t n, const double* x_aos,
+                 double* f);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PPS_H */

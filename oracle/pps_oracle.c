@@ -1,0 +1,465 @@
+/*
+ * pps_oracle.c — CPU restatement of ParallelParticleSwarms.jl's particle-swarm
+ * hot path.  TEST INFRASTRUCTURE ONLY: the parity tests, the smoke check and
+ * the bench's CPU-baseline leg call it as the checker.  Nothing in the
+ * product links or calls it.
+ *
+ * It follows the reference (v1.4.0) as text, function by function:
+ *   - particle layout: SPSOParticle{SVector{d,Float32},Float32} as an array of
+ *     structs, (3d+2) floats per particle: position, velocity, cost,
+ *     best_position, best_cost (src/ParallelParticleSwarms.jl:34-40);
+ *   - the per-particle step update_particle_state (src/kernels.jl:1-26) in
+ *     Julia's evaluation order, FP32, no FMA contraction;
+ *   - the sync CPU kernel + driver (src/kernels.jl:182-194,
+ *     src/lowerlevel_solve.jl:38-65) and the GPU sync driver's gbest choice
+ *     (src/kernels.jl:127-177, src/lowerlevel_solve.jl:1-36);
+ *   - the async kernel (src/kernels.jl:196-225, src/lowerlevel_solve.jl:95-121);
+ *   - the queue-lock kernel (src/kernels.jl:38-125, src/lowerlevel_solve.jl:67-93)
+ *     with its blocks run one after another in index order (one legal schedule);
+ *   - uniform initialisation (src/utils.jl:1-5,57-84; src/init.jl:38-54);
+ *   - the ODE clamp quirk (src/ode_pso.jl:18-19) and loss (src/ode_pso.jl:47-52).
+ * Julia's task-local RNG is replaced by Philox4x32-10 under the stream layout
+ * that the kernels use (the reference's own RNG cannot be shared with a GPU
+ * kernel written elsewhere).
+ *
+ * Compile with -ffp-contract=off and without -ffast-math.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORA_MAXD 64
+#define ORA_MAXPAR 8
+
+enum {
+  ORA_ROSENBROCK = 1, ORA_RASTRIGIN = 2, ORA_SPHERE = 3, ORA_BEALE = 4,
+  ORA_BOOTH = 5, ORA_HIMMELBLAU = 6, ORA_QUADRATIC = 7,
+  ORA_BOUNDARY_QUADRATIC = 8, ORA_ROSEN2 = 9, ORA_LOTKA_VOLTERRA = 16
+};
+enum { ORA_GBEST_PBEST = 0, ORA_GBEST_CURRENT = 1 };
+enum { ORA_CLAMP_COMPONENT = 0, ORA_CLAMP_LEXICOGRAPHIC = 1 };
+
+typedef struct ora_problem {
+  int32_t obj, d, has_shift, clamp;
+  float lb[ORA_MAXD], ub[ORA_MAXD], shift[ORA_MAXD], par[ORA_MAXPAR];
+  const float* obs;
+  int32_t n_obs, steps_per_obs;
+} ora_problem;
+
+/* ---------------------------------------------------------------- Philox */
+/* Philox4x32-10, Salmon et al. SC'11: round multipliers 0xD2511F53,
+ * 0xCD9E8D57, Weyl increments 0x9E3779B9, 0xBB67AE85. */
+void ora_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    uint64_t a = (uint64_t)0xD2511F53u * c0;
+    uint64_t b = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(b >> 32) ^ c1 ^ k0;
+    uint32_t n1 = (uint32_t)b;
+    uint32_t n2 = (uint32_t)(a >> 32) ^ c3 ^ k1;
+    uint32_t n3 = (uint32_t)a;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline float u01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
+
+static inline void seed_key(uint64_t seed, uint32_t key[2]) {
+  key[0] = (uint32_t)seed;
+  key[1] = (uint32_t)(seed >> 32);
+}
+
+/* ------------------------------------------------- Julia scalar semantics */
+/* Base.max/Base.min on Float32 (NaN-propagating, max(-0,+0) = +0). */
+static inline float jl_max(float x, float y) {
+  float diff = x - y;
+  float pick = signbit(diff) ? y : x;
+  return (isnan(x) || isnan(y)) ? diff : pick;
+}
+static inline float jl_min(float x, float y) {
+  float diff = x - y;
+  float pick = signbit(diff) ? x : y;
+  return (isnan(x) || isnan(y)) ? diff : pick;
+}
+/* Base.isless on Float32: total order with -0 < +0 and NaN last. */
+static inline int jl_isless(float a, float b) {
+  if (isnan(a) || isnan(b)) return !isnan(a) && isnan(b);
+  if (a == 0.0f && b == 0.0f) return signbit(a) && !signbit(b);
+  return a < b;
+}
+static inline int jl_isequal(float a, float b) {
+  if (isnan(a) && isnan(b)) return 1;
+  if (a == 0.0f && b == 0.0f) return signbit(a) == signbit(b);
+  return a == b;
+}
+/* Base.cmp(::AbstractVector, ::AbstractVector): lexicographic with isequal /
+ * isless on the first differing component. */
+static int jl_cmp_vec(const float* a, const float* b, int d) {
+  for (int j = 0; j < d; ++j)
+    if (!jl_isequal(a[j], b[j])) return jl_isless(a[j], b[j]) ? -1 : 1;
+  return 0;
+}
+
+/* ------------------------------------------------------------ objectives */
+/* cos(2 pi x): reduce r = x - rint(x) (exact), a = |r|, fold a > 1/4 to
+ * 1/2 - a with a sign flip, then the degree-14 Taylor polynomial of
+ * cos(2 pi a) in s = a^2 (coefficients (-1)^k (2pi)^2k/(2k)! rounded to FP32),
+ * Horner with fmaf. */
+float ora_cos2pi(float x) {
+  static const float C[8] = {1.0f,          -19.739208802178716f, 64.93939402266828f,
+                             -85.45681720669371f, 60.24464137187664f, -26.426256783374388f,
+                             7.903536371318465f,  -1.7143907110886711f};
+  float r = x - rintf(x);
+  float a = fabsf(r);
+  int flip = a > 0.25f;
+  if (flip) a = 0.5f - a;
+  float s = a * a;
+  float p = C[7];
+  for (int k = 6; k >= 0; --k) p = fmaf(p, s, C[k]);
+  return flip ? -p : p;
+}
+float ora_sin2pi(float x) {
+  float r = x - rintf(x);
+  return ora_cos2pi(r - 0.25f);
+}
+
+/* Lotka-Volterra (benchmarks/Lotka_Volterra/lotka_volterra.jl:9-13) and the
+ * fixed-step classical RK4 SSE loss against observations (src/ode_pso.jl:47-52:
+ * per sample sum(x.^2) of the 2-vector error, samples summed in order). */
+static void lv_rhs(float u1, float u2, const float* p, float* f1, float* f2) {
+  *f1 = p[0] * u1 - (p[1] * u1) * u2;
+  *f2 = (-p[2]) * u2 + (p[3] * u1) * u2;
+}
+float ora_lv_sse(const float* th, const ora_problem* P) {
+  float u1 = P->par[0], u2 = P->par[1];
+  float dt = P->par[2];
+  float h2 = dt * 0.5f;
+  float h6 = dt / 6.0f;
+  float loss = 0.0f;
+  for (int k = 0; k < P->n_obs; ++k) {
+    if (k > 0) {
+      for (int s = 0; s < P->steps_per_obs; ++s) {
+        float k1a, k1b, k2a, k2b, k3a, k3b, k4a, k4b;
+        lv_rhs(u1, u2, th, &k1a, &k1b);
+        lv_rhs(u1 + h2 * k1a, u2 + h2 * k1b, th, &k2a, &k2b);
+        lv_rhs(u1 + h2 * k2a, u2 + h2 * k2b, th, &k3a, &k3b);
+        lv_rhs(u1 + dt * k3a, u2 + dt * k3b, th, &k4a, &k4b);
+        u1 = u1 + h6 * (((k1a + 2.0f * k2a) + 2.0f * k3a) + k4a);
+        u2 = u2 + h6 * (((k1b + 2.0f * k2b) + 2.0f * k3b) + k4b);
+      }
+    }
+    float e1 = P->obs[2 * k] - u1;
+    float e2 = P->obs[2 * k + 1] - u2;
+    loss = loss + (e1 * e1 + e2 * e2);
+  }
+  return loss;
+}
+
+/* f(x - shift).  Rosenbrock in the loop form of test/gpu.jl:14-20:
+ * res += p[2] * (x[i+1] - x[i]^2)^2 + (p[1] - x[i])^2. */
+float ora_objective(const ora_problem* P, const float* xin) {
+  float z[ORA_MAXD];
+  int d = P->d;
+  for (int j = 0; j < d; ++j) z[j] = P->has_shift ? xin[j] - P->shift[j] : xin[j];
+  switch (P->obj) {
+    case ORA_ROSENBROCK: {
+      float res = 0.0f;
+      for (int i = 0; i + 1 < d; ++i) {
+        float dd = z[i + 1] - z[i] * z[i];
+        float e = P->par[0] - z[i];
+        res = res + (P->par[1] * (dd * dd) + e * e);
+      }
+      return res;
+    }
+    case ORA_RASTRIGIN: {
+      float res = 10.0f * (float)d;
+      for (int i = 0; i < d; ++i) res = res + (z[i] * z[i] - 10.0f * ora_cos2pi(z[i]));
+      return res;
+    }
+    case ORA_SPHERE: {
+      float res = 0.0f;
+      for (int i = 0; i < d; ++i) res = res + z[i] * z[i];
+      return res;
+    }
+    case ORA_BEALE: {
+      float x1 = z[0], x2 = z[1];
+      float a = (1.5f - x1) + x1 * x2;
+      float b = (2.25f - x1) + x1 * (x2 * x2);
+      float c = (2.625f - x1) + x1 * ((x2 * x2) * x2);
+      return (a * a + b * b) + c * c;
+    }
+    case ORA_BOOTH: {
+      float a = (z[0] + 2.0f * z[1]) - 7.0f, b = (2.0f * z[0] + z[1]) - 5.0f;
+      return a * a + b * b;
+    }
+    case ORA_HIMMELBLAU: {
+      float a = (z[0] * z[0] + z[1]) - 11.0f, b = (z[0] + z[1] * z[1]) - 7.0f;
+      return a * a + b * b;
+    }
+    case ORA_QUADRATIC: {
+      float a = z[0] - 1.0f, b = z[1] + 2.0f;
+      return P->par[0] * (a * a) + b * b;
+    }
+    case ORA_BOUNDARY_QUADRATIC: {
+      float a = z[0] + 1.0f;
+      return a * a;
+    }
+    case ORA_ROSEN2: {
+      float a = P->par[0] - z[0], b = z[1] - z[0] * z[0];
+      return a * a + P->par[1] * (b * b);
+    }
+    case ORA_LOTKA_VOLTERRA:
+      return ora_lv_sse(z, P);
+  }
+  return NAN;
+}
+
+/* ---------------------------------------------------------- particle AoS */
+#define POS(p, d) (p)
+#define VEL(p, d) ((p) + (d))
+#define COST(p, d) ((p)[2 * (d)])
+#define BPOS(p, d) ((p) + 2 * (d) + 1)
+#define BCOST(p, d) ((p)[3 * (d) + 1])
+static inline int64_t stride_of(int d) { return 3 * (int64_t)d + 2; }
+
+/* Argmin of best_cost, ties to the lowest index, NaN never below a number
+ * (the key the kernels reduce: order(cost) << 32 | index). */
+static inline int better(float a, int64_t ia, float b, int64_t ib) {
+  int an = isnan(a), bn = isnan(b);
+  if (an != bn) return bn;           /* number beats NaN */
+  if (!an && a != b) return a < b;   /* +0 == -0 */
+  return ia < ib;
+}
+int64_t ora_argmin(const float* P, int64_t n, int d) {
+  int64_t s = stride_of(d), best = 0;
+  for (int64_t i = 1; i < n; ++i)
+    if (better(BCOST(P + i * s, d), i, BCOST(P + best * s, d), best)) best = i;
+  return best;
+}
+
+/* update_particle_state, src/kernels.jl:1-26 (velocity_position part):
+ *   v = w .* v .+ c1 .* r1 .* (best - x) .+ c2 .* r2 .* (gbest - x)
+ *   x = x + v ; x = min.(max.(x, lb), ub)          (per-component clamp)
+ *   or min(max(x, lb), ub) on whole vectors        (ODE clamp, ode_pso.jl:18-19)
+ * r1[j], r2[j] from philox(ctr = (i, t, floor(j/2) | 0 << 16, epoch)):
+ * lanes (0,1) for even j, lanes (2,3) for odd j. */
+static void move_particle(float* p, const ora_problem* P, const float* gb, float w,
+                          float c1, float c2, uint32_t i, uint32_t t, uint32_t epoch,
+                          const uint32_t key[2]) {
+  int d = P->d;
+  float* x = POS(p, d);
+  float* v = VEL(p, d);
+  const float* pb = BPOS(p, d);
+  uint32_t r[4];
+  for (int j = 0; j < d; ++j) {
+    if ((j & 1) == 0) {
+      uint32_t ctr[4] = {i, t, (uint32_t)(j >> 1), epoch};
+      ora_philox(ctr, key, r);
+    }
+    float r1 = u01(r[(j & 1) ? 2 : 0]);
+    float r2 = u01(r[(j & 1) ? 3 : 1]);
+    float a = w * v[j];
+    float b = (c1 * r1) * (pb[j] - x[j]);
+    float c = (c2 * r2) * (gb[j] - x[j]);
+    v[j] = (a + b) + c;
+  }
+  for (int j = 0; j < d; ++j) x[j] = x[j] + v[j];
+  if (P->clamp == ORA_CLAMP_COMPONENT) {
+    for (int j = 0; j < d; ++j) x[j] = jl_max(x[j], P->lb[j]);
+    for (int j = 0; j < d; ++j) x[j] = jl_min(x[j], P->ub[j]);
+  } else {
+    /* max(x, lb) = isless(lb, x) ? x : lb ; min(y, ub) = isless(ub, y) ? ub : y */
+    if (!(jl_cmp_vec(P->lb, x, d) < 0)) memcpy(x, P->lb, sizeof(float) * d);
+    if (jl_cmp_vec(P->ub, x, d) < 0) memcpy(x, P->ub, sizeof(float) * d);
+  }
+}
+
+/* cost = f(x); if cost < best_cost: best = x (src/kernels.jl:19-24). */
+static void evaluate_particle(float* p, const ora_problem* P) {
+  int d = P->d;
+  COST(p, d) = ora_objective(P, POS(p, d));
+  if (COST(p, d) < BCOST(p, d)) {
+    memcpy(BPOS(p, d), POS(p, d), sizeof(float) * d);
+    BCOST(p, d) = COST(p, d);
+  }
+}
+
+static void step_particle(float* p, const ora_problem* P, const float* gb, float w,
+                          float c1, float c2, uint32_t i, uint32_t t, uint32_t epoch,
+                          const uint32_t key[2]) {
+  move_particle(p, P, gb, w, c1, c2, i, t, epoch, key);
+  evaluate_particle(p, P);
+}
+
+/* Uniform init (src/utils.jl:1-5,57-84): x_j = rand() * (ub_j - lb_j) + lb_j,
+ * rand() for dimension j from philox(ctr = (i, 0, floor(j/4) | 1 << 16,
+ * epoch)) lane j mod 4; v = 0; cost = f(x); best = x.  The init global best is
+ * (best_position, best_cost) of the argmin (src/init.jl:49-51). */
+void ora_init(const ora_problem* P, int64_t n, uint64_t seed, uint32_t epoch, float* parts,
+              float* gb_pos, float* gb_cost, int64_t* gb_idx, int nthreads) {
+  int d = P->d;
+  int64_t s = stride_of(d);
+  uint32_t key[2];
+  seed_key(seed, key);
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t i = 0; i < n; ++i) {
+    float* p = parts + i * s;
+    uint32_t r[4];
+    for (int j = 0; j < d; ++j) {
+      if ((j & 3) == 0) {
+        uint32_t ctr[4] = {(uint32_t)i, 0u, (uint32_t)(j >> 2) | (1u << 16), epoch};
+        ora_philox(ctr, key, r);
+      }
+      POS(p, d)[j] = u01(r[j & 3]) * (P->ub[j] - P->lb[j]) + P->lb[j];
+      VEL(p, d)[j] = 0.0f;
+    }
+    COST(p, d) = ora_objective(P, POS(p, d));
+    memcpy(BPOS(p, d), POS(p, d), sizeof(float) * d);
+    BCOST(p, d) = COST(p, d);
+  }
+  int64_t b = ora_argmin(parts, n, d);
+  memcpy(gb_pos, BPOS(parts + b * s, d), sizeof(float) * d);
+  *gb_cost = BCOST(parts + b * s, d);
+  *gb_idx = b;
+}
+
+/* Sync PSO (ParallelSyncPSOKernel): every iteration steps all particles
+ * against the previous global best (src/kernels.jl:182-194 on the CPU
+ * backend, :127-177 on a GPU backend), then takes the argmin of best_cost and
+ * sets gbest = (position, best_cost) on the CPU backend
+ * (src/lowerlevel_solve.jl:59-60) or (best_position, best_cost) on a GPU
+ * backend (src/kernels.jl:153,171); w = w * wdamp after each iteration.
+ * Iteration k of this call uses random-stream iteration t0 + k (k = 1..).
+ * gb_pos holds the attractor on entry and the final global best on exit. */
+void ora_sync_run(const ora_problem* P, int64_t n, uint64_t seed, uint32_t epoch, uint32_t t0,
+                  int32_t iters, float w, float wdamp, float c1, float c2, int32_t gbest_from,
+                  float* parts, float* gb_pos, float* gb_cost, int64_t* gb_idx, int nthreads) {
+  int d = P->d;
+  int64_t s = stride_of(d);
+  uint32_t key[2];
+  seed_key(seed, key);
+  float gb[ORA_MAXD];
+  memcpy(gb, gb_pos, sizeof(float) * d);
+  for (int32_t k = 1; k <= iters; ++k) {
+    uint32_t t = t0 + (uint32_t)k;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t i = 0; i < n; ++i)
+      step_particle(parts + i * s, P, gb, w, c1, c2, (uint32_t)i, t, epoch, key);
+    int64_t b = ora_argmin(parts, n, d);
+    const float* src = gbest_from == ORA_GBEST_CURRENT ? POS(parts + b * s, d)
+                                                       : BPOS(parts + b * s, d);
+    memcpy(gb, src, sizeof(float) * d);
+    *gb_cost = BCOST(parts + b * s, d);
+    *gb_idx = b;
+    w = w * wdamp;
+  }
+  memcpy(gb_pos, gb, sizeof(float) * d);
+}
+
+/* Async PSO (ParallelPSOKernel, global_update = false; src/kernels.jl:196-225):
+ * each particle runs all iterations against its own running best, seeded with
+ * the global best at launch and replaced whenever best_cost < its cost; the
+ * result is the argmin over particles of (best_position, best_cost)
+ * (src/lowerlevel_solve.jl:119-120). */
+void ora_async_run(const ora_problem* P, int64_t n, uint64_t seed, uint32_t epoch, uint32_t t0,
+                   int32_t iters, float w0, float wdamp, float c1, float c2, float* parts,
+                   float* gb_pos, float* gb_cost, int64_t* gb_idx, int nthreads) {
+  int d = P->d;
+  int64_t s = stride_of(d);
+  uint32_t key[2];
+  seed_key(seed, key);
+  float g0[ORA_MAXD];
+  memcpy(g0, gb_pos, sizeof(float) * d);
+  float g0c = *gb_cost;
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t i = 0; i < n; ++i) {
+    float* p = parts + i * s;
+    float g[ORA_MAXD];
+    memcpy(g, g0, sizeof(float) * d);
+    float gc = g0c;
+    float w = w0;
+    for (int32_t k = 1; k <= iters; ++k) {
+      step_particle(p, P, g, w, c1, c2, (uint32_t)i, t0 + (uint32_t)k, epoch, key);
+      if (BCOST(p, d) < gc) {
+        memcpy(g, BPOS(p, d), sizeof(float) * d);
+        gc = BCOST(p, d);
+      }
+      w = w * wdamp;
+    }
+  }
+  int64_t b = ora_argmin(parts, n, d);
+  memcpy(gb_pos, BPOS(parts + b * s, d), sizeof(float) * d);
+  *gb_cost = BCOST(parts + b * s, d);
+  *gb_idx = b;
+}
+
+/* Queue-lock PSO (ParallelPSOKernel, global_update = true; src/kernels.jl:38-125):
+ * per iteration, particles are processed in blocks of `block`; each block
+ * steps against the live global best, queues the particles whose best_cost
+ * beats it, and when at least `min_improvers` queued (the reference's
+ * `queue_num[1] > 1`) publishes the queue minimum if it still beats the global
+ * best.  Blocks run one after another in index order; the queue is in particle
+ * order and its minimum keeps the first of equal costs. */
+void ora_qlock_run(const ora_problem* P, int64_t n, uint64_t seed, uint32_t epoch, uint32_t t0,
+                   int32_t iters, float w, float wdamp, float c1, float c2, int32_t block,
+                   int32_t min_improvers, float* parts, float* gb_pos, float* gb_cost,
+                   int64_t* gb_idx) {
+  int d = P->d;
+  int64_t s = stride_of(d);
+  uint32_t key[2];
+  seed_key(seed, key);
+  for (int32_t k = 1; k <= iters; ++k) {
+    uint32_t t = t0 + (uint32_t)k;
+    for (int64_t b0 = 0; b0 < n; b0 += block) {
+      int64_t b1 = b0 + block < n ? b0 + block : n;
+      float gb[ORA_MAXD];
+      memcpy(gb, gb_pos, sizeof(float) * d);
+      for (int64_t i = b0; i < b1; ++i)
+        step_particle(parts + i * s, P, gb, w, c1, c2, (uint32_t)i, t, epoch, key);
+      int64_t count = 0, best = -1;
+      for (int64_t i = b0; i < b1; ++i) {
+        float bc = BCOST(parts + i * s, d);
+        if (bc < *gb_cost) {
+          ++count;
+          if (best < 0 || bc < BCOST(parts + best * s, d)) best = i;
+        }
+      }
+      if (count >= min_improvers && BCOST(parts + best * s, d) < *gb_cost) {
+        memcpy(gb_pos, BPOS(parts + best * s, d), sizeof(float) * d);
+        *gb_cost = BCOST(parts + best * s, d);
+        *gb_idx = best;
+      }
+    }
+    w = w * wdamp;
+  }
+}
+
+/* Evaluate the objective over AoS positions (n x d). */
+void ora_eval(const ora_problem* P, int64_t n, const float* x, float* f) {
+  for (int64_t i = 0; i < n; ++i) f[i] = ora_objective(P, x + i * P->d);
+}
+
+/* One velocity/position move without evaluation (the ODE path's
+ * ode_update_particle_states!, src/ode_pso.jl:1-24), exposed for unit tests. */
+void ora_move(const ora_problem* P, float* particle, const float* gb, float w, float c1,
+              float c2, uint32_t i, uint32_t t, uint32_t epoch, uint64_t seed) {
+  uint32_t key[2];
+  seed_key(seed, key);
+  move_particle(particle, P, gb, w, c1, c2, i, t, epoch, key);
+}
+
+int ora_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

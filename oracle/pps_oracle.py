@@ -1,0 +1,243 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/libpps_oracle.so).
+
+TEST INFRASTRUCTURE ONLY.  The oracle restates ParallelParticleSwarms.jl's
+particle-swarm path on the CPU (see pps_oracle.c for the file:line map); the
+parity tests, the smoke check and the bench's CPU-baseline leg use it as the
+checker.  The product never imports this module.
+
+Particle state crosses this wrapper in the kernels' SoA convention
+(x, v, best: d x n float32; best_cost: n float32) and is converted to and from
+the oracle's AoS SPSOParticle layout here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libpps_oracle.so")
+
+MAXD = 64
+MAXPAR = 8
+
+ROSENBROCK, RASTRIGIN, SPHERE, BEALE, BOOTH, HIMMELBLAU, QUADRATIC, BOUNDARY_QUADRATIC, ROSEN2 = (
+    1, 2, 3, 4, 5, 6, 7, 8, 9)
+LOTKA_VOLTERRA = 16
+GBEST_PBEST, GBEST_CURRENT = 0, 1
+CLAMP_COMPONENT, CLAMP_LEXICOGRAPHIC = 0, 1
+
+
+class _Prob(C.Structure):
+    _fields_ = [
+        ("obj", C.c_int32), ("d", C.c_int32), ("has_shift", C.c_int32), ("clamp", C.c_int32),
+        ("lb", C.c_float * MAXD), ("ub", C.c_float * MAXD), ("shift", C.c_float * MAXD),
+        ("par", C.c_float * MAXPAR), ("obs", C.POINTER(C.c_float)), ("n_obs", C.c_int32),
+        ("steps_per_obs", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(LIB_PATH)
+        fp = C.POINTER(C.c_float)
+        i64p = C.POINTER(C.c_int64)
+        L.ora_philox.argtypes = [C.POINTER(C.c_uint32)] * 3
+        L.ora_cos2pi.argtypes = [C.c_float]
+        L.ora_cos2pi.restype = C.c_float
+        L.ora_sin2pi.argtypes = [C.c_float]
+        L.ora_sin2pi.restype = C.c_float
+        L.ora_objective.argtypes = [C.POINTER(_Prob), fp]
+        L.ora_objective.restype = C.c_float
+        L.ora_lv_sse.argtypes = [fp, C.POINTER(_Prob)]
+        L.ora_lv_sse.restype = C.c_float
+        L.ora_eval.argtypes = [C.POINTER(_Prob), C.c_int64, fp, fp]
+        L.ora_argmin.argtypes = [fp, C.c_int64, C.c_int]
+        L.ora_argmin.restype = C.c_int64
+        L.ora_init.argtypes = [C.POINTER(_Prob), C.c_int64, C.c_uint64, C.c_uint32, fp, fp, fp,
+                               i64p, C.c_int]
+        L.ora_sync_run.argtypes = [C.POINTER(_Prob), C.c_int64, C.c_uint64, C.c_uint32,
+                                   C.c_uint32, C.c_int32, C.c_float, C.c_float, C.c_float,
+                                   C.c_float, C.c_int32, fp, fp, fp, i64p, C.c_int]
+        L.ora_async_run.argtypes = [C.POINTER(_Prob), C.c_int64, C.c_uint64, C.c_uint32,
+                                    C.c_uint32, C.c_int32, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, fp, fp, fp, i64p, C.c_int]
+        L.ora_qlock_run.argtypes = [C.POINTER(_Prob), C.c_int64, C.c_uint64, C.c_uint32,
+                                    C.c_uint32, C.c_int32, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, C.c_int32, C.c_int32, fp, fp, fp, i64p]
+        L.ora_move.argtypes = [C.POINTER(_Prob), fp, fp, C.c_float, C.c_float, C.c_float,
+                               C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]
+        L.ora_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+@dataclass
+class Problem:
+    """Objective + bounds, mirroring the product's pps_problem."""
+
+    obj: int
+    d: int
+    lb: np.ndarray
+    ub: np.ndarray
+    params: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
+    shift: np.ndarray | None = None
+    clamp: int = CLAMP_COMPONENT
+    obs: np.ndarray | None = None  # Lotka-Volterra observations, (n_obs, 2)
+
+    def c(self) -> _Prob:
+        p = _Prob()
+        p.obj, p.d, p.clamp = self.obj, self.d, self.clamp
+        lb = np.asarray(self.lb, np.float32).reshape(-1)
+        ub = np.asarray(self.ub, np.float32).reshape(-1)
+        for j in range(self.d):
+            p.lb[j], p.ub[j] = float(lb[j]), float(ub[j])
+        par = np.asarray(self.params, np.float32).reshape(-1)
+        for k in range(len(par)):
+            p.par[k] = float(par[k])
+        if self.shift is not None:
+            p.has_shift = 1
+            sh = np.asarray(self.shift, np.float32).reshape(-1)
+            for j in range(self.d):
+                p.shift[j] = float(sh[j])
+        if self.obs is not None:
+            self._obs_keep = np.ascontiguousarray(self.obs, np.float32).reshape(-1)
+            p.obs = _fp(self._obs_keep)
+            p.n_obs = len(self._obs_keep) // 2
+            p.steps_per_obs = int(par[3])
+        return p
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+    k = (C.c_uint32 * 2)(*[int(v) & 0xFFFFFFFF for v in key])
+    o = (C.c_uint32 * 4)()
+    lib().ora_philox(c, k, o)
+    return [o[i] for i in range(4)]
+
+
+def cos2pi(x: float) -> float:
+    return float(lib().ora_cos2pi(float(x)))
+
+
+def sin2pi(x: float) -> float:
+    return float(lib().ora_sin2pi(float(x)))
+
+
+def objective(prob: Problem, x) -> np.float32:
+    x = np.ascontiguousarray(x, np.float32)
+    p = prob.c()
+    return np.float32(lib().ora_objective(C.byref(p), _fp(x)))
+
+
+def evaluate(prob: Problem, xs) -> np.ndarray:
+    xs = np.ascontiguousarray(xs, np.float32).reshape(-1, prob.d)
+    out = np.zeros(len(xs), np.float32)
+    p = prob.c()
+    lib().ora_eval(C.byref(p), len(xs), _fp(xs), _fp(out))
+    return out
+
+
+class Swarm:
+    """Oracle swarm in AoS SPSOParticle layout ((3d+2) float32 per particle)."""
+
+    def __init__(self, prob: Problem, n: int, seed: int, nthreads: int = 0):
+        self.prob, self.n, self.seed = prob, int(n), int(seed)
+        self.d = prob.d
+        self.nthreads = nthreads or lib().ora_num_threads()
+        self.parts = np.zeros((self.n, 3 * self.d + 2), np.float32)
+        self.gb_pos = np.zeros(MAXD, np.float32)
+        self.gb_cost = C.c_float(0.0)
+        self.gb_idx = C.c_int64(0)
+        self.epoch = 0
+        self.iter = 0
+        self._p = prob.c()
+        self.init()
+
+    # -- views in the kernels' SoA convention
+    @property
+    def x(self):
+        return self.parts[:, : self.d].T.copy()
+
+    @property
+    def v(self):
+        return self.parts[:, self.d: 2 * self.d].T.copy()
+
+    @property
+    def cost(self):
+        return self.parts[:, 2 * self.d].copy()
+
+    @property
+    def best(self):
+        return self.parts[:, 2 * self.d + 1: 3 * self.d + 1].T.copy()
+
+    @property
+    def best_cost(self):
+        return self.parts[:, 3 * self.d + 1].copy()
+
+    def set_state(self, x, v, best, best_cost, it: int):
+        d = self.d
+        self.parts[:, :d] = np.asarray(x, np.float32).T
+        self.parts[:, d: 2 * d] = np.asarray(v, np.float32).T
+        self.parts[:, 2 * d + 1: 3 * d + 1] = np.asarray(best, np.float32).T
+        self.parts[:, 3 * d + 1] = np.asarray(best_cost, np.float32)
+        self.iter = int(it)
+
+    def gbest(self):
+        return self.gb_pos[: self.d].copy(), np.float32(self.gb_cost.value), int(self.gb_idx.value)
+
+    def set_gbest(self, pos, cost):
+        self.gb_pos[: self.d] = np.asarray(pos, np.float32)
+        self.gb_cost.value = float(cost)
+
+    def init(self):
+        lib().ora_init(C.byref(self._p), self.n, self.seed, self.epoch, _fp(self.parts),
+                       _fp(self.gb_pos), C.byref(self.gb_cost), C.byref(self.gb_idx),
+                       self.nthreads)
+        self.iter = 0
+
+    def reinit(self):
+        self.epoch += 1
+        self.init()
+
+    def run_sync(self, iters, w=0.7298, wdamp=1.0, c1=1.4962, c2=1.4962, gbest_from=GBEST_PBEST):
+        lib().ora_sync_run(C.byref(self._p), self.n, self.seed, self.epoch, self.iter, int(iters),
+                           w, wdamp, c1, c2, gbest_from, _fp(self.parts), _fp(self.gb_pos),
+                           C.byref(self.gb_cost), C.byref(self.gb_idx), self.nthreads)
+        self.iter += int(iters)
+
+    def run_async(self, iters, w=0.7298, wdamp=1.0, c1=1.4962, c2=1.4962):
+        lib().ora_async_run(C.byref(self._p), self.n, self.seed, self.epoch, self.iter,
+                            int(iters), w, wdamp, c1, c2, _fp(self.parts), _fp(self.gb_pos),
+                            C.byref(self.gb_cost), C.byref(self.gb_idx), self.nthreads)
+        self.iter += int(iters)
+
+    def run_qlock(self, iters, w=0.7298, wdamp=1.0, c1=1.4962, c2=1.4962, block=1024,
+                  min_improvers=2):
+        lib().ora_qlock_run(C.byref(self._p), self.n, self.seed, self.epoch, self.iter,
+                            int(iters), w, wdamp, c1, c2, block, min_improvers,
+                            _fp(self.parts), _fp(self.gb_pos), C.byref(self.gb_cost),
+                            C.byref(self.gb_idx))
+        self.iter += int(iters)
+
+
+def move(prob: Problem, particle, gb, w, c1, c2, i, t, epoch, seed):
+    """One velocity/position move of a single AoS particle (no evaluation)."""
+    p = np.ascontiguousarray(particle, np.float32).copy()
+    g = np.zeros(MAXD, np.float32)
+    g[: prob.d] = np.asarray(gb, np.float32)
+    pc = prob.c()
+    lib().ora_move(C.byref(pc), _fp(p), _fp(g), w, c1, c2, i, t, epoch, seed)
+    return p

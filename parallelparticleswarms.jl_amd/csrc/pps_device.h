@@ -1,0 +1,348 @@
+// pps_device.h — device-side arithmetic shared by every libpps kernel.
+//
+// Everything here is FP32 and written so that each operation happens in a
+// fixed order with no contraction (the library is compiled with
+// -ffp-contract=off): a particle update is then a pure function of its inputs,
+// reproducible bit for bit by any IEEE-754 FP32 implementation that follows
+// the same order.  The order follows the reference's Julia expressions
+// (src/kernels.jl:1-26), which Julia evaluates left to right without FMA.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pps_shared.h"
+
+namespace pps {
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+// as 1, 2, 3", SC'11).  Ten rounds of the 4x32 S-box/P-box with the published
+// multipliers and Weyl key increments.  The key is per-swarm (uniform across a
+// wavefront), so the key schedule lives in scalar registers.
+// ---------------------------------------------------------------------------
+struct u4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)PHILOX_M0 * c.x;
+    const uint64_t p1 = (uint64_t)PHILOX_M1 * c.z;
+    u4 n;
+    n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+    n.y = (uint32_t)p1;
+    n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+    n.w = (uint32_t)p0;
+    c = n;
+    k0 += PHILOX_W0;
+    k1 += PHILOX_W1;
+  }
+  return c;
+}
+
+// Uniform in [0, 1): the top 24 bits scaled by 2^-24 (exact in FP32).
+__device__ __forceinline__ float u01(uint32_t u) {
+  return (float)(u >> 8) * 0x1.0p-24f;
+}
+
+// ---------------------------------------------------------------------------
+// Julia's Base.max / Base.min for IEEE floats: NaN-propagating, and
+// max(-0.0, +0.0) = +0.0, min(-0.0, +0.0) = -0.0.  AMD v_max_f32 / v_min_f32
+// implement maxNum/minNum instead, so the selection is written out.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float jl_max(float x, float y) {
+  const float diff = x - y;
+  const float pick = __builtin_signbit(diff) ? y : x;
+  return (__builtin_isnan(x) | __builtin_isnan(y)) ? diff : pick;
+}
+__device__ __forceinline__ float jl_min(float x, float y) {
+  const float diff = x - y;
+  const float pick = __builtin_signbit(diff) ? x : y;
+  return (__builtin_isnan(x) | __builtin_isnan(y)) ? diff : pick;
+}
+
+// Julia's isless for Float32 (total order: -0.0 < +0.0, NaN above +Inf).
+__device__ __forceinline__ bool jl_isless(float a, float b) {
+  const bool an = __builtin_isnan(a), bn = __builtin_isnan(b);
+  if (an | bn) return !an & bn;
+  // Signed-integer image of the bits: negative floats have their magnitude
+  // bits flipped, so -0.0 maps to -1 and +0.0 to 0.
+  int32_t sa = (int32_t)__float_as_uint(a), sb = (int32_t)__float_as_uint(b);
+  if (sa < 0) sa ^= 0x7FFFFFFF;
+  if (sb < 0) sb ^= 0x7FFFFFFF;
+  return sa < sb;
+}
+
+// ---------------------------------------------------------------------------
+// Argmin keys.  key = order(cost) << 32 | particle index.  order() is monotone
+// in the FP32 value with +0 == -0 (the reference compares costs with `<`), and
+// NaN maps above +Inf so a NaN cost never wins.  The smallest key is the
+// lowest-index particle among those with the smallest cost.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t cost_order(float c) {
+  if (__builtin_isnan(c)) return 0xFFFFFFFFu;
+  uint32_t u = __float_as_uint(c);
+  if ((u & 0x7FFFFFFFu) == 0u) u = 0u;
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ uint64_t make_key(float cost, uint32_t idx) {
+  return ((uint64_t)cost_order(cost) << 32) | (uint64_t)idx;
+}
+__device__ __forceinline__ float key_cost(uint64_t key) {
+  const uint32_t o = (uint32_t)(key >> 32);
+  if (o == 0xFFFFFFFFu) return __uint_as_float(0x7FC00000u);
+  const uint32_t u = (o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o;
+  return __uint_as_float(u);
+}
+
+__device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) {
+  return a < b ? a : b;
+}
+
+// Minimum over the 64 lanes of a wavefront (every lane receives it).
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t k) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const uint64_t o = __shfl_xor(k, off, 64);
+    k = umin64(k, o);
+  }
+  return k;
+}
+
+// ---------------------------------------------------------------------------
+// cos(2 pi x), the same function on every device and in every build:
+//   r = x - rint(x)          exact, r in [-1/2, 1/2]
+//   a = |r|; if a > 1/4: a = 1/2 - a (exact), negate
+//   cos(2 pi a) = sum_{k=0}^{7} (-1)^k (2 pi)^{2k} a^{2k} / (2k)!
+// The coefficients are the Taylor coefficients of cos(2 pi a) rounded to FP32;
+// the truncation error on [0, 1/4] is below (pi/2)^16/16! ~ 7e-11.  Horner in
+// s = a^2 with explicit FMAs.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float cos2pi(float x) {
+  const float r = x - __builtin_rintf(x);
+  float a = __builtin_fabsf(r);
+  const bool flip = a > 0.25f;
+  a = flip ? (0.5f - a) : a;
+  const float s = a * a;
+  float p = COS2PI_C7;
+  p = __builtin_fmaf(p, s, COS2PI_C6);
+  p = __builtin_fmaf(p, s, COS2PI_C5);
+  p = __builtin_fmaf(p, s, COS2PI_C4);
+  p = __builtin_fmaf(p, s, COS2PI_C3);
+  p = __builtin_fmaf(p, s, COS2PI_C2);
+  p = __builtin_fmaf(p, s, COS2PI_C1);
+  p = __builtin_fmaf(p, s, 1.0f);
+  return flip ? -p : p;
+}
+// sin(2 pi x) = cos(2 pi (x - 1/4)); used by the Rastrigin gradient.
+__device__ __forceinline__ float sin2pi(float x) {
+  const float r = x - __builtin_rintf(x);  // exact
+  return cos2pi(r - 0.25f);
+}
+
+// Lotka-Volterra right-hand side, in the operation order of the reference's
+// benchmark model (benchmarks/Lotka_Volterra/lotka_volterra.jl:9-13):
+//   du1 = p1*u1 - p2*u1*u2 ;  du2 = -p3*u2 + p4*u1*u2
+__device__ __forceinline__ void lv_rhs(float u1, float u2, float p1, float p2,
+                                       float p3, float p4, float& f1,
+                                       float& f2) {
+  f1 = p1 * u1 - (p2 * u1) * u2;
+  f2 = (-p3) * u2 + (p4 * u1) * u2;
+}
+
+// Sum of squared errors of a fixed-step RK4 trajectory (dt = par[2], sampled
+// every steps_per_obs steps, starting at u0 = par[0..1] at sample 0) against
+// the observations; the per-sample error is e1^2 + e2^2 and samples are added
+// in time order (reference src/ode_pso.jl:47-52).
+__device__ __forceinline__ float lv_sse(const float* th, const ProbArgs& P) {
+  const float p1 = th[0], p2 = th[1], p3 = th[2], p4 = th[3];
+  float u1 = P.par[0], u2 = P.par[1];
+  const float dt = P.par[2];
+  const float h2 = dt * 0.5f;
+  const float h6 = dt / 6.0f;
+  float loss = 0.0f;
+  const float* obs = P.obs;
+  for (int k = 0; k < P.n_obs; ++k) {
+    if (k > 0) {
+      for (int s = 0; s < P.steps_per_obs; ++s) {
+        float k1a, k1b, k2a, k2b, k3a, k3b, k4a, k4b;
+        lv_rhs(u1, u2, p1, p2, p3, p4, k1a, k1b);
+        lv_rhs(u1 + h2 * k1a, u2 + h2 * k1b, p1, p2, p3, p4, k2a, k2b);
+        lv_rhs(u1 + h2 * k2a, u2 + h2 * k2b, p1, p2, p3, p4, k3a, k3b);
+        lv_rhs(u1 + dt * k3a, u2 + dt * k3b, p1, p2, p3, p4, k4a, k4b);
+        u1 = u1 + h6 * (((k1a + 2.0f * k2a) + 2.0f * k3a) + k4a);
+        u2 = u2 + h6 * (((k1b + 2.0f * k2b) + 2.0f * k3b) + k4b);
+      }
+    }
+    const float e1 = obs[2 * k] - u1;
+    const float e2 = obs[2 * k + 1] - u2;
+    loss = loss + (e1 * e1 + e2 * e2);
+  }
+  return loss;
+}
+
+// Objective f(x - shift) over a register array (compile-time capacity MAXD,
+// runtime dimension d <= MAXD).  OBJ is a compile-time PPS_OBJ_* constant.
+template <int MAXD, int OBJ>
+__device__ __forceinline__ float objective(const float (&xin)[MAXD], const int d,
+                                           const ProbArgs& P) {
+  float z[MAXD];
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) z[j] = P.has_shift ? (xin[j] - P.shift[j]) : xin[j];
+  if constexpr (OBJ == PPS_OBJ_ROSENBROCK) {
+    // res += p[2] * (x[i+1] - x[i]^2)^2 + (p[1] - x[i])^2   (test/gpu.jl:14-20)
+    const float p1 = P.par[0], p2 = P.par[1];
+    float res = 0.0f;
+#pragma unroll
+    for (int i = 0; i + 1 < MAXD; ++i) {
+      if (i + 1 < d) {
+        const float dd = z[i + 1] - z[i] * z[i];
+        const float e = p1 - z[i];
+        res = res + (p2 * (dd * dd) + e * e);
+      }
+    }
+    return res;
+  } else if constexpr (OBJ == PPS_OBJ_RASTRIGIN) {
+    float res = 10.0f * (float)d;
+#pragma unroll
+    for (int i = 0; i < MAXD; ++i) {
+      if (i < d) res = res + (z[i] * z[i] - 10.0f * cos2pi(z[i]));
+    }
+    return res;
+  } else if constexpr (OBJ == PPS_OBJ_SPHERE) {
+    float res = 0.0f;
+#pragma unroll
+    for (int i = 0; i < MAXD; ++i) {
+      if (i < d) res = res + z[i] * z[i];
+    }
+    return res;
+  } else if constexpr (OBJ == PPS_OBJ_BEALE) {
+    const float x1 = z[0], x2 = z[MAXD > 1 ? 1 : 0];
+    const float a = (1.5f - x1) + x1 * x2;
+    const float b = (2.25f - x1) + x1 * (x2 * x2);
+    const float c = (2.625f - x1) + x1 * ((x2 * x2) * x2);
+    return (a * a + b * b) + c * c;
+  } else if constexpr (OBJ == PPS_OBJ_BOOTH) {
+    const float x1 = z[0], x2 = z[MAXD > 1 ? 1 : 0];
+    const float a = (x1 + 2.0f * x2) - 7.0f;
+    const float b = (2.0f * x1 + x2) - 5.0f;
+    return a * a + b * b;
+  } else if constexpr (OBJ == PPS_OBJ_HIMMELBLAU) {
+    const float x1 = z[0], x2 = z[MAXD > 1 ? 1 : 0];
+    const float a = (x1 * x1 + x2) - 11.0f;
+    const float b = (x1 + x2 * x2) - 7.0f;
+    return a * a + b * b;
+  } else if constexpr (OBJ == PPS_OBJ_QUADRATIC) {
+    const float x1 = z[0], x2 = z[MAXD > 1 ? 1 : 0];
+    const float a = x1 - 1.0f, b = x2 + 2.0f;
+    return P.par[0] * (a * a) + b * b;
+  } else if constexpr (OBJ == PPS_OBJ_BOUNDARY_QUADRATIC) {
+    const float a = z[0] + 1.0f;
+    return a * a;
+  } else if constexpr (OBJ == PPS_OBJ_ROSEN2) {
+    const float x1 = z[0], x2 = z[MAXD > 1 ? 1 : 0];
+    const float a = P.par[0] - x1;
+    const float b = x2 - x1 * x1;
+    return a * a + P.par[1] * (b * b);
+  } else if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA) {
+    return lv_sse(z, P);
+  } else {
+    return __uint_as_float(0x7FC00000u);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The per-particle PSO step (reference update_particle_state,
+// src/kernels.jl:1-26), component j, with r1/r2 drawn from the counter-based
+// stream:
+//   v' = (w*v + (c1*r1)*(pbest - x)) + (c2*r2)*(gbest - x)
+//   x' = clamp(x + v')
+// Random stream: for particle i at iteration t (1-based) of epoch e, call
+// philox(ctr = (i, t, k | TAG_STEP << 16, e), key = seed) for k = 0..ceil(d/2)-1
+// and use lanes (x, y, z, w) as (r1[2k], r2[2k], r1[2k+1], r2[2k+1]).
+// ---------------------------------------------------------------------------
+template <int MAXD>
+__device__ __forceinline__ void velocity_position(
+    float (&x)[MAXD], float (&v)[MAXD], const float (&pb)[MAXD],
+    const float* gb, float w, float c1, float c2, uint32_t i, uint32_t t,
+    uint32_t epoch, uint32_t k0, uint32_t k1, const int d, const ProbArgs& P) {
+#pragma unroll
+  for (int k = 0; 2 * k < MAXD; ++k) {
+    if (2 * k < d) {
+      const u4 r = philox4x32_10(u4{i, t, (uint32_t)k | (PPS_TAG_STEP << 16), epoch}, k0, k1);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int j = 2 * k + h;
+        if (j < MAXD && j < d) {
+          const float r1 = u01(h == 0 ? r.x : r.z);
+          const float r2 = u01(h == 0 ? r.y : r.w);
+          const float a = w * v[j];
+          const float b = (c1 * r1) * (pb[j] - x[j]);
+          const float c = (c2 * r2) * (gb[j] - x[j]);
+          v[j] = (a + b) + c;
+          x[j] = x[j] + v[j];
+        }
+      }
+    }
+  }
+  if (P.clamp == PPS_CLAMP_COMPONENT) {
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j) {
+      if (j < d) x[j] = jl_min(jl_max(x[j], P.lb[j]), P.ub[j]);
+    }
+  } else {
+    // Whole-vector max/min under Julia's lexicographic isless
+    // (src/ode_pso.jl:18-19): max(x, lb) = isless(lb, x) ? x : lb.
+    int c = 0;  // cmp(lb, x): -1, 0, 1
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j) {
+      if (j < d && c == 0) {
+        const float a = P.lb[j], b = x[j];
+        const bool eq = (__float_as_uint(a) == __float_as_uint(b)) ||
+                        (__builtin_isnan(a) & __builtin_isnan(b));
+        if (!eq) c = jl_isless(a, b) ? -1 : 1;
+      }
+    }
+    const bool take_lb = !(c < 0);
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j) if (j < d && take_lb) x[j] = P.lb[j];
+    c = 0;  // cmp(ub, x)
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j) {
+      if (j < d && c == 0) {
+        const float a = P.ub[j], b = x[j];
+        const bool eq = (__float_as_uint(a) == __float_as_uint(b)) ||
+                        (__builtin_isnan(a) & __builtin_isnan(b));
+        if (!eq) c = jl_isless(a, b) ? -1 : 1;
+      }
+    }
+    const bool take_ub = (c < 0);
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j) if (j < d && take_ub) x[j] = P.ub[j];
+  }
+}
+
+// Uniform initial position (reference src/utils.jl:1-5,57-84):
+// x_j = r * (ub_j - lb_j) + lb_j with r from philox(ctr = (i, 0,
+// m | TAG_INIT << 16, e)) lanes (x, y, z, w) for dimensions 4m..4m+3.
+template <int MAXD>
+__device__ __forceinline__ void init_position(float (&x)[MAXD], uint32_t i,
+                                              uint32_t epoch, uint32_t k0,
+                                              uint32_t k1, const int d,
+                                              const ProbArgs& P) {
+#pragma unroll
+  for (int m = 0; 4 * m < MAXD; ++m) {
+    if (4 * m < d) {
+      const u4 r = philox4x32_10(u4{i, 0u, (uint32_t)m | (PPS_TAG_INIT << 16), epoch}, k0, k1);
+      const uint32_t rr[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int j = 4 * m + h;
+        if (j < MAXD && j < d) x[j] = u01(rr[h]) * (P.ub[j] - P.lb[j]) + P.lb[j];
+      }
+    }
+  }
+}
+
+}  // namespace pps

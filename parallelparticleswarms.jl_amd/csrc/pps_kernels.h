@@ -1,0 +1,51 @@
+// pps_kernels.h — host-side launch interface of the libpps kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pps_shared.h"
+
+namespace pps {
+
+struct StepArgs {
+  float c1, c2;
+  int32_t gbest_from;  // PPS_GBEST_*
+  int32_t min_improvers;
+};
+
+// One set of launchers per (objective, dimension bucket).
+struct KernelSet {
+  int maxd;
+  // Uniform init of every particle, then block keys/candidates for parity
+  // `par` (G workgroups of `block` threads).
+  hipError_t (*init)(const SwarmDev&, const ProbArgs&, uint32_t par, hipStream_t);
+  // Recompute block keys/candidates from the current state (no step).
+  hipError_t (*rekey)(const SwarmDev&, const ProbArgs&, uint32_t par, int32_t gbest_from,
+                      hipStream_t);
+  // One sync iteration t = *tbase + k (reads parity (t-1)&1, writes t&1).
+  hipError_t (*sync_step)(const SwarmDev&, const ProbArgs&, const StepArgs&, float w,
+                          uint32_t k, hipStream_t);
+  // Sync iterations t0+1 .. t0+T in one workgroup with the swarm in registers
+  // (n <= 1024, G == 1).
+  hipError_t (*sync_wg)(const SwarmDev&, const ProbArgs&, const StepArgs&, float w0,
+                        float wdamp, uint32_t t0, int32_t T, hipStream_t);
+  // Async: every particle runs T iterations against its running best.
+  hipError_t (*async_run)(const SwarmDev&, const ProbArgs&, const StepArgs&, float w0,
+                          float wdamp, uint32_t t0, int32_t T, hipStream_t);
+  // Queue-lock: one iteration t = *tbase + k against the live global best
+  // (workgroups of 1024 particles).
+  hipError_t (*qlock_step)(const SwarmDev&, const ProbArgs&, const StepArgs&, float w,
+                           uint32_t k, hipStream_t);
+};
+
+// nullptr when (obj, d) has no compiled kernels.
+const KernelSet* find_kernels(int obj, int d);
+
+// Reduce the block keys of parity `par` into swarm.gbrec (cost, idx, pos).
+hipError_t launch_finalize(const SwarmDev&, int d, uint32_t par, hipStream_t);
+// *p = v on the stream (a one-lane kernel, so the write can sit in a graph).
+hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t);
+// Write gbrec and a single-candidate parity `par` from host values.
+hipError_t launch_set_gbest(const SwarmDev&, int d, uint32_t par, const float* pos_dev,
+                            float cost, hipStream_t);
+
+}  // namespace pps

@@ -1,0 +1,457 @@
+#pragma once
+// pps_kernels_impl.h — particle-swarm kernel templates for gfx950 (included by
+// one translation unit per objective family).
+//
+// Layout: swarm state is structure-of-arrays in HBM (x, v, best position:
+// d rows of npad floats; best cost: npad floats), so a wavefront's load of
+// coordinate j of 64 consecutive particles is one 256-byte coalesced access.
+// One work-item steps one particle per tile; workgroup b owns tiles b, b+G, ...
+//
+// Kernels (reference counterparts in brackets):
+//   k_init      uniform initialisation + block argmin   [gpu_init_particles!,
+//               src/utils.jl:57-84; minimum, src/init.jl:49-51]
+//   k_sync_step fused velocity/position/clamp/objective/pbest + block argmin
+//               [update_particle_states! sync, src/kernels.jl:127-194 and the
+//               per-iteration minimum of src/lowerlevel_solve.jl:23-33]
+//   k_sync_wg   the whole sync loop in one workgroup, swarm in registers
+//   k_async     all iterations per particle in registers
+//               [update_particles_states_async!, src/kernels.jl:196-225]
+//   k_qlock     one iteration against the live global best, lock-protected
+//               publication [src/kernels.jl:38-125]
+//   k_finalize  grid pass over the block keys -> global best record
+#include <hip/hip_runtime.h>
+
+#include "pps_device.h"
+#include "pps_kernels.h"
+
+namespace pps {
+
+constexpr uint64_t KEY_NONE = ~0ull;
+
+// Minimum over a workgroup.  Every thread receives the result.
+template <int BLOCK>
+__device__ __forceinline__ uint64_t block_min_u64(uint64_t k, uint64_t* s_red) {
+  k = wave_min_u64(k);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (BLOCK == 64) return k;
+  if (lane == 0) s_red[wid] = k;
+  __syncthreads();
+  uint64_t r = s_red[0];
+#pragma unroll
+  for (int i = 1; i < BLOCK / 64; ++i) r = umin64(r, s_red[i]);
+  __syncthreads();
+  return r;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ int block_sum_i32(int c, int* s_cnt) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (BLOCK == 64) return c;
+  if (lane == 0) s_cnt[wid] = c;
+  __syncthreads();
+  int r = 0;
+#pragma unroll
+  for (int i = 0; i < BLOCK / 64; ++i) r += s_cnt[i];
+  __syncthreads();
+  return r;
+}
+
+template <int MAXD>
+__device__ __forceinline__ void load_vec(float (&r)[MAXD], const float* base, int64_t npad,
+                                         int64_t i, int d) {
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j)
+    if (j < d) r[j] = base[(int64_t)j * npad + i];
+}
+template <int MAXD>
+__device__ __forceinline__ void store_vec(const float (&r)[MAXD], float* base, int64_t npad,
+                                          int64_t i, int d) {
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j)
+    if (j < d) base[(int64_t)j * npad + i] = r[j];
+}
+
+// Global best of parity `par`: minimum block key, then the owning block's
+// candidate position into s_gb (all threads must call).
+template <int BLOCK>
+__device__ __forceinline__ uint64_t fetch_gbest(const SwarmDev& S, uint32_t par, int d,
+                                                float* s_gb, uint64_t* s_red) {
+  const int G = S.G;
+  uint64_t k = KEY_NONE;
+  for (int j = threadIdx.x; j < G; j += BLOCK) k = umin64(k, S.bkey[(int64_t)par * G + j]);
+  k = block_min_u64<BLOCK>(k, s_red);
+  const uint32_t gidx = (uint32_t)k;
+  const int gblk = (int)((gidx / (uint32_t)S.block) % (uint32_t)G);
+  if ((int)threadIdx.x < d) s_gb[threadIdx.x] = S.cand[((int64_t)par * G + gblk) * d + threadIdx.x];
+  __syncthreads();
+  return k;
+}
+
+// Block argmin over `mykey`; the owning thread copies the winner's position
+// (already written by this same thread) into cand[par][b].
+template <int BLOCK>
+__device__ __forceinline__ void publish_block(const SwarmDev& S, uint32_t par, int d,
+                                              uint64_t mykey, const float* src,
+                                              uint64_t* s_red) {
+  const uint64_t bk = block_min_u64<BLOCK>(mykey, s_red);
+  const int b = blockIdx.x;
+  if (bk != KEY_NONE && mykey == bk) {
+    const int64_t wi = (int64_t)(uint32_t)bk;
+    for (int j = 0; j < d; ++j)
+      S.cand[((int64_t)par * S.G + b) * d + j] = src[(int64_t)j * S.npad + wi];
+  }
+  if (threadIdx.x == 0) S.bkey[(int64_t)par * S.G + b] = bk;
+}
+
+// ---------------------------------------------------------------------------
+template <int MAXD, bool EXACT, int OBJ, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_init(SwarmDev S, ProbArgs P, uint32_t par) {
+  __shared__ uint64_t s_red[BLOCK / 64];
+  const int d = EXACT ? MAXD : P.d;
+  uint64_t mykey = KEY_NONE;
+  for (int64_t tile = blockIdx.x; tile * BLOCK < S.n; tile += gridDim.x) {
+    const int64_t i = tile * BLOCK + threadIdx.x;
+    if (i < S.n) {
+      float x[MAXD], z[MAXD];
+      init_position<MAXD>(x, (uint32_t)i, S.epoch, S.key0, S.key1, d, P);
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j) z[j] = 0.0f;
+      const float c = objective<MAXD, OBJ>(x, d, P);
+      store_vec<MAXD>(x, S.x, S.npad, i, d);
+      store_vec<MAXD>(z, S.v, S.npad, i, d);
+      store_vec<MAXD>(x, S.pb, S.npad, i, d);
+      S.pc[i] = c;
+      mykey = umin64(mykey, make_key(c, (uint32_t)i));
+    }
+  }
+  publish_block<BLOCK>(S, par, d, mykey, S.pb, s_red);
+}
+
+template <int MAXD, bool EXACT, int OBJ, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_rekey(SwarmDev S, ProbArgs P, uint32_t par,
+                                                 int32_t gbest_from) {
+  __shared__ uint64_t s_red[BLOCK / 64];
+  const int d = EXACT ? MAXD : P.d;
+  uint64_t mykey = KEY_NONE;
+  for (int64_t tile = blockIdx.x; tile * BLOCK < S.n; tile += gridDim.x) {
+    const int64_t i = tile * BLOCK + threadIdx.x;
+    if (i < S.n) mykey = umin64(mykey, make_key(S.pc[i], (uint32_t)i));
+  }
+  publish_block<BLOCK>(S, par, d, mykey, gbest_from == PPS_GBEST_CURRENT ? S.x : S.pb, s_red);
+}
+
+template <int MAXD, bool EXACT, int OBJ, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sync_step(SwarmDev S, ProbArgs P, StepArgs A,
+                                                     float w, uint32_t k) {
+  __shared__ uint64_t s_red[BLOCK / 64];
+  const uint32_t t = *S.tbase + k;
+  __shared__ float s_gb[MAXD];
+  const int d = EXACT ? MAXD : P.d;
+  fetch_gbest<BLOCK>(S, (t - 1u) & 1u, d, s_gb, s_red);
+  float gb[MAXD];
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
+  uint64_t mykey = KEY_NONE;
+  for (int64_t tile = blockIdx.x; tile * BLOCK < S.n; tile += gridDim.x) {
+    const int64_t i = tile * BLOCK + threadIdx.x;
+    if (i < S.n) {
+      float x[MAXD], v[MAXD], pb[MAXD];
+      load_vec<MAXD>(x, S.x, S.npad, i, d);
+      load_vec<MAXD>(v, S.v, S.npad, i, d);
+      load_vec<MAXD>(pb, S.pb, S.npad, i, d);
+      float pc = S.pc[i];
+      velocity_position<MAXD>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch, S.key0,
+                              S.key1, d, P);
+      const float c = objective<MAXD, OBJ>(x, d, P);
+      store_vec<MAXD>(x, S.x, S.npad, i, d);
+      store_vec<MAXD>(v, S.v, S.npad, i, d);
+      if (c < pc) {
+        store_vec<MAXD>(x, S.pb, S.npad, i, d);
+        pc = c;
+        S.pc[i] = c;
+      }
+      mykey = umin64(mykey, make_key(pc, (uint32_t)i));
+    }
+  }
+  publish_block<BLOCK>(S, t & 1u, d, mykey,
+                       A.gbest_from == PPS_GBEST_CURRENT ? S.x : S.pb, s_red);
+}
+
+// Whole sync loop in one workgroup of BLOCK threads, one particle per thread
+// held in registers; the winner hands its position over through LDS.
+template <int MAXD, bool EXACT, int OBJ, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepArgs A,
+                                                   float w, float wdamp, uint32_t t0,
+                                                   int32_t T) {
+  __shared__ uint64_t s_red[BLOCK / 64];
+  __shared__ float s_gb[MAXD];
+  const int d = EXACT ? MAXD : P.d;
+  fetch_gbest<BLOCK>(S, t0 & 1u, d, s_gb, s_red);
+  const int64_t i = threadIdx.x;
+  const bool act = i < S.n;
+  float x[MAXD], v[MAXD], pb[MAXD], pc = 0.0f;
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) x[j] = v[j] = pb[j] = 0.0f;
+  if (act) {
+    load_vec<MAXD>(x, S.x, S.npad, i, d);
+    load_vec<MAXD>(v, S.v, S.npad, i, d);
+    load_vec<MAXD>(pb, S.pb, S.npad, i, d);
+    pc = S.pc[i];
+  }
+  const bool cur = A.gbest_from == PPS_GBEST_CURRENT;
+  for (int32_t k = 1; k <= T; ++k) {
+    float gb[MAXD];
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
+    uint64_t mykey = KEY_NONE;
+    if (act) {
+      velocity_position<MAXD>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
+                              S.epoch, S.key0, S.key1, d, P);
+      const float c = objective<MAXD, OBJ>(x, d, P);
+      if (c < pc) {
+#pragma unroll
+        for (int j = 0; j < MAXD; ++j) pb[j] = x[j];
+        pc = c;
+      }
+      mykey = make_key(pc, (uint32_t)i);
+    }
+    const uint64_t bk = block_min_u64<BLOCK>(mykey, s_red);  // ends with a barrier
+    if (act && mykey == bk) {
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j)
+        if (j < d) s_gb[j] = cur ? x[j] : pb[j];
+    }
+    __syncthreads();
+    w = w * wdamp;
+  }
+  if (act) {
+    store_vec<MAXD>(x, S.x, S.npad, i, d);
+    store_vec<MAXD>(v, S.v, S.npad, i, d);
+    store_vec<MAXD>(pb, S.pb, S.npad, i, d);
+    S.pc[i] = pc;
+  }
+  // Hand-over state for whatever runs next (parity of t0 + T).
+  const uint64_t bk = block_min_u64<BLOCK>(act ? make_key(pc, (uint32_t)i) : KEY_NONE, s_red);
+  const uint32_t par = (t0 + (uint32_t)T) & 1u;
+  if ((int)threadIdx.x < d) S.cand[(int64_t)par * d + threadIdx.x] = s_gb[threadIdx.x];
+  if (threadIdx.x == 0) S.bkey[par] = bk;
+}
+
+// Async: reference semantics per particle (src/kernels.jl:205-221):
+//   gbest = gbest_ref (at launch); for iter: step; if best_cost < gbest.cost:
+//   gbest = (best_position, best_cost); w *= wdamp.
+// Since best_cost never increases, after the first switch the running best
+// is the particle's own best, so gbest = follow ? best : gbest_ref with
+// follow = (best_cost < gbest_ref.cost) evaluated after each step.
+template <int MAXD, bool EXACT, int OBJ, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_async(SwarmDev S, ProbArgs P, StepArgs A,
+                                                 float w0, float wdamp, uint32_t t0,
+                                                 int32_t T) {
+  __shared__ uint64_t s_red[BLOCK / 64];
+  __shared__ float s_g0[MAXD];
+  const int d = EXACT ? MAXD : P.d;
+  if ((int)threadIdx.x < d) s_g0[threadIdx.x] = S.gbrec->pos[threadIdx.x];
+  const float g0c = S.gbrec->cost;
+  __syncthreads();
+  float g0[MAXD];
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) g0[j] = j < d ? s_g0[j] : 0.0f;
+  uint64_t mykey = KEY_NONE;
+  for (int64_t tile = blockIdx.x; tile * BLOCK < S.n; tile += gridDim.x) {
+    const int64_t i = tile * BLOCK + threadIdx.x;
+    if (i < S.n) {
+      float x[MAXD], v[MAXD], pb[MAXD];
+      load_vec<MAXD>(x, S.x, S.npad, i, d);
+      load_vec<MAXD>(v, S.v, S.npad, i, d);
+      load_vec<MAXD>(pb, S.pb, S.npad, i, d);
+      float pc = S.pc[i];
+      bool follow = false;
+      float w = w0;
+      for (int32_t k = 1; k <= T; ++k) {
+        float g[MAXD];
+#pragma unroll
+        for (int j = 0; j < MAXD; ++j) g[j] = follow ? pb[j] : g0[j];
+        velocity_position<MAXD>(x, v, pb, g, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
+                                S.epoch, S.key0, S.key1, d, P);
+        const float c = objective<MAXD, OBJ>(x, d, P);
+        if (c < pc) {
+#pragma unroll
+          for (int j = 0; j < MAXD; ++j) pb[j] = x[j];
+          pc = c;
+        }
+        follow = follow || (pc < g0c);
+        w = w * wdamp;
+      }
+      store_vec<MAXD>(x, S.x, S.npad, i, d);
+      store_vec<MAXD>(v, S.v, S.npad, i, d);
+      store_vec<MAXD>(pb, S.pb, S.npad, i, d);
+      S.pc[i] = pc;
+      mykey = umin64(mykey, make_key(pc, (uint32_t)i));
+    }
+  }
+  publish_block<BLOCK>(S, (t0 + (uint32_t)T) & 1u, d, mykey, S.pb, s_red);
+}
+
+__device__ __forceinline__ float ld_acq(const float* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Queue-lock iteration (src/kernels.jl:38-125) with BLOCK = 1024 particles
+// per workgroup, one tile per workgroup.  Each workgroup snapshots the live
+// global best, steps, counts the particles whose best cost beats the live
+// global best, and, if at least min_improvers did, publishes its best under a
+// bounded spin lock when it still beats the global best.
+template <int MAXD, bool EXACT, int OBJ, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArgs A, float w,
+                                                 uint32_t k) {
+  const uint32_t t = *S.tbase + k;
+  __shared__ uint64_t s_red[BLOCK / 64];
+  __shared__ int s_cnt[BLOCK / 64];
+  __shared__ float s_gb[MAXD];
+  const int d = EXACT ? MAXD : P.d;
+  GbRec* R = S.gbrec;
+  if ((int)threadIdx.x < d) s_gb[threadIdx.x] = ld_acq(&R->pos[threadIdx.x]);
+  __syncthreads();
+  float gb[MAXD];
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const bool act = i < S.n;
+  float pc = 0.0f;
+  float pb[MAXD];
+  if (act) {
+    float x[MAXD], v[MAXD];
+    load_vec<MAXD>(x, S.x, S.npad, i, d);
+    load_vec<MAXD>(v, S.v, S.npad, i, d);
+    load_vec<MAXD>(pb, S.pb, S.npad, i, d);
+    pc = S.pc[i];
+    velocity_position<MAXD>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch, S.key0,
+                            S.key1, d, P);
+    const float c = objective<MAXD, OBJ>(x, d, P);
+    store_vec<MAXD>(x, S.x, S.npad, i, d);
+    store_vec<MAXD>(v, S.v, S.npad, i, d);
+    if (c < pc) {
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j) pb[j] = x[j];
+      store_vec<MAXD>(pb, S.pb, S.npad, i, d);
+      pc = c;
+      S.pc[i] = c;
+    }
+  }
+  const bool imp = act && (pc < ld_acq(&R->cost));
+  const int count = block_sum_i32<BLOCK>(imp ? 1 : 0, s_cnt);
+  const uint64_t mykey = imp ? make_key(pc, (uint32_t)i) : KEY_NONE;
+  const uint64_t bk = block_min_u64<BLOCK>(mykey, s_red);
+  if (count >= A.min_improvers && imp && mykey == bk) {
+    // Bounded spin: a holder is a running wave that releases within a few
+    // hundred cycles; give up (and flag it) rather than wait forever.
+    bool got = false;
+    for (int spin = 0; spin < (1 << 22); ++spin) {
+      if (atomicCAS(&R->lock, 0u, 1u) == 0u) {
+        got = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (got) {
+      __atomic_thread_fence(__ATOMIC_ACQUIRE);
+      if (pc < ld_acq(&R->cost)) {
+        for (int j = 0; j < d; ++j)
+          __hip_atomic_store(&R->pos[j], pb[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&R->idx, (uint32_t)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&R->cost, pc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __hip_atomic_exchange(&R->lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      atomicOr(&R->err, 1u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+constexpr int BLK = 256;   // workgroup size of the streaming kernels
+constexpr int QBLK = 1024; // queue-lock block (the reference's workgroup size)
+constexpr int WGB = 1024;  // single-workgroup sync loop
+
+template <int MAXD, bool EXACT, int OBJ>
+struct Launch {
+  static hipError_t init(const SwarmDev& S, const ProbArgs& P, uint32_t par, hipStream_t st) {
+    if (S.block == WGB && S.G == 1)
+      hipLaunchKernelGGL((k_init<MAXD, EXACT, OBJ, WGB>), dim3(1), dim3(WGB), 0, st, S, P, par);
+    else
+      hipLaunchKernelGGL((k_init<MAXD, EXACT, OBJ, BLK>), dim3(S.G), dim3(BLK), 0, st, S, P, par);
+    return hipGetLastError();
+  }
+  static hipError_t rekey(const SwarmDev& S, const ProbArgs& P, uint32_t par, int32_t gf,
+                          hipStream_t st) {
+    if (S.block == WGB && S.G == 1)
+      hipLaunchKernelGGL((k_rekey<MAXD, EXACT, OBJ, WGB>), dim3(1), dim3(WGB), 0, st, S, P, par, gf);
+    else
+      hipLaunchKernelGGL((k_rekey<MAXD, EXACT, OBJ, BLK>), dim3(S.G), dim3(BLK), 0, st, S, P, par, gf);
+    return hipGetLastError();
+  }
+  static hipError_t sync_step(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w,
+                              uint32_t k, hipStream_t st) {
+    if (S.block == WGB && S.G == 1)
+      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, WGB>), dim3(1), dim3(WGB), 0, st, S, P, A, w, k);
+    else
+      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, BLK>), dim3(S.G), dim3(BLK), 0, st, S, P, A, w, k);
+    return hipGetLastError();
+  }
+  static hipError_t sync_wg(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w0,
+                            float wdamp, uint32_t t0, int32_t T, hipStream_t st) {
+    if (!(S.G == 1 && S.block == WGB && S.n <= WGB)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_sync_wg<MAXD, EXACT, OBJ, WGB>), dim3(1), dim3(WGB), 0, st, S, P, A, w0,
+                       wdamp, t0, T);
+    return hipGetLastError();
+  }
+  static hipError_t async_run(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w0,
+                              float wdamp, uint32_t t0, int32_t T, hipStream_t st) {
+    if (S.block == WGB && S.G == 1)
+      hipLaunchKernelGGL((k_async<MAXD, EXACT, OBJ, WGB>), dim3(1), dim3(WGB), 0, st, S, P, A, w0,
+                         wdamp, t0, T);
+    else
+      hipLaunchKernelGGL((k_async<MAXD, EXACT, OBJ, BLK>), dim3(S.G), dim3(BLK), 0, st, S, P, A, w0,
+                         wdamp, t0, T);
+    return hipGetLastError();
+  }
+  static hipError_t qlock_step(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w,
+                               uint32_t k, hipStream_t st) {
+    const int64_t nb = (S.n + QBLK - 1) / QBLK;
+    hipLaunchKernelGGL((k_qlock<MAXD, EXACT, OBJ, QBLK>), dim3((unsigned)nb), dim3(QBLK), 0, st, S,
+                       P, A, w, k);
+    return hipGetLastError();
+  }
+  static const KernelSet* set() {
+    static const KernelSet ks = {MAXD, &init, &rekey, &sync_step, &sync_wg, &async_run, &qlock_step};
+    return &ks;
+  }
+};
+
+// (objective, dimension) -> compiled kernel set.  Dimensions used by the
+// benchmark shapes get exact-size instantiations; others run in the next
+// power-of-two bucket with a runtime dimension.
+template <int OBJ>
+const KernelSet* generic_set(int d) {
+  switch (d) {
+    case 1: return Launch<1, true, OBJ>::set();
+    case 2: return Launch<2, true, OBJ>::set();
+    case 3: return Launch<3, true, OBJ>::set();
+    case 4: return Launch<4, true, OBJ>::set();
+    case 8: return Launch<8, true, OBJ>::set();
+    case 10: return Launch<10, true, OBJ>::set();
+    case 16: return Launch<16, true, OBJ>::set();
+    case 32: return Launch<32, true, OBJ>::set();
+    default: break;
+  }
+  if (d <= 8) return Launch<8, false, OBJ>::set();
+  if (d <= 16) return Launch<16, false, OBJ>::set();
+  if (d <= 32) return Launch<32, false, OBJ>::set();
+  if (d <= 64) return Launch<64, false, OBJ>::set();
+  return nullptr;
+}
+
+}  // namespace pps

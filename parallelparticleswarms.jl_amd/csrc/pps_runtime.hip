@@ -1,0 +1,508 @@
+// pps_runtime.hip — host runtime behind the C ABI of include/pps.h:
+// contexts, swarm ownership in HBM, the iteration drivers (eager launches, a
+// captured hipGraph of the iteration loop, or one resident workgroup), result
+// read-back and the FP64 host evaluator.
+//
+// Reference drivers this replaces: vectorized_solve! (src/lowerlevel_solve.jl:1-121),
+// init/reinit! (src/init.jl:15-141), solve! result assembly (src/solve.jl:11-59).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pps_kernels.h"
+#include "pps_runtime.h"
+
+using namespace pps;
+
+namespace pps {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  return fail(PPS_E_HIP, "%s: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+}
+
+int fill_prob_args(const pps_problem* prob, ProbArgs* P) {
+  if (!prob) return fail(PPS_E_INVALID, "problem is NULL");
+  memset(P, 0, sizeof *P);
+  const int d = prob->d;
+  if (d < 1 || d > PPS_MAXD) return fail(PPS_E_INVALID, "dimension %d outside 1..%d", d, PPS_MAXD);
+  if (!prob->lb || !prob->ub) return fail(PPS_E_INVALID, "bounds are required (lb/ub NULL)");
+  P->obj = prob->objective;
+  P->d = d;
+  for (int j = 0; j < d; ++j) {
+    const float lo = prob->lb[j], hi = prob->ub[j];
+    if (!std::isfinite(lo) || !std::isfinite(hi))
+      return fail(PPS_E_INVALID, "bounds must be finite (dimension %d)", j);
+    if (!(lo <= hi)) return fail(PPS_E_INVALID, "lb[%d] > ub[%d]", j, j);
+    P->lb[j] = lo;
+    P->ub[j] = hi;
+  }
+  if (prob->n_params < 0 || prob->n_params > PPS_MAXPAR)
+    return fail(PPS_E_INVALID, "n_params %d outside 0..%d", prob->n_params, PPS_MAXPAR);
+  if (prob->n_params > 0 && !prob->params) return fail(PPS_E_INVALID, "params is NULL");
+  for (int k = 0; k < prob->n_params; ++k) P->par[k] = prob->params[k];
+  if (prob->shift) {
+    P->has_shift = 1;
+    for (int j = 0; j < d; ++j) P->shift[j] = prob->shift[j];
+  }
+  int need_par = 0;
+  switch (prob->objective) {
+    case PPS_OBJ_ROSENBROCK:
+    case PPS_OBJ_ROSEN2: need_par = 2; break;
+    case PPS_OBJ_QUADRATIC: need_par = 1; break;
+    case PPS_OBJ_LOTKA_VOLTERRA: need_par = 4; break;
+    default: break;
+  }
+  if (prob->n_params < need_par)
+    return fail(PPS_E_INVALID, "objective %d needs %d parameters, got %d", prob->objective,
+                need_par, prob->n_params);
+  if (prob->objective == PPS_OBJ_LOTKA_VOLTERRA) {
+    if (d != 4) return fail(PPS_E_INVALID, "Lotka-Volterra estimation has 4 parameters, d=%d", d);
+    if (!prob->obs || prob->n_obs < 1) return fail(PPS_E_INVALID, "Lotka-Volterra needs observations");
+    const float spo = prob->params[3];
+    if (!(spo >= 1.0f && spo <= 1.0e6f) || spo != std::floor(spo))
+      return fail(PPS_E_INVALID, "steps per observation must be a positive integer");
+    P->n_obs = prob->n_obs;
+    P->steps_per_obs = (int32_t)spo;
+  }
+  return PPS_OK;
+}
+
+}  // namespace pps
+
+// ---------------------------------------------------------------------------
+struct pps_swarm {
+  pps_ctx* ctx = nullptr;
+  ProbArgs P{};
+  pps_swarm_params prm{};
+  const KernelSet* ks = nullptr;
+  SwarmDev S{};
+  void* mem = nullptr;
+  size_t mem_bytes = 0;
+  uint32_t iter = 0;  // completed iterations in this epoch
+  int launch = PPS_LAUNCH_EAGER;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0.0f;
+  // Captured iteration loop, replayed while (T, w, wdamp, step args) match.
+  hipGraphExec_t gexec = nullptr;
+  int32_t g_T = -1;
+  float g_w = 0.0f, g_wdamp = 0.0f;
+  StepArgs g_args{};
+};
+
+#define HIPCHK(expr)                                                   \
+  do {                                                                 \
+    hipError_t e_ = (expr);                                            \
+    if (e_ != hipSuccess) return hip_fail(e_, #expr, __FILE__, __LINE__); \
+  } while (0)
+
+extern "C" {
+
+const char* pps_last_error(void) { return g_err.c_str(); }
+int pps_abi_version(void) { return PPS_ABI_VERSION; }
+
+void pps_swarm_params_default(pps_swarm_params* p) {
+  if (!p) return;
+  memset(p, 0, sizeof *p);
+  p->n = 100;
+  p->mode = PPS_MODE_SYNC;
+  p->gbest_from = PPS_GBEST_PBEST;
+  p->clamp = PPS_CLAMP_COMPONENT;
+  p->launch = PPS_LAUNCH_AUTO;
+  p->seed = 0;
+  p->c1 = 1.4962f;
+  p->c2 = 1.4962f;
+  p->qlock_min_improvers = 2;
+  p->block = 0;
+}
+
+int pps_ctx_create(int device, pps_ctx** out) {
+  if (!out) return fail(PPS_E_INVALID, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  HIPCHK(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count)
+    return fail(PPS_E_INVALID, "device %d not present (%d devices)", device, count);
+  HIPCHK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  pps_ctx* c = new pps_ctx();
+  c->device = device;
+  c->num_cus = prop.multiProcessorCount;
+  c->clock_khz = prop.clockRate;
+  c->total_mem = prop.totalGlobalMem;
+  c->arch = prop.gcnArchName;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "hipStreamCreateWithFlags", __FILE__, __LINE__);
+  }
+  *out = c;
+  return PPS_OK;
+}
+
+int pps_ctx_destroy(pps_ctx* c) {
+  if (!c) return PPS_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return PPS_OK;
+}
+
+int pps_ctx_info(pps_ctx* c, int32_t* num_cus, int32_t* clock_khz, int64_t* total_mem,
+                 int64_t* free_mem) {
+  if (!c) return fail(PPS_E_INVALID, "ctx is NULL");
+  HIPCHK(hipSetDevice(c->device));
+  if (num_cus) *num_cus = c->num_cus;
+  if (clock_khz) *clock_khz = c->clock_khz;
+  if (total_mem) *total_mem = (int64_t)c->total_mem;
+  if (free_mem) {
+    size_t fr = 0, tot = 0;
+    HIPCHK(hipMemGetInfo(&fr, &tot));
+    *free_mem = (int64_t)fr;
+  }
+  return PPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static int swarm_prepare_launch(pps_swarm* s) {
+  const int64_t n = s->prm.n;
+  const int mode = s->prm.mode;
+  int launch = s->prm.launch;
+  if (mode == PPS_MODE_SYNC) {
+    if (launch == PPS_LAUNCH_AUTO) launch = n <= 1024 ? PPS_LAUNCH_WORKGROUP : PPS_LAUNCH_GRAPH;
+    if (launch == PPS_LAUNCH_WORKGROUP && n > 1024)
+      return fail(PPS_E_INVALID, "workgroup launch holds at most 1024 particles (n=%lld)",
+                  (long long)n);
+  } else {
+    launch = PPS_LAUNCH_EAGER;
+  }
+  s->launch = launch;
+  if (launch == PPS_LAUNCH_WORKGROUP) {
+    s->S.G = 1;
+    s->S.block = 1024;
+  } else {
+    s->S.block = 256;
+    const int64_t tiles = (n + 255) / 256;
+    int64_t gmax = s->prm.block > 0 ? s->prm.block : 0;  // reserved: tuned grid
+    (void)gmax;
+    const int64_t cap = (int64_t)s->ctx->num_cus * (mode == PPS_MODE_ASYNC ? 8 : 2);
+    s->S.G = (int32_t)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
+  }
+  return PPS_OK;
+}
+
+static int swarm_alloc(pps_swarm* s, const pps_problem* prob) {
+  const int d = s->P.d;
+  const int64_t n = s->prm.n;
+  const int64_t npad = (n + 63) / 64 * 64;
+  const int G = s->S.G;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align256(off + bytes);
+    return o;
+  };
+  const size_t o_x = take(sizeof(float) * d * npad);
+  const size_t o_v = take(sizeof(float) * d * npad);
+  const size_t o_pb = take(sizeof(float) * d * npad);
+  const size_t o_pc = take(sizeof(float) * npad);
+  const size_t o_bkey = take(sizeof(uint64_t) * 2 * G);
+  const size_t o_cand = take(sizeof(float) * 2 * G * d);
+  const size_t o_rec = take(sizeof(GbRec));
+  const size_t o_t = take(sizeof(uint32_t) * 64);
+  const size_t o_obs = take(sizeof(float) * 2 * (size_t)std::max(0, s->P.n_obs));
+  s->mem_bytes = off;
+  HIPCHK(hipSetDevice(s->ctx->device));
+  hipError_t e = hipMalloc(&s->mem, off);
+  if (e != hipSuccess) {
+    s->mem = nullptr;
+    return fail(PPS_E_NOMEM, "hipMalloc of %zu bytes of swarm state failed: %s", off,
+                hipGetErrorString(e));
+  }
+  char* base = (char*)s->mem;
+  s->S.x = (float*)(base + o_x);
+  s->S.v = (float*)(base + o_v);
+  s->S.pb = (float*)(base + o_pb);
+  s->S.pc = (float*)(base + o_pc);
+  s->S.bkey = (uint64_t*)(base + o_bkey);
+  s->S.cand = (float*)(base + o_cand);
+  s->S.gbrec = (GbRec*)(base + o_rec);
+  s->S.tbase = (const uint32_t*)(base + o_t);
+  s->S.n = n;
+  s->S.npad = npad;
+  s->S.key0 = (uint32_t)s->prm.seed;
+  s->S.key1 = (uint32_t)(s->prm.seed >> 32);
+  HIPCHK(hipMemsetAsync(s->mem, 0, off, s->ctx->stream));
+  if (s->P.n_obs > 0) {
+    float* obs = (float*)(base + o_obs);
+    HIPCHK(hipMemcpyAsync(obs, prob->obs, sizeof(float) * 2 * s->P.n_obs, hipMemcpyHostToDevice,
+                          s->ctx->stream));
+    s->P.obs = obs;
+  }
+  return PPS_OK;
+}
+
+// Uniform init of the current epoch + global best of the initial swarm.
+static int swarm_do_init(pps_swarm* s) {
+  s->S.epoch = s->S.epoch;  // epoch is set by the caller
+  s->iter = 0;
+  HIPCHK(s->ks->init(s->S, s->P, 0u, s->ctx->stream));
+  HIPCHK(launch_finalize(s->S, s->P.d, 0u, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return PPS_OK;
+}
+
+int pps_swarm_init(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_params* prm,
+                   pps_swarm** out) {
+  if (!ctx || !prob || !prm || !out) return fail(PPS_E_INVALID, "NULL argument");
+  *out = nullptr;
+  if (prm->n < 1 || prm->n > (int64_t)0x7FFFFFFF)
+    return fail(PPS_E_INVALID, "num_particles %lld outside 1..2^31-1", (long long)prm->n);
+  if (prm->mode < PPS_MODE_SYNC || prm->mode > PPS_MODE_QLOCK)
+    return fail(PPS_E_INVALID, "unknown mode %d", prm->mode);
+  if (prm->gbest_from != PPS_GBEST_PBEST && prm->gbest_from != PPS_GBEST_CURRENT)
+    return fail(PPS_E_INVALID, "unknown gbest_from %d", prm->gbest_from);
+  if (prm->clamp != PPS_CLAMP_COMPONENT && prm->clamp != PPS_CLAMP_LEXICOGRAPHIC)
+    return fail(PPS_E_INVALID, "unknown clamp %d", prm->clamp);
+  pps_swarm* s = new pps_swarm();
+  s->ctx = ctx;
+  s->prm = *prm;
+  int rc = fill_prob_args(prob, &s->P);
+  if (rc == PPS_OK) {
+    s->P.clamp = prm->clamp;
+    s->ks = find_kernels(s->P.obj, s->P.d);
+    if (!s->ks) rc = fail(PPS_E_INVALID, "no kernels for objective %d at d=%d", s->P.obj, s->P.d);
+  }
+  if (rc == PPS_OK) rc = swarm_prepare_launch(s);
+  if (rc == PPS_OK) rc = swarm_alloc(s, prob);
+  if (rc == PPS_OK) {
+    hipError_t e = hipEventCreate(&s->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+    if (e != hipSuccess) rc = hip_fail(e, "hipEventCreate", __FILE__, __LINE__);
+  }
+  if (rc == PPS_OK) {
+    s->S.epoch = 0;
+    rc = swarm_do_init(s);
+  }
+  if (rc != PPS_OK) {
+    std::string keep = g_err;
+    pps_swarm_free(s);
+    g_err = keep;
+    return rc;
+  }
+  *out = s;
+  return PPS_OK;
+}
+
+int pps_swarm_reinit(pps_swarm* s) {
+  if (!s) return fail(PPS_E_INVALID, "swarm is NULL");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  s->S.epoch += 1;
+  return swarm_do_init(s);
+}
+
+static int run_sync_graph(pps_swarm* s, int32_t T, float w, float wdamp, const StepArgs& A) {
+  hipStream_t st = s->ctx->stream;
+  const bool hit = s->gexec && s->g_T == T && s->g_w == w && s->g_wdamp == wdamp &&
+                   s->g_args.c1 == A.c1 && s->g_args.c2 == A.c2 &&
+                   s->g_args.gbest_from == A.gbest_from;
+  if (!hit) {
+    if (s->gexec) {
+      hipGraphExecDestroy(s->gexec);
+      s->gexec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    float wk = w;
+    hipError_t le = hipSuccess;
+    for (int32_t k = 1; k <= T && le == hipSuccess; ++k) {
+      le = s->ks->sync_step(s->S, s->P, A, wk, (uint32_t)k, st);
+      wk = wk * wdamp;
+    }
+    hipError_t ce = hipStreamEndCapture(st, &g);
+    if (le != hipSuccess) {
+      if (g) hipGraphDestroy(g);
+      return hip_fail(le, "sync_step (capture)", __FILE__, __LINE__);
+    }
+    if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture", __FILE__, __LINE__);
+    hipError_t ie = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      s->gexec = nullptr;
+      return hip_fail(ie, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    s->g_T = T;
+    s->g_w = w;
+    s->g_wdamp = wdamp;
+    s->g_args = A;
+  }
+  HIPCHK(hipGraphLaunch(s->gexec, st));
+  return PPS_OK;
+}
+
+int pps_swarm_run(pps_swarm* s, int32_t maxiters, float w, float wdamp) {
+  if (!s) return fail(PPS_E_INVALID, "swarm is NULL");
+  if (maxiters < 0) return fail(PPS_E_INVALID, "maxiters < 0");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  StepArgs A{s->prm.c1, s->prm.c2, s->prm.gbest_from, s->prm.qlock_min_improvers};
+  const uint32_t t0 = s->iter;
+  const int32_t T = maxiters;
+  HIPCHK(launch_set_u32((uint32_t*)s->S.tbase, t0, st));
+  HIPCHK(hipEventRecord(s->ev0, st));
+  if (T > 0) {
+    switch (s->prm.mode) {
+      case PPS_MODE_SYNC:
+        if (s->launch == PPS_LAUNCH_WORKGROUP) {
+          HIPCHK(s->ks->sync_wg(s->S, s->P, A, w, wdamp, t0, T, st));
+        } else if (s->launch == PPS_LAUNCH_GRAPH) {
+          int rc = run_sync_graph(s, T, w, wdamp, A);
+          if (rc != PPS_OK) return rc;
+        } else {
+          float wk = w;
+          for (int32_t k = 1; k <= T; ++k) {
+            HIPCHK(s->ks->sync_step(s->S, s->P, A, wk, (uint32_t)k, st));
+            wk = wk * wdamp;
+          }
+        }
+        break;
+      case PPS_MODE_ASYNC:
+        HIPCHK(s->ks->async_run(s->S, s->P, A, w, wdamp, t0, T, st));
+        break;
+      case PPS_MODE_QLOCK: {
+        float wk = w;
+        for (int32_t k = 1; k <= T; ++k) {
+          HIPCHK(s->ks->qlock_step(s->S, s->P, A, wk, (uint32_t)k, st));
+          wk = wk * wdamp;
+        }
+        break;
+      }
+    }
+  }
+  HIPCHK(hipEventRecord(s->ev1, st));
+  s->iter = t0 + (uint32_t)T;
+  if (s->prm.mode != PPS_MODE_QLOCK)
+    HIPCHK(launch_finalize(s->S, s->P.d, s->iter & 1u, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
+  if (s->prm.mode == PPS_MODE_QLOCK) {
+    uint32_t err = 0;
+    HIPCHK(hipMemcpy(&err, &s->S.gbrec->err, sizeof err, hipMemcpyDeviceToHost));
+    if (err) return fail(PPS_E_STATE, "queue-lock publication gave up waiting for the lock");
+  }
+  return PPS_OK;
+}
+
+int pps_swarm_get_gbest(pps_swarm* s, float* pos, float* cost, int64_t* idx) {
+  if (!s) return fail(PPS_E_INVALID, "swarm is NULL");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  GbRec r;
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  HIPCHK(hipMemcpy(&r, s->S.gbrec, sizeof r, hipMemcpyDeviceToHost));
+  if (pos) memcpy(pos, r.pos, sizeof(float) * s->P.d);
+  if (cost) *cost = r.cost;
+  if (idx) *idx = (int64_t)r.idx;
+  return PPS_OK;
+}
+
+int pps_swarm_set_gbest(pps_swarm* s, const float* pos, float cost) {
+  if (!s || !pos) return fail(PPS_E_INVALID, "NULL argument");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  // Stage the position in the global-best record, then fan it out to the
+  // hand-over slots of the next iteration.
+  HIPCHK(hipMemcpy(s->S.gbrec->pos, pos, sizeof(float) * s->P.d, hipMemcpyHostToDevice));
+  HIPCHK(launch_set_gbest(s->S, s->P.d, s->iter & 1u, s->S.gbrec->pos, cost, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return PPS_OK;
+}
+
+int pps_swarm_get_positions(pps_swarm* s, float* out) {
+  if (!s || !out) return fail(PPS_E_INVALID, "NULL argument");
+  const int d = s->P.d;
+  const int64_t n = s->prm.n;
+  std::vector<float> tmp((size_t)d * n);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  HIPCHK(hipMemcpy2D(tmp.data(), sizeof(float) * n, s->S.x, sizeof(float) * s->S.npad,
+                     sizeof(float) * n, d, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n; ++i)
+    for (int j = 0; j < d; ++j) out[i * d + j] = tmp[(size_t)j * n + i];
+  return PPS_OK;
+}
+
+int pps_swarm_get_state(pps_swarm* s, float* x, float* v, float* best, float* best_cost,
+                        uint32_t* iter) {
+  if (!s) return fail(PPS_E_INVALID, "swarm is NULL");
+  const int d = s->P.d;
+  const int64_t n = s->prm.n;
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  const size_t w = sizeof(float) * n, sp = sizeof(float) * s->S.npad;
+  if (x) HIPCHK(hipMemcpy2D(x, w, s->S.x, sp, w, d, hipMemcpyDeviceToHost));
+  if (v) HIPCHK(hipMemcpy2D(v, w, s->S.v, sp, w, d, hipMemcpyDeviceToHost));
+  if (best) HIPCHK(hipMemcpy2D(best, w, s->S.pb, sp, w, d, hipMemcpyDeviceToHost));
+  if (best_cost) HIPCHK(hipMemcpy(best_cost, s->S.pc, w, hipMemcpyDeviceToHost));
+  if (iter) *iter = s->iter;
+  return PPS_OK;
+}
+
+int pps_swarm_set_state(pps_swarm* s, const float* x, const float* v, const float* best,
+                        const float* best_cost, uint32_t iter) {
+  if (!s) return fail(PPS_E_INVALID, "swarm is NULL");
+  const int d = s->P.d;
+  const int64_t n = s->prm.n;
+  HIPCHK(hipSetDevice(s->ctx->device));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  const size_t w = sizeof(float) * n, sp = sizeof(float) * s->S.npad;
+  if (x) HIPCHK(hipMemcpy2D(s->S.x, sp, x, w, w, d, hipMemcpyHostToDevice));
+  if (v) HIPCHK(hipMemcpy2D(s->S.v, sp, v, w, w, d, hipMemcpyHostToDevice));
+  if (best) HIPCHK(hipMemcpy2D(s->S.pb, sp, best, w, w, d, hipMemcpyHostToDevice));
+  if (best_cost) HIPCHK(hipMemcpy(s->S.pc, best_cost, w, hipMemcpyHostToDevice));
+  s->iter = iter;
+  // Re-derive the hand-over state (block keys, candidates, global best).
+  HIPCHK(s->ks->rekey(s->S, s->P, iter & 1u, s->prm.gbest_from, s->ctx->stream));
+  HIPCHK(launch_finalize(s->S, d, iter & 1u, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  return PPS_OK;
+}
+
+int pps_swarm_elapsed_ms(pps_swarm* s, float* ms) {
+  if (!s || !ms) return fail(PPS_E_INVALID, "NULL argument");
+  *ms = s->last_ms;
+  return PPS_OK;
+}
+
+int pps_swarm_free(pps_swarm* s) {
+  if (!s) return PPS_OK;
+  if (s->ctx) hipSetDevice(s->ctx->device);
+  if (s->ctx && s->ctx->stream) hipStreamSynchronize(s->ctx->stream);
+  if (s->gexec) hipGraphExecDestroy(s->gexec);
+  if (s->ev0) hipEventDestroy(s->ev0);
+  if (s->ev1) hipEventDestroy(s->ev1);
+  if (s->mem) hipFree(s->mem);
+  delete s;
+  return PPS_OK;
+}
+
+}  // extern "C"

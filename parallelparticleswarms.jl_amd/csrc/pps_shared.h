@@ -1,0 +1,86 @@
+// pps_shared.h — constants shared by libpps device code and host runtime.
+#pragma once
+#include "pps.h"
+
+#define PPS_MAXD 64    // largest supported dimension
+#define PPS_MAXPAR 8   // objective parameters carried in kernel arguments
+
+// Philox4x32 round multipliers and Weyl key increments (Salmon et al., SC'11).
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+// Third counter word = block index | tag << 16.
+#define PPS_TAG_STEP 0u
+#define PPS_TAG_INIT 1u
+
+// Taylor coefficients (-1)^k (2 pi)^(2k) / (2k)! rounded to FP32, k = 1..7.
+#define COS2PI_C1 (-0x1.3bd3ccp+4f)
+#define COS2PI_C2 (0x1.03c1f0p+6f)
+#define COS2PI_C3 (-0x1.55d3c8p+6f)
+#define COS2PI_C4 (0x1.e1f506p+5f)
+#define COS2PI_C5 (-0x1.a6d1f2p+4f)
+#define COS2PI_C6 (0x1.f9d38ap+2f)
+#define COS2PI_C7 (-0x1.b6e250p+0f)
+
+#include <stdint.h>
+
+namespace pps {
+
+// Problem constants passed by value as a kernel argument (a few hundred bytes
+// of kernarg, read through scalar loads).
+struct ProbArgs {
+  int32_t obj;
+  int32_t d;
+  int32_t has_shift;
+  int32_t clamp;  // PPS_CLAMP_*
+  float lb[PPS_MAXD];
+  float ub[PPS_MAXD];
+  float shift[PPS_MAXD];
+  float par[PPS_MAXPAR];
+  const float* obs;  // Lotka-Volterra observations (device), 2*n_obs floats
+  int32_t n_obs;
+  int32_t steps_per_obs;
+};
+
+// Global-best record in device memory (async and queue-lock modes keep their
+// live global best here; every mode leaves its final result here).
+struct GbRec {
+  float cost;
+  uint32_t idx;
+  uint32_t lock;     // queue-lock mutex word
+  uint32_t err;      // set when a bounded lock wait gave up
+  float pos[PPS_MAXD];
+};
+
+// Device view of one swarm.  SoA: coordinate j of particle i is at
+// [j * npad + i]; npad is n rounded up to 64.
+//
+// Hand-over of the global best between sync iterations: each of the G
+// workgroups owns the particle tiles b, b+G, b+2G, ... (tile = `block`
+// consecutive particles) and, after stepping them, writes its block argmin key
+// bkey[par][b] and the winner's position cand[par][b][0..d-1]; iteration t
+// writes parity t & 1 and reads parity (t-1) & 1, so no workgroup reads what
+// another is writing within one launch.
+struct SwarmDev {
+  float* x;
+  float* v;
+  float* pb;
+  float* pc;
+  int64_t n;
+  int64_t npad;
+  uint64_t* bkey;  // [2][G]
+  float* cand;     // [2][G][d]
+  GbRec* gbrec;
+  // Iteration counter base: launch k of a run executes random-stream
+  // iteration t = *tbase + k, so a captured iteration loop can be replayed
+  // from any starting iteration.
+  const uint32_t* tbase;
+  uint32_t key0, key1;  // Philox key (seed)
+  uint32_t epoch;
+  int32_t G;
+  int32_t block;
+};
+
+}  // namespace pps

@@ -1,0 +1,438 @@
+"""Host-side mirror of ParallelParticleSwarms.jl's public interface over libpps.
+
+Names and argument meanings follow the reference (v1.4.0):
+
+* algorithm descriptors ``ParallelPSOKernel(num_particles; global_update,
+  backend)``, ``ParallelSyncPSOKernel(num_particles; backend)``,
+  ``HybridPSO(; backend, pso, local_opt)``, ``SimpleLBFGS()``
+  (src/algorithms.jl:78-121,192-249);
+* ``init`` / ``solve_`` (Julia ``solve!``) / ``reinit_`` (``reinit!``) /
+  ``solve`` (src/init.jl:56-184, src/solve.jl:11-67);
+* the solution fields ``u``, ``objective``, ``original``, ``stats.time``,
+  ``retcode`` (src/solve.jl:28-33,53-58).
+
+The backend is ``MI355XBackend()``: the swarm lives in HBM on one GPU and every
+iteration runs in the HIP kernels of libpps.  Objectives are a closed set of
+markers (``Rosenbrock()``, ``Rastrigin()``, ...) because an arbitrary Python or
+Julia callable cannot cross the C ABI into a kernel; anything else raises
+``ValueError`` (the reference compiles any closure into its kernel,
+src/kernels.jl:31-33).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+from typing import Any, Optional
+
+import numpy as np
+
+from . import _lib as L
+
+# ---------------------------------------------------------------------------
+# random seeding (stands in for Julia's Random.seed! + task-local RNG)
+_seed_state = {"base": 0x5EED_0F_5EED, "count": 0}
+
+
+def seed_(s: int) -> None:
+    """Reset the seed sequence that ``init`` draws swarm seeds from."""
+    _seed_state["base"] = int(s) & 0xFFFFFFFFFFFFFFFF
+    _seed_state["count"] = 0
+
+
+def _splitmix64(z: int) -> int:
+    z = (z + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return z ^ (z >> 31)
+
+
+def _next_seed() -> int:
+    s = _splitmix64(_seed_state["base"] + _seed_state["count"])
+    _seed_state["count"] += 1
+    return s
+
+
+# ---------------------------------------------------------------------------
+# backend and contexts
+@dataclass(frozen=True)
+class MI355XBackend:
+    """One MI355X (gfx950) device, addressed by HIP device ordinal."""
+
+    device: int = 0
+
+
+_contexts: dict[int, "Context"] = {}
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        L.check(L.lib().pps_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def info(self) -> dict:
+        cu, clk = C.c_int32(), C.c_int32()
+        tot, fr = C.c_int64(), C.c_int64()
+        L.check(L.lib().pps_ctx_info(self.h, C.byref(cu), C.byref(clk), C.byref(tot), C.byref(fr)))
+        return {"num_cus": cu.value, "clock_khz": clk.value, "total_mem": tot.value,
+                "free_mem": fr.value}
+
+    def close(self):
+        if self.h:
+            L.lib().pps_ctx_destroy(self.h)
+            self.h = None
+
+
+def context(backend: MI355XBackend = MI355XBackend()) -> Context:
+    c = _contexts.get(backend.device)
+    if c is None:
+        c = _contexts[backend.device] = Context(backend.device)
+    return c
+
+
+# ---------------------------------------------------------------------------
+# objectives
+@dataclass(frozen=True)
+class Objective:
+    code: int
+    name: str
+    fixed_dim: Optional[int] = None
+    n_params: int = 0
+
+    def __call__(self, x, p=None):
+        """FP64 host evaluation (the reference's ``prob.f(u, p)``)."""
+        x = np.asarray(x, np.float64).reshape(1, -1)
+        lb = np.full(x.shape[1], -np.inf)
+        prob = OptimizationProblem(self, np.zeros(x.shape[1], np.float32), p, lb=lb, ub=-lb,
+                                   _validate=False)
+        return float(eval_f64(prob, x)[0])
+
+
+def Rosenbrock():
+    """Σ_{i<d} p[2](x[i+1]-x[i]^2)^2 + (p[1]-x[i])^2 (test/gpu.jl:14-20)."""
+    return Objective(L.OBJ_ROSENBROCK, "rosenbrock", None, 2)
+
+
+def Rastrigin():
+    """10 d + Σ (x_i^2 - 10 cos(2π x_i))."""
+    return Objective(L.OBJ_RASTRIGIN, "rastrigin")
+
+
+def Sphere():
+    return Objective(L.OBJ_SPHERE, "sphere")
+
+
+def Beale():
+    return Objective(L.OBJ_BEALE, "beale", 2)
+
+
+def Booth():
+    return Objective(L.OBJ_BOOTH, "booth", 2)
+
+
+def Himmelblau():
+    return Objective(L.OBJ_HIMMELBLAU, "himmelblau", 2)
+
+
+def Quadratic():
+    """p[1] (x1 - 1)^2 + (x2 + 2)^2 (test/lbfgs.jl:32)."""
+    return Objective(L.OBJ_QUADRATIC, "quadratic", 2, 1)
+
+
+def BoundaryQuadratic():
+    """(x1 + 1)^2 (test/lbfgs.jl:33)."""
+    return Objective(L.OBJ_BOUNDARY_QUADRATIC, "boundary_quadratic", 1)
+
+
+def Rosen2():
+    """(p[1] - x1)^2 + p[2] (x2 - x1^2)^2 (test/lbfgs.jl:34)."""
+    return Objective(L.OBJ_ROSEN2, "rosen2", 2, 2)
+
+
+@dataclass(frozen=True)
+class LotkaVolterraSSE(Objective):
+    """Sum of squared errors of an FP32 RK4 Lotka-Volterra trajectory against
+    observations (the loss of src/ode_pso.jl:47-52).  The decision variable is
+    (α, β, γ, δ); ``obs`` is (n_obs, 2), sampled every ``steps_per_obs`` RK4
+    steps of ``dt`` starting from ``u0`` at sample 0."""
+
+    obs: Any = None
+    u0: tuple = (1.0, 1.0)
+    dt: float = 0.01
+    steps_per_obs: int = 10
+
+
+def lotka_volterra_sse(obs, u0=(1.0, 1.0), dt=0.01, steps_per_obs=10) -> LotkaVolterraSSE:
+    obs = np.ascontiguousarray(obs, np.float32).reshape(-1, 2)
+    return LotkaVolterraSSE(L.OBJ_LOTKA_VOLTERRA, "lotka_volterra", 4, 4, obs, tuple(u0),
+                            float(dt), int(steps_per_obs))
+
+
+# ---------------------------------------------------------------------------
+# problem
+class OptimizationProblem:
+    """``OptimizationProblem(f, u0, p; lb, ub)``.  Bounds are required: the
+    unbounded and QMC samplers of the reference (src/init.jl:15-36) are not part
+    of this backend."""
+
+    def __init__(self, f, u0, p=None, *, lb=None, ub=None, shift=None, _validate=True):
+        if not isinstance(f, Objective):
+            raise ValueError(
+                "objective must be one of the ppswarm objective markers (Rosenbrock(), "
+                f"Rastrigin(), ...); got {type(f).__name__}")
+        self.f = f
+        self.u0 = np.asarray(u0, np.float32).reshape(-1)
+        self.d = int(self.u0.size)
+        if f.fixed_dim is not None and self.d != f.fixed_dim:
+            raise ValueError(f"{f.name} is {f.fixed_dim}-dimensional, u0 has {self.d}")
+        if isinstance(f, LotkaVolterraSSE):
+            self.p = np.array([f.u0[0], f.u0[1], f.dt, f.steps_per_obs], np.float32)
+        elif p is None:
+            self.p = np.zeros(0, np.float32)
+        else:
+            self.p = np.asarray(p, np.float32).reshape(-1)
+        if _validate and self.p.size < f.n_params:
+            raise ValueError(f"{f.name} needs {f.n_params} parameters, got {self.p.size}")
+        if lb is None or ub is None:
+            if _validate:
+                raise ValueError("lb and ub are required (unbounded sampling is not supported)")
+        self.lb = None if lb is None else np.asarray(lb, np.float32).reshape(-1)
+        self.ub = None if ub is None else np.asarray(ub, np.float32).reshape(-1)
+        if _validate and (self.lb.size != self.d or self.ub.size != self.d):
+            raise ValueError("lb/ub must have the dimension of u0")
+        self.shift = None if shift is None else np.asarray(shift, np.float32).reshape(-1)
+
+    def c(self):
+        """pps_problem view; the returned tuple keeps the arrays alive."""
+        keep = []
+
+        def ptr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float32)
+            keep.append(a)
+            return a.ctypes.data_as(L.fp)
+
+        pr = L.pps_problem()
+        pr.objective = self.f.code
+        pr.d = self.d
+        pr.lb = ptr(self.lb)
+        pr.ub = ptr(self.ub)
+        pr.params = ptr(self.p) if self.p.size else None
+        pr.n_params = int(self.p.size)
+        pr.shift = ptr(self.shift)
+        if isinstance(self.f, LotkaVolterraSSE):
+            pr.obs = ptr(self.f.obs)
+            pr.n_obs = int(np.asarray(self.f.obs).size // 2)
+        return pr, keep
+
+
+def eval_f64(prob: OptimizationProblem, xs) -> np.ndarray:
+    """FP64 host evaluation of the objective at the rows of xs (n x d)."""
+    xs = np.ascontiguousarray(xs, np.float64).reshape(-1, prob.d)
+    out = np.zeros(len(xs), np.float64)
+    pr, keep = prob.c()
+    L.check(L.lib().pps_eval_f64(C.byref(pr), len(xs), xs.ctypes.data_as(L.dp),
+                                 out.ctypes.data_as(L.dp)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# algorithms (src/algorithms.jl)
+@dataclass
+class ParallelPSOKernel:
+    num_particles: int
+    global_update: bool = True
+    backend: MI355XBackend = field(default_factory=MI355XBackend)
+
+
+@dataclass
+class ParallelSyncPSOKernel:
+    num_particles: int
+    backend: MI355XBackend = field(default_factory=MI355XBackend)
+
+
+@dataclass
+class SimpleLBFGS:
+    """Projected L-BFGS local search (history m, Armijo backtracking)."""
+
+    threshold: int = 10
+
+
+@dataclass
+class HybridPSO:
+    backend: MI355XBackend = field(default_factory=MI355XBackend)
+    pso: Any = None
+    local_opt: Any = field(default_factory=SimpleLBFGS)
+
+    def __post_init__(self):
+        if self.pso is None:
+            self.pso = ParallelPSOKernel(100, global_update=False, backend=self.backend)
+
+
+# ---------------------------------------------------------------------------
+@dataclass
+class GBest:
+    position: np.ndarray
+    cost: float
+    index: int = -1
+
+
+@dataclass
+class Stats:
+    time: float
+    device_ms: float = 0.0
+
+
+@dataclass
+class OptimizationSolution:
+    u: np.ndarray
+    objective: float          # FP64 host evaluation of f(u)
+    objective_f32: float      # FP32 cost the device computed for u
+    original: Optional[np.ndarray]
+    stats: Stats
+    retcode: str = "Default"
+    index: int = -1
+
+
+class Swarm:
+    """Owning wrapper of a pps_swarm handle."""
+
+    def __init__(self, ctx: Context, prob: OptimizationProblem, params: L.pps_swarm_params):
+        self.ctx, self.prob, self.params = ctx, prob, params
+        pr, keep = prob.c()
+        h = C.c_void_p()
+        L.check(L.lib().pps_swarm_init(ctx.h, C.byref(pr), C.byref(params), C.byref(h)))
+        self.h = h
+        self.d, self.n = prob.d, int(params.n)
+
+    def __del__(self):
+        self.free()
+
+    def free(self):
+        if getattr(self, "h", None):
+            L.lib().pps_swarm_free(self.h)
+            self.h = None
+
+    def run(self, maxiters: int, w: float = 0.7298, wdamp: float = 1.0) -> None:
+        L.check(L.lib().pps_swarm_run(self.h, int(maxiters), float(np.float32(w)),
+                                      float(np.float32(wdamp))))
+
+    def reinit(self):
+        L.check(L.lib().pps_swarm_reinit(self.h))
+
+    def gbest(self) -> GBest:
+        pos = np.zeros(self.d, np.float32)
+        cost = C.c_float()
+        idx = C.c_int64()
+        L.check(L.lib().pps_swarm_get_gbest(self.h, pos.ctypes.data_as(L.fp), C.byref(cost),
+                                            C.byref(idx)))
+        return GBest(pos, float(cost.value), int(idx.value))
+
+    def set_gbest(self, pos, cost) -> None:
+        pos = np.ascontiguousarray(pos, np.float32)
+        L.check(L.lib().pps_swarm_set_gbest(self.h, pos.ctypes.data_as(L.fp), float(cost)))
+
+    def positions(self) -> np.ndarray:
+        out = np.zeros((self.n, self.d), np.float32)
+        L.check(L.lib().pps_swarm_get_positions(self.h, out.ctypes.data_as(L.fp)))
+        return out
+
+    def state(self):
+        x = np.zeros((self.d, self.n), np.float32)
+        v = np.zeros((self.d, self.n), np.float32)
+        b = np.zeros((self.d, self.n), np.float32)
+        bc = np.zeros(self.n, np.float32)
+        it = C.c_uint32()
+        L.check(L.lib().pps_swarm_get_state(self.h, x.ctypes.data_as(L.fp), v.ctypes.data_as(L.fp),
+                                            b.ctypes.data_as(L.fp), bc.ctypes.data_as(L.fp),
+                                            C.byref(it)))
+        return x, v, b, bc, int(it.value)
+
+    def set_state(self, x, v, best, best_cost, it: int):
+        a = [np.ascontiguousarray(q, np.float32) for q in (x, v, best, best_cost)]
+        L.check(L.lib().pps_swarm_set_state(self.h, *[q.ctypes.data_as(L.fp) for q in a],
+                                            int(it)))
+
+    def elapsed_ms(self) -> float:
+        ms = C.c_float()
+        L.check(L.lib().pps_swarm_elapsed_ms(self.h, C.byref(ms)))
+        return float(ms.value)
+
+
+def swarm_params(n: int, mode: int = L.MODE_SYNC, *, seed: int = 0,
+                 gbest_from: int = L.GBEST_PBEST, clamp: int = L.CLAMP_COMPONENT,
+                 launch: int = L.LAUNCH_AUTO, qlock_min_improvers: int = 2,
+                 c1: float = 1.4962, c2: float = 1.4962) -> L.pps_swarm_params:
+    p = L.pps_swarm_params()
+    L.lib().pps_swarm_params_default(C.byref(p))
+    p.n, p.mode, p.seed = int(n), int(mode), int(seed) & 0xFFFFFFFFFFFFFFFF
+    p.gbest_from, p.clamp, p.launch = int(gbest_from), int(clamp), int(launch)
+    p.qlock_min_improvers = int(qlock_min_improvers)
+    p.c1, p.c2 = float(c1), float(c2)
+    return p
+
+
+# ---------------------------------------------------------------------------
+# init / solve! / reinit! (src/init.jl, src/solve.jl)
+@dataclass
+class PSOCache:
+    prob: OptimizationProblem
+    alg: Any
+    swarm: Swarm
+    gbest: GBest
+    seed: int
+
+
+def _mode_of(alg) -> int:
+    if isinstance(alg, ParallelSyncPSOKernel):
+        return L.MODE_SYNC
+    if isinstance(alg, ParallelPSOKernel):
+        return L.MODE_QLOCK if alg.global_update else L.MODE_ASYNC
+    raise ValueError(f"unsupported algorithm {type(alg).__name__}")
+
+
+def init(prob: OptimizationProblem, alg, *, seed: Optional[int] = None, **kw) -> PSOCache:
+    if isinstance(alg, HybridPSO):
+        return init(prob, alg.pso, seed=seed, **kw)
+    seed = _next_seed() if seed is None else int(seed)
+    ctx = context(alg.backend)
+    params = swarm_params(alg.num_particles, _mode_of(alg), seed=seed, **kw)
+    sw = Swarm(ctx, prob, params)
+    return PSOCache(prob, alg, sw, sw.gbest(), seed)
+
+
+def reinit_(cache: PSOCache) -> None:
+    cache.swarm.reinit()
+    cache.gbest = cache.swarm.gbest()
+
+
+def solve_(cache: PSOCache, maxiters: int = 100, *, w: float = 0.7298, wdamp: float = 1.0,
+           original: bool = True) -> OptimizationSolution:
+    """``solve!``.  For ParallelSyncPSOKernel the attractor of the first
+    iteration is the cache's initial global best on every call, as in the
+    reference (its solve! does not store the result back, src/solve.jl:36-59)."""
+    sw = cache.swarm
+    if isinstance(cache.alg, ParallelSyncPSOKernel):
+        sw.set_gbest(cache.gbest.position, cache.gbest.cost)
+    t0 = time.perf_counter()
+    sw.run(maxiters, w, wdamp)
+    t1 = time.perf_counter()
+    g = sw.gbest()
+    if not isinstance(cache.alg, ParallelSyncPSOKernel):
+        cache.gbest = g
+    obj = float(eval_f64(cache.prob, g.position[None, :])[0])
+    return OptimizationSolution(g.position, obj, g.cost, sw.positions() if original else None,
+                                Stats(t1 - t0, sw.elapsed_ms()), "Default", g.index)
+
+
+def solve(prob: OptimizationProblem, alg, maxiters: int = 100, *, seed: Optional[int] = None,
+          w: float = 0.7298, wdamp: float = 1.0, **kw) -> OptimizationSolution:
+    if isinstance(alg, HybridPSO):
+        from .hybrid import solve_hybrid
+
+        return solve_hybrid(prob, alg, maxiters, seed=seed, w=w, wdamp=wdamp, **kw)
+    cache = init(prob, alg, seed=seed)
+    return solve_(cache, maxiters, w=w, wdamp=wdamp)

@@ -1,0 +1,167 @@
+"""Parity of the HIP kernels with the CPU oracle, through the C ABI.
+
+The bar is bit-exact: positions, velocities, best positions and best costs of
+every particle, and the global-best index (ties to the lowest index), equal
+the oracle's after every checked iteration.  The update is FP32 in a fixed
+operation order without contraction on both sides, so no tolerance is needed;
+a failure means the kernel computes something else.
+"""
+import numpy as np
+import pytest
+
+import pps_oracle as O
+from parity_util import assert_same_gbest, assert_same_state, make_pair, note
+from ppswarm import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("obj,d,n", [(O.ROSENBROCK, 3, 1000), (O.RASTRIGIN, 10, 5000),
+                                     (O.SPHERE, 7, 333), (O.ROSENBROCK, 32, 700),
+                                     (O.RASTRIGIN, 45, 300), (O.BEALE, 2, 100)])
+def test_init_matches_oracle(obj, d, n):
+    note(f"init obj={obj} d={d} n={n}")
+    g, o = make_pair(obj, d, n, seed=1000 + d)
+    assert_same_state(g, o, "after init")
+    assert_same_gbest(g, o, "after init")
+    g.reinit()
+    o.reinit()
+    assert_same_state(g, o, "after reinit")
+    assert_same_gbest(g, o, "after reinit")
+
+
+@pytest.mark.parametrize("gbest_from", [L.GBEST_PBEST, L.GBEST_CURRENT])
+def test_sync_c0_workgroup_trajectory(gbest_from):
+    # 3-D Rosenbrock, lb=-1, ub=10, 1000 particles, 500 iterations
+    note(f"sync C0 workgroup gbest_from={gbest_from}")
+    g, o = make_pair(O.ROSENBROCK, 3, 1000, seed=42, lb=-1.0, ub=10.0, gbest_from=gbest_from)
+    done = 0
+    for chunk in (1, 1, 8, 40, 450):
+        g.run(chunk)
+        o.run_sync(chunk, gbest_from=gbest_from)
+        done += chunk
+        assert_same_state(g, o, f"after {done} iterations")
+        assert_same_gbest(g, o, f"after {done} iterations")
+
+
+@pytest.mark.parametrize("launch", [L.LAUNCH_EAGER, L.LAUNCH_GRAPH])
+@pytest.mark.parametrize("obj,d,n", [(O.RASTRIGIN, 10, 5000), (O.ROSENBROCK, 10, 3001),
+                                     (O.ROSENBROCK, 5, 70000)])
+def test_sync_multiblock_trajectory(launch, obj, d, n):
+    note(f"sync multiblock launch={launch} obj={obj} d={d} n={n}")
+    g, o = make_pair(obj, d, n, seed=7 + n, launch=launch)
+    done = 0
+    for chunk in (1, 4, 15):
+        g.run(chunk, 0.7298, 0.99)
+        o.run_sync(chunk, 0.7298, 0.99)
+        done += chunk
+        assert_same_state(g, o, f"after {done} iterations")
+        assert_same_gbest(g, o, f"after {done} iterations")
+
+
+def test_sync_current_mode_multiblock():
+    note("sync multiblock gbest_from=CURRENT")
+    g, o = make_pair(O.ROSENBROCK, 4, 4000, seed=99, lb=-1.0, ub=10.0,
+                     gbest_from=L.GBEST_CURRENT, launch=L.LAUNCH_GRAPH)
+    g.run(25)
+    o.run_sync(25, gbest_from=O.GBEST_CURRENT)
+    assert_same_state(g, o)
+    assert_same_gbest(g, o)
+
+
+@pytest.mark.parametrize("n", [500, 5000])
+def test_one_step_from_identical_state(n):
+    # arbitrary states, including non-finite velocities and best costs, fed to
+    # both sides, then one step against the same attractor
+    d = 6
+    note(f"one step from state n={n}")
+    g, o = make_pair(O.RASTRIGIN, d, n, seed=5, launch=L.LAUNCH_EAGER)
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-5.12, 5.12, (d, n)).astype(np.float32)
+    v = rng.normal(0, 3, (d, n)).astype(np.float32)
+    v[0, :5] = [np.inf, -np.inf, np.nan, 1e30, -0.0]
+    b = rng.uniform(-5.12, 5.12, (d, n)).astype(np.float32)
+    bc = rng.uniform(0, 200, n).astype(np.float32)
+    bc[5:8] = [np.nan, np.inf, 0.0]
+    g.set_state(x, v, b, bc, 17)
+    o.set_state(x, v, b, bc, 17)
+    gb = rng.uniform(-1, 1, d).astype(np.float32)
+    g.set_gbest(gb, 1.0)
+    o.set_gbest(gb, 1.0)
+    g.run(1, 0.5, 1.0)
+    o.run_sync(1, 0.5, 1.0)
+    assert_same_state(g, o, "after one step")
+    assert_same_gbest(g, o, "after one step")
+
+
+@pytest.mark.parametrize("obj,d,n,T", [(O.RASTRIGIN, 10, 3000, 60), (O.ROSENBROCK, 3, 1000, 200),
+                                       (O.ROSENBROCK, 16, 2500, 30), (O.SPHERE, 20, 600, 40)])
+def test_async_matches_oracle(obj, d, n, T):
+    note(f"async obj={obj} d={d} n={n} T={T}")
+    g, o = make_pair(obj, d, n, seed=11 * d, mode=L.MODE_ASYNC)
+    g.run(T, 0.7298, 0.995)
+    o.run_async(T, 0.7298, 0.995)
+    assert_same_state(g, o, f"after {T} iterations")
+    assert_same_gbest(g, o)
+    # a second launch starts from the stored global best
+    g.run(7)
+    o.run_async(7)
+    assert_same_state(g, o, "after the second launch")
+    assert_same_gbest(g, o)
+
+
+@pytest.mark.parametrize("min_improvers", [2, 1])
+def test_qlock_single_block_matches_oracle(min_improvers):
+    # one workgroup (n <= 1024): the live global best is deterministic
+    note(f"qlock single block min_improvers={min_improvers}")
+    g, o = make_pair(O.ROSENBROCK, 3, 1000, seed=3, mode=L.MODE_QLOCK, lb=-1.0, ub=10.0,
+                     min_improvers=min_improvers)
+    for chunk in (1, 9, 90):
+        g.run(chunk)
+        o.run_qlock(chunk, min_improvers=min_improvers)
+        assert_same_state(g, o)
+        assert_same_gbest(g, o)
+
+
+def test_qlock_multiblock_invariants():
+    # several workgroups race on the live global best, as in the reference:
+    # check what every schedule guarantees
+    d, n = 4, 20000
+    note("qlock multiblock invariants")
+    g, o = make_pair(O.ROSENBROCK, d, n, seed=8, mode=L.MODE_QLOCK, lb=-1.0, ub=10.0)
+    prev = g.gbest().cost
+    for _ in range(20):
+        g.run(5)
+        gb = g.gbest()
+        x, v, b, bc, it = g.state()
+        assert gb.cost <= prev
+        assert bc.min() <= gb.cost
+        assert (x >= -1).all() and (x <= 10).all()
+        # the published global best is a real best position of its particle
+        # (or an earlier one of the same particle with a higher cost)
+        assert bc[gb.index] <= gb.cost
+        prev = gb.cost
+
+
+def test_lotka_volterra_sync_with_lexicographic_clamp():
+    rng = np.random.default_rng(0)
+    obs = (rng.uniform(0.5, 3, (101, 2))).astype(np.float32)
+    note("LV sync lexicographic clamp")
+    g, o = make_pair(O.LOTKA_VOLTERRA, 4, 300, seed=21, lb=0.0, ub=5.0, obs=obs,
+                     clamp=L.CLAMP_LEXICOGRAPHIC, launch=L.LAUNCH_EAGER)
+    assert_same_state(g, o, "after init")
+    g.run(3)
+    o.run_sync(3)
+    assert_same_state(g, o, "after 3 iterations")
+    assert_same_gbest(g, o)
+
+
+def test_shifted_objective_parity():
+    d, n = 8, 2048
+    shift = np.linspace(-1, 1, d).astype(np.float32)
+    note("shifted rastrigin parity")
+    g, o = make_pair(O.RASTRIGIN, d, n, seed=77, shift=shift, launch=L.LAUNCH_GRAPH)
+    g.run(10)
+    o.run_sync(10)
+    assert_same_state(g, o)
+    assert_same_gbest(g, o)
