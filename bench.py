@@ -70,7 +70,7 @@ def traffic_from_profiles():
     return None
 
 
-def cpu_baseline(sample_iters=6):
+def cpu_baseline(sample_iters=200):
     import pps_oracle as O
 
     O.lib()

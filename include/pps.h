@@ -186,6 +186,25 @@ int pps_swarm_set_state(pps_swarm* s, const float* x, const float* v,
 int pps_swarm_elapsed_ms(pps_swarm* s, float* ms);
 int pps_swarm_free(pps_swarm* s);
 
+/* ---- HybridPSO local phase (src/hybrid.jl:100-158, src/lbfgs.jl:9-21) --- */
+typedef struct pps_lbfgs_params {
+  int32_t maxiters;        /* local_maxiters                                */
+  int32_t history;         /* m, 1..16                                      */
+  float abstol;            /* stop when max |P(x - grad f) - x| <= abstol   */
+  float armijo_c;          /* sufficient-decrease constant (1e-4)           */
+  float armijo_rho;        /* backtracking factor (0.5)                     */
+  int32_t max_backtracks;  /* step reductions per line search (20)          */
+} pps_lbfgs_params;
+void pps_lbfgs_params_default(pps_lbfgs_params* p);
+/* Projected L-BFGS from every particle's best position (objectives with an
+ * analytic gradient: Rosenbrock, Rastrigin, Sphere and the 2-D functions).
+ * out_x: N x d AoS (may be NULL), out_f: N values (non-finite -> +Inf,
+ * src/lbfgs.jl:20; may be NULL), best_x/best_f/best_idx: the argmin (lowest
+ * index; any may be NULL).  The swarm state is not modified. */
+int pps_swarm_lbfgs(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x,
+                    float* out_f, float* best_x, float* best_f,
+                    int64_t* best_idx);
+
 /* ---- host FP64 re-evaluation -------------------------------------------- */
 int pps_eval_f64(const pps_problem* prob, int64_t n, const double* x_aos,
                  double* f);

@@ -39,6 +39,18 @@ class _Prob(C.Structure):
     ]
 
 
+class _LbfgsParams(C.Structure):
+    _fields_ = [("maxiters", C.c_int32), ("history", C.c_int32), ("abstol", C.c_float),
+                ("armijo_c", C.c_float), ("armijo_rho", C.c_float),
+                ("max_backtracks", C.c_int32)]
+
+
+def lbfgs_params(maxiters=100, history=10, abstol=0.0, armijo_c=1e-4, armijo_rho=0.5,
+                 max_backtracks=20):
+    return _LbfgsParams(int(maxiters), int(history), float(abstol), float(armijo_c),
+                        float(armijo_rho), int(max_backtracks))
+
+
 _lib = None
 
 
@@ -76,6 +88,10 @@ def lib():
         L.ora_move.argtypes = [C.POINTER(_Prob), fp, fp, C.c_float, C.c_float, C.c_float,
                                C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]
         L.ora_num_threads.restype = C.c_int
+        L.ora_lbfgs_one.argtypes = [C.POINTER(_Prob), C.POINTER(_LbfgsParams), fp]
+        L.ora_lbfgs_one.restype = C.c_float
+        L.ora_lbfgs_run.argtypes = [C.POINTER(_Prob), C.POINTER(_LbfgsParams), C.c_int64, fp, fp,
+                                    fp, C.c_int]
         _lib = L
     return _lib
 
@@ -241,3 +257,24 @@ def move(prob: Problem, particle, gb, w, c1, c2, i, t, epoch, seed):
     pc = prob.c()
     lib().ora_move(C.byref(pc), _fp(p), _fp(g), w, c1, c2, i, t, epoch, seed)
     return p
+
+
+def lbfgs_one(prob: Problem, x0, params=None):
+    """Projected L-BFGS from x0; returns (x, f)."""
+    x = np.zeros(MAXD, np.float32)
+    x[: prob.d] = np.asarray(x0, np.float32)
+    p = prob.c()
+    lp = params or lbfgs_params()
+    f = lib().ora_lbfgs_one(C.byref(p), C.byref(lp), _fp(x))
+    return x[: prob.d].copy(), np.float32(f)
+
+
+def lbfgs_swarm(swarm: "Swarm", params=None):
+    """L-BFGS from every particle's best position: (x n x d, f n)."""
+    n, d = swarm.n, swarm.d
+    ox = np.zeros((n, d), np.float32)
+    of = np.zeros(n, np.float32)
+    lp = params or lbfgs_params()
+    lib().ora_lbfgs_run(C.byref(swarm._p), C.byref(lp), n, _fp(swarm.parts), _fp(ox), _fp(of),
+                        swarm.nthreads)
+    return ox, of

@@ -48,4 +48,12 @@ hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t);
 hipError_t launch_set_gbest(const SwarmDev&, int d, uint32_t par, const float* pos_dev,
                             float cost, hipStream_t);
 
+// Projected L-BFGS from every particle's best position into out_x (SoA,
+// d x npad) and out_f (n).
+bool lbfgs_supported(int obj, int d);
+hipError_t launch_lbfgs(const SwarmDev&, const ProbArgs&, const LbfgsArgs&, float* out_x,
+                        float* out_f, hipStream_t);
+// Lowest-index argmin key of vals[0..n) (NaN never wins) into *out_key.
+hipError_t launch_argmin_f32(const float* vals, int64_t n, uint64_t* out_key, hipStream_t);
+
 }  // namespace pps

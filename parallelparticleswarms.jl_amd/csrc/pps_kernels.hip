@@ -72,6 +72,19 @@ hipError_t launch_set_gbest(const SwarmDev& S, int d, uint32_t par, const float*
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(1024) void k_argmin_f32(const float* v, int64_t n, uint64_t* out) {
+  __shared__ uint64_t s_red[16];
+  uint64_t k = KEY_NONE_;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) k = umin64(k, make_key(v[i], (uint32_t)i));
+  k = block_min_u64_<1024>(k, s_red);
+  if (threadIdx.x == 0) *out = k;
+}
+
+hipError_t launch_argmin_f32(const float* vals, int64_t n, uint64_t* out_key, hipStream_t st) {
+  hipLaunchKernelGGL(k_argmin_f32, dim3(1), dim3(1024), 0, st, vals, n, out_key);
+  return hipGetLastError();
+}
+
 const KernelSet* kernels_rosenbrock(int d);
 const KernelSet* kernels_rastrigin(int d);
 const KernelSet* kernels_sphere(int d);

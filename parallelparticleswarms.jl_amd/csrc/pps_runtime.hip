@@ -105,6 +105,10 @@ struct pps_swarm {
   int32_t g_T = -1;
   float g_w = 0.0f, g_wdamp = 0.0f;
   StepArgs g_args{};
+  // Local-search outputs (allocated on first use): x (d x npad), f, argmin.
+  float* lx = nullptr;
+  float* lf = nullptr;
+  uint64_t* lkey = nullptr;
 };
 
 #define HIPCHK(expr)                                                   \
@@ -501,7 +505,66 @@ int pps_swarm_free(pps_swarm* s) {
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
   if (s->mem) hipFree(s->mem);
+  if (s->lx) hipFree(s->lx);
   delete s;
+  return PPS_OK;
+}
+
+void pps_lbfgs_params_default(pps_lbfgs_params* p) {
+  if (!p) return;
+  p->maxiters = 100;
+  p->history = 10;
+  p->abstol = 0.0f;
+  p->armijo_c = 1e-4f;
+  p->armijo_rho = 0.5f;
+  p->max_backtracks = 20;
+}
+
+int pps_swarm_lbfgs(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x, float* out_f,
+                    float* best_x, float* best_f, int64_t* best_idx) {
+  if (!s || !lp) return fail(PPS_E_INVALID, "NULL argument");
+  if (!lbfgs_supported(s->P.obj, s->P.d))
+    return fail(PPS_E_INVALID, "no analytic gradient for objective %d at d=%d", s->P.obj, s->P.d);
+  if (lp->history < 1 || lp->history > 16) return fail(PPS_E_INVALID, "history must be 1..16");
+  if (lp->maxiters < 0 || lp->max_backtracks < 0) return fail(PPS_E_INVALID, "negative iteration count");
+  if (!(lp->armijo_rho > 0.0f && lp->armijo_rho < 1.0f)) return fail(PPS_E_INVALID, "armijo_rho must be in (0,1)");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  const int d = s->P.d;
+  const int64_t n = s->prm.n, npad = s->S.npad;
+  hipStream_t st = s->ctx->stream;
+  if (!s->lx) {
+    const size_t bx = align256(sizeof(float) * d * npad), bf = align256(sizeof(float) * npad);
+    void* m = nullptr;
+    hipError_t e = hipMalloc(&m, bx + bf + 256);
+    if (e != hipSuccess) return fail(PPS_E_NOMEM, "hipMalloc of local-search buffers failed");
+    s->lx = (float*)m;
+    s->lf = (float*)((char*)m + bx);
+    s->lkey = (uint64_t*)((char*)m + bx + bf);
+  }
+  LbfgsArgs A{lp->maxiters, lp->history, lp->abstol, lp->armijo_c, lp->armijo_rho,
+              lp->max_backtracks};
+  HIPCHK(hipEventRecord(s->ev0, st));
+  HIPCHK(launch_lbfgs(s->S, s->P, A, s->lx, s->lf, st));
+  HIPCHK(launch_argmin_f32(s->lf, n, s->lkey, st));
+  HIPCHK(hipEventRecord(s->ev1, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
+  uint64_t key = 0;
+  HIPCHK(hipMemcpy(&key, s->lkey, sizeof key, hipMemcpyDeviceToHost));
+  const int64_t bi = (int64_t)(uint32_t)key;
+  if (best_idx) *best_idx = bi;
+  if (best_f) HIPCHK(hipMemcpy(best_f, s->lf + bi, sizeof(float), hipMemcpyDeviceToHost));
+  if (best_x)
+    HIPCHK(hipMemcpy2D(best_x, sizeof(float), s->lx + bi, sizeof(float) * npad, sizeof(float), d,
+                       hipMemcpyDeviceToHost));
+  if (out_f) HIPCHK(hipMemcpy(out_f, s->lf, sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (out_x) {
+    std::vector<float> tmp((size_t)d * n);
+    HIPCHK(hipMemcpy2D(tmp.data(), sizeof(float) * n, s->lx, sizeof(float) * npad,
+                       sizeof(float) * n, d, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i)
+      for (int j = 0; j < d; ++j) out_x[i * d + j] = tmp[(size_t)j * n + i];
+  }
   return PPS_OK;
 }
 

@@ -83,4 +83,14 @@ struct SwarmDev {
   int32_t block;
 };
 
+// Local-search (projected L-BFGS) settings passed to the kernel.
+struct LbfgsArgs {
+  int32_t maxiters;
+  int32_t history;
+  float abstol;
+  float armijo_c;
+  float armijo_rho;
+  int32_t max_backtracks;
+};
+
 }  // namespace pps

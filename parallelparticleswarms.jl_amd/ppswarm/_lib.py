@@ -54,6 +54,12 @@ class pps_swarm_params(C.Structure):
     ]
 
 
+class pps_lbfgs_params(C.Structure):
+    _fields_ = [("maxiters", C.c_int32), ("history", C.c_int32), ("abstol", C.c_float),
+                ("armijo_c", C.c_float), ("armijo_rho", C.c_float),
+                ("max_backtracks", C.c_int32)]
+
+
 _lib = None
 
 
@@ -92,6 +98,8 @@ def lib():
     _sig(L, "pps_swarm_elapsed_ms", [vp, fp])
     _sig(L, "pps_swarm_free", [vp])
     _sig(L, "pps_eval_f64", [C.POINTER(pps_problem), C.c_int64, dp, dp])
+    _sig(L, "pps_lbfgs_params_default", [C.POINTER(pps_lbfgs_params)], None)
+    _sig(L, "pps_swarm_lbfgs", [vp, C.POINTER(pps_lbfgs_params), fp, fp, fp, fp, i64p])
     _lib = L
     return L
 

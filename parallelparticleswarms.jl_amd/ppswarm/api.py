@@ -256,9 +256,15 @@ class ParallelSyncPSOKernel:
 
 @dataclass
 class SimpleLBFGS:
-    """Projected L-BFGS local search (history m, Armijo backtracking)."""
+    """Projected L-BFGS local search: history ``threshold`` (the reference's
+    history size, SimpleOptimization.__get_threshold), Armijo backtracking
+    with constant ``armijo_c``, factor ``armijo_rho``, at most
+    ``max_backtracks`` reductions per search."""
 
     threshold: int = 10
+    armijo_c: float = 1e-4
+    armijo_rho: float = 0.5
+    max_backtracks: int = 20
 
 
 @dataclass
@@ -355,6 +361,23 @@ class Swarm:
         a = [np.ascontiguousarray(q, np.float32) for q in (x, v, best, best_cost)]
         L.check(L.lib().pps_swarm_set_state(self.h, *[q.ctypes.data_as(L.fp) for q in a],
                                             int(it)))
+
+    def lbfgs(self, maxiters=100, history=10, abstol=0.0, armijo_c=1e-4, armijo_rho=0.5,
+              max_backtracks=20, want_all=True):
+        """Projected L-BFGS from every particle's best position.  Returns
+        (x n x d or None, f n or None, best_x, best_f, best_idx)."""
+        lp = L.pps_lbfgs_params(int(maxiters), int(history), float(abstol), float(armijo_c),
+                                float(armijo_rho), int(max_backtracks))
+        ox = np.zeros((self.n, self.d), np.float32) if want_all else None
+        of = np.zeros(self.n, np.float32) if want_all else None
+        bx = np.zeros(self.d, np.float32)
+        bf = C.c_float()
+        bi = C.c_int64()
+        L.check(L.lib().pps_swarm_lbfgs(
+            self.h, C.byref(lp), ox.ctypes.data_as(L.fp) if want_all else None,
+            of.ctypes.data_as(L.fp) if want_all else None, bx.ctypes.data_as(L.fp), C.byref(bf),
+            C.byref(bi)))
+        return ox, of, bx, float(bf.value), int(bi.value)
 
     def elapsed_ms(self) -> float:
         ms = C.c_float()

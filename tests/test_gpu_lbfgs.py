@@ -1,0 +1,97 @@
+"""HybridPSO local phase on the GPU: projected L-BFGS kernel vs the oracle
+(bit for bit: positions and values of every particle, argmin index), and the
+reference's known-answer hybrid tests (test/lbfgs.jl:38-72) through
+solve(prob, HybridPSO(...))."""
+import numpy as np
+import pytest
+
+import pps_oracle as O
+import ppswarm as P
+from parity_util import make_pair, note
+from ppswarm import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_both(g, o, maxiters, history=10, abstol=0.0):
+    gx, gf, bx, bf, bi = g.lbfgs(maxiters=maxiters, history=history, abstol=abstol)
+    ox, of = O.lbfgs_swarm(o, O.lbfgs_params(maxiters=maxiters, history=history, abstol=abstol))
+    return gx, gf, bx, bf, bi, ox, of
+
+
+@pytest.mark.parametrize("obj,d,n,lb,ub", [
+    (O.RASTRIGIN, 16, 4096, -5.12, 5.12),   # G = 16, the benchmark layout
+    (O.ROSENBROCK, 10, 2000, -5.0, 5.0),    # G = 16 with 6 padding lanes
+    (O.SPHERE, 5, 1000, -1.0, 2.0),         # G = 8
+    (O.RASTRIGIN, 64, 256, -5.12, 5.12),    # G = 64, one particle per wave
+    (O.ROSENBROCK, 3, 999, -1.0, 10.0),     # G = 4
+    (O.BEALE, 2, 500, -4.5, 4.5),
+    (O.HIMMELBLAU, 2, 500, -5.0, 5.0),
+    (O.BOUNDARY_QUADRATIC, 1, 300, 0.0, 2.0),  # G = 1
+])
+def test_lbfgs_matches_oracle(obj, d, n, lb, ub):
+    note(f"lbfgs parity obj={obj} d={d} n={n}")
+    g, o = make_pair(obj, d, n, seed=600 + d, lb=lb, ub=ub, launch=L.LAUNCH_EAGER)
+    g.run(5)
+    o.run_sync(5)
+    gx, gf, bx, bf, bi, ox, of = _run_both(g, o, maxiters=40)
+    np.testing.assert_array_equal(gx, ox)
+    np.testing.assert_array_equal(gf, of)
+    i = int(np.argmin(of))
+    assert bi == i and bf == of[i]
+    np.testing.assert_array_equal(bx, ox[i])
+    # the swarm itself is untouched
+    x, v, b, bc, it = g.state()
+    np.testing.assert_array_equal(b, o.best)
+
+
+def test_lbfgs_c2_shape_matches_oracle():
+    # 16-D Rastrigin, 2^16 particles, 200 sync iterations, then L-BFGS m=10 x 100
+    note("lbfgs C2 shape")
+    g, o = make_pair(O.RASTRIGIN, 16, 1 << 16, seed=2, launch=L.LAUNCH_GRAPH)
+    g.run(200)
+    o.run_sync(200)
+    gx, gf, bx, bf, bi, ox, of = _run_both(g, o, maxiters=100, history=10)
+    np.testing.assert_array_equal(gf, of)
+    np.testing.assert_array_equal(gx, ox)
+    assert bi == int(np.argmin(of))
+    # the local phase never makes a particle worse than its PSO best
+    assert (of <= o.best_cost + 1e-4).all()
+
+
+KNOWN = [
+    (P.Rosen2, [1.0, 100.0], [0.0, 0.0], None, [1, 1], 0.0),
+    (P.Rosen2, [1.0, 100.0], [-1.2, 1.0], None, [1, 1], 0.0),
+    (P.Rosen2, [1.0, 100.0], [0.0, 0.0], (-2.0, 2.0), [1, 1], 0.0),
+    (P.Beale, None, [1.0, 1.0], None, [3, 0.5], 0.0),
+    (P.Booth, None, [1.0, 1.0], None, [1, 3], 0.0),
+    (P.Himmelblau, None, [1.0, 1.0], None, None, 0.0),
+    (P.Quadratic, [1000.0], [4.0, 4.0], (-5.0, 5.0), [1, -2], 0.0),
+]
+
+
+@pytest.mark.parametrize("f,p,u0,bounds,ustar,fstar", KNOWN)
+def test_hybrid_known_answers(f, p, u0, bounds, ustar, fstar):
+    # test/lbfgs.jl:5-65: HybridPSO(pso=ParallelSyncPSOKernel(64), SimpleLBFGS()),
+    # maxiters=50, local_maxiters=200, abstol=1e-10 (unbounded problems get
+    # the box [-10, 10]: this backend samples the initial swarm in the bounds)
+    note(f"hybrid known answer {f().name} u0={u0}")
+    lb, ub = bounds or (-10.0, 10.0)
+    prob = P.OptimizationProblem(f(), np.asarray(u0, np.float32), p, lb=np.full(2, lb),
+                                 ub=np.full(2, ub))
+    sol = P.solve(prob, P.HybridPSO(pso=P.ParallelSyncPSOKernel(64)), maxiters=50,
+                  local_maxiters=200, abstol=1e-10, seed=7)
+    if ustar is not None:
+        np.testing.assert_allclose(sol.u, ustar, atol=1e-3)
+    assert sol.objective < 1e-6
+    assert (sol.u >= lb).all() and (sol.u <= ub).all()
+
+
+def test_hybrid_boundary_minimum():
+    # test/lbfgs.jl:67-72: (x+1)^2 on [0, 2] -> u = 0, objective = 1
+    note("hybrid boundary case")
+    prob = P.OptimizationProblem(P.BoundaryQuadratic(), np.array([1.5], np.float32),
+                                 lb=[0.0], ub=[2.0])
+    sol = P.solve(prob, P.HybridPSO(pso=P.ParallelSyncPSOKernel(64)), maxiters=20,
+                  local_maxiters=50, seed=3)
+    assert abs(sol.u[0]) < 1e-6 and abs(sol.objective - 1.0) < 1e-6
