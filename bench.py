@@ -15,6 +15,14 @@ Also reported on the same line:
   async         ParallelPSOKernel(global_update=false) at the same shape.
   c0            3-D Rosenbrock, 1000 particles, 500 iterations (one
                 resident workgroup): time per solve and per iteration.
+  batched       the 4096-problem served mix solved in packed launches
+                (all submitted at once): solves/s and particle-iters/s, next
+                to the plain per-problem launch loop (init + one launch per
+                iteration, problem after problem) timed on the first 256
+                problems of the same mix, whose results must equal the packed
+                results bit for bit.
+  served        the same mix arriving as a Poisson stream at a fixed rate
+                below saturation: solves/s and p50/p99 request latency.
   cpu_baseline  the CPU oracle (reference-faithful AoS restatement, OpenMP
                 over particles + serial argmin per iteration) on a bounded
                 sample of the same workload on this host.
@@ -90,6 +98,55 @@ def cpu_baseline(sample_iters=200):
             "sample": f"sync PSO, 10-D Rastrigin + 10-D Rosenbrock, 2^20 particles each, "
                       f"{sample_iters} timed iterations each (after 1 warm-up), "
                       f"{total_s:.1f} s of CPU time"}
+
+
+def c4_mix(P, L, ctx, count=4096, plain_sample=256, served_qps=None):
+    reqs, _ = P.served_mix(count, seed=2024)
+    pi = sum(q.n * q.maxiters for q in reqs)
+    P.batch_solve(reqs[:64])  # warm-up (module load, arena)
+    t0 = time.perf_counter()
+    res, ms = P.batch_solve(reqs)
+    wall = time.perf_counter() - t0
+    # plain per-problem launch loop on a sample of the same mix
+    t0 = time.perf_counter()
+    plain = [P.solve_alone(q, L.LAUNCH_EAGER) for q in reqs[:plain_sample]]
+    pwall = time.perf_counter() - t0
+    mismatch = sum(1 for a, b in zip(res, plain)
+                   if not (np.array_equal(a.u, b.u) and a.cost_f32 == b.cost_f32))
+    max_df64 = max(abs(a.cost_f64 - b.cost_f64) for a, b in zip(res, plain))
+    plain_rate = plain_sample / pwall
+    batched = {"solves_per_s": count / wall, "particle_iters_per_s": pi / wall,
+               "wall_s": wall, "problems": count,
+               "plain_loop_solves_per_s": plain_rate,
+               "plain_loop_sample": plain_sample,
+               "speedup_vs_plain_loop": (count / wall) / plain_rate,
+               "bitwise_mismatches_vs_plain": mismatch,
+               "max_abs_f64_diff_vs_plain": max_df64,
+               "max_abs_f32_vs_f64": max(abs(r.cost_f64 - r.cost_f32) for r in res)}
+    # served: Poisson arrivals at half the packed throughput
+    qps = served_qps or 0.5 * count / wall
+    reqs2, arr = P.served_mix(count, seed=2024, qps=qps)
+    with P.Server(ctx) as sv:
+        t0 = time.perf_counter()
+        tickets = []
+        for q, a in zip(reqs2, arr):
+            d = a - (time.perf_counter() - t0)
+            if d > 0.0005:
+                time.sleep(d - 0.0003)
+            while time.perf_counter() - t0 < a:
+                pass
+            tickets.append(sv.submit(q))
+        sres = [sv.wait(t) for t in tickets]
+        swall = time.perf_counter() - t0
+        st = sv.stats()
+    lat = np.array([r.latency for r in sres]) * 1e3
+    served = {"qps_offered": qps, "solves_per_s": count / swall,
+              "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+              "mean_ms": float(lat.mean()), "launches": st["launches"],
+              "peak_resident_bytes": st["peak_bytes"],
+              "bitwise_equal_to_batched": all(np.array_equal(a.u, b.u) and a.cost_f32 == b.cost_f32
+                                              for a, b in zip(res, sres))}
+    return batched, served
 
 
 def main():
@@ -186,6 +243,10 @@ def main():
                      "device_us_per_iter": float(np.median(dms[1:]) * 1e3 / 500),
                      "particle_iters_per_s": 1000 * 500 / float(np.median(times[1:]))}
         log("[bench] c0 done")
+
+    if not args.no_extras:
+        out["batched"], out["served"] = c4_mix(P, L, ctx)
+        log("[bench] c4 done")
 
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -209,6 +209,55 @@ int pps_swarm_lbfgs(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x,
 int pps_eval_f64(const pps_problem* prob, int64_t n, const double* x_aos,
                  double* f);
 
+/* ---- served / batched solving (no reference counterpart) --------------- */
+/* Independent sync-PSO problems (ParallelSyncPSOKernel semantics with the
+ * best-position global best) of Rosenbrock or Rastrigin at d in 1..32.  Each problem's result equals what pps_swarm_init +
+ * pps_swarm_run compute for it alone, bit for bit. */
+typedef struct pps_server_config {
+  int64_t hbm_budget;      /* bytes of swarm state admitted at once; 0 = 80%
+                              of free HBM at create time, at most 16 GiB    */
+  int32_t max_active;      /* problems resident at once (0 = 4096)         */
+  int32_t slice_us;        /* device time per launch slice (0 = 1000 us)   */
+  int32_t reserved;
+} pps_server_config;
+
+typedef struct pps_result {
+  float gbest_cost;        /* FP32 cost of the final global best            */
+  double gbest_cost_f64;   /* same position re-evaluated in FP64            */
+  int64_t gbest_idx;
+  int32_t d;
+  float gbest_pos[64];
+  double t_submit_s, t_admit_s, t_done_s; /* host steady-clock seconds      */
+} pps_result;
+
+/* Start a server (its own host thread and streams) on ctx's device. */
+int pps_server_create(pps_ctx* ctx, const pps_server_config* cfg,
+                      pps_server** out);
+/* Queue one problem; arrays are copied during the call.  Problems whose state
+ * exceeds the whole HBM budget are refused with PPS_E_NOMEM. */
+int pps_server_submit(pps_server* sv, const pps_problem* prob,
+                      const pps_swarm_params* params, int32_t maxiters,
+                      float w, float wdamp, uint64_t* ticket);
+/* PPS_OK with the result, PPS_E_PENDING while queued or running. */
+int pps_server_poll(pps_server* sv, uint64_t ticket, pps_result* out);
+/* Block until done (timeout_s <= 0: no timeout; PPS_E_TIMEOUT otherwise). */
+int pps_server_wait(pps_server* sv, uint64_t ticket, double timeout_s,
+                    pps_result* out);
+/* Cancel a request that is still queued (PPS_E_STATE once admitted). */
+int pps_server_cancel(pps_server* sv, uint64_t ticket);
+/* Counters: submitted, completed, packed launches, peak admitted bytes. */
+int pps_server_stats(pps_server* sv, int64_t* submitted, int64_t* completed,
+                     int64_t* launches, int64_t* peak_bytes);
+/* Finish the running problems, cancel queued ones, free everything. */
+int pps_server_destroy(pps_server* sv);
+
+/* Solve `count` independent problems through a temporary server (all
+ * submitted at once); results in order; wall time in elapsed_ms. */
+int pps_batch_solve(pps_ctx* ctx, int32_t count, const pps_problem* probs,
+                    const pps_swarm_params* params, const int32_t* maxiters,
+                    const float* w, const float* wdamp, pps_result* out,
+                    float* elapsed_ms);
+
 #ifdef __cplusplus
 }
 #endif

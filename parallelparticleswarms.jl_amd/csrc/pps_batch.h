@@ -8,7 +8,8 @@
 
 namespace pps {
 
-constexpr int BATCH_BLOCK = 256;
+constexpr int BATCH_BLOCK = 256;         // workgroup of a small problem
+constexpr int BATCH_BLOCK_LARGE = 1024;  // workgroup of a large problem
 
 // One admitted problem.  Written by the host at admission, then owned by the
 // device until the problem finishes (remaining == 0).
@@ -44,10 +45,15 @@ struct BatchOut {
 };
 
 bool batch_supported(int obj, int d);
-// Problems probs[ids[0..count)] each run up to `chunk` iterations in one
-// workgroup; if `out` is non-null each problem's result and done flag are
-// published to out[id] when its remaining count reaches zero.
-hipError_t launch_batch(BatchProb* probs, const int32_t* ids, int32_t count, int32_t chunk,
-                        BatchOut* out, hipStream_t st);
+// Problems probs[ids[0..count)] each run in one workgroup until finished, or
+// `chunk` iterations, or `budget_ticks` of the device wall clock (checked
+// after every iteration).  When fresh_in[id] == launch_no the workgroup first
+// copies its descriptor fresh[id] (host-visible memory) -> probs[id].  If
+// `out` is non-null each problem's result and done flag are published to
+// out[id] when its remaining count reaches zero.
+// `block` selects the workgroup size (BATCH_BLOCK or BATCH_BLOCK_LARGE).
+hipError_t launch_batch(BatchProb* probs, const BatchProb* fresh, const uint32_t* fresh_in,
+                        uint32_t launch_no, const int32_t* ids, int32_t count, int32_t chunk,
+                        uint64_t budget_ticks, BatchOut* out, int block, hipStream_t st);
 
 }  // namespace pps

@@ -9,7 +9,9 @@
 // problem's result depends only on its own seed, particle indices and
 // iteration numbers, so it is the same whether it runs alone (the per-problem
 // launch loop of pps_swarm_run) or packed with any other problems, in any
-// chunking.
+// chunking.  A launch is time-sliced: each workgroup stops its problem after
+// the first iteration that ends past a device-clock deadline, so the
+// workgroups of one launch finish together whatever their problem sizes.
 #include <hip/hip_runtime.h>
 
 #include "pps_batch.h"
@@ -34,50 +36,89 @@ __device__ __forceinline__ uint64_t block_min(uint64_t k, uint64_t* s_red) {
   return r;
 }
 
-template <int D>
-__device__ __forceinline__ void ld(float (&r)[D], const float* b, int64_t np, int64_t i) {
-#pragma unroll
-  for (int j = 0; j < D; ++j) r[j] = b[(int64_t)j * np + i];
-}
-template <int D>
-__device__ __forceinline__ void st(const float (&r)[D], float* b, int64_t np, int64_t i) {
-#pragma unroll
-  for (int j = 0; j < D; ++j) b[(int64_t)j * np + i] = r[j];
-}
+// Per-problem constants staged in LDS (runtime-indexed by coordinate).
+struct ProbLds {
+  float lb[PPS_BMAXD], ub[PPS_BMAXD], shift[PPS_BMAXD], gb[PPS_BMAXD];
+};
 
-// Run up to `chunk` iterations of one problem in this workgroup.
-template <int D, int OBJ, int BLOCK>
-__device__ void run_problem(BatchProb* B, int32_t chunk, uint64_t* s_red, float* s_gb) {
-  const ProbArgs& P = B->P;
+// Objective accumulated coordinate by coordinate, in the order of
+// objective<>() (pps_device.h): Rastrigin adds z_j^2 - 10 cos(2 pi z_j) to
+// 10 d for j = 0, 1, ...; Rosenbrock adds term(z_{j-1}, z_j) for j = 1, 2, ...
+template <int OBJ>
+struct Accum {
+  float res, zprev;
+  __device__ __forceinline__ void start(int d) {
+    res = OBJ == PPS_OBJ_RASTRIGIN ? 10.0f * (float)d : 0.0f;
+    zprev = 0.0f;
+  }
+  __device__ __forceinline__ void add(int j, float z, float p1, float p2) {
+    if constexpr (OBJ == PPS_OBJ_RASTRIGIN) {
+      res = res + (z * z - 10.0f * cos2pi(z));
+    } else {
+      if (j > 0) {
+        const float dd = z - zprev * zprev;
+        const float e = p1 - zprev;
+        res = res + (p2 * (dd * dd) + e * e);
+      }
+      zprev = z;
+    }
+  }
+};
+
+// Run iterations of one problem in this workgroup until it is finished, or
+// `chunk` iterations are done, or the device clock passes `deadline` (checked
+// by thread 0 once per iteration and broadcast through LDS, so the whole
+// workgroup stops after the same iteration).
+//
+// The swarm is streamed one coordinate pair at a time (one Philox call per
+// pair, as in velocity_position()), so register use does not grow with d; a
+// particle whose cost improves re-reads its new position to store its best.
+template <int OBJ, int BLOCK, int PPT>
+__device__ void run_problem(BatchProb* B, int32_t chunk, uint64_t deadline, uint64_t* s_red,
+                            ProbLds* L, int* s_stop) {
+  const int d = B->P.d;
   const int64_t n = B->n, np = B->npad;
   const uint32_t k0 = B->key0, k1 = B->key1;
-  float* X = B->x;
-  float* V = B->v;
-  float* PB = B->pb;
-  float* PC = B->pc;
+  const float p1 = B->P.par[0], p2 = B->P.par[1];
+  const bool has_shift = B->P.has_shift != 0;
+  float* __restrict__ X = B->x;
+  float* __restrict__ V = B->v;
+  float* __restrict__ PB = B->pb;
+  float* __restrict__ PC = B->pc;
+  if (threadIdx.x < d) {
+    L->lb[threadIdx.x] = B->P.lb[threadIdx.x];
+    L->ub[threadIdx.x] = B->P.ub[threadIdx.x];
+    L->shift[threadIdx.x] = has_shift ? B->P.shift[threadIdx.x] : 0.0f;
+  }
+  __syncthreads();
   if (!B->initialized) {
-    // uniform init (as k_init) and the initial global best
+    // uniform init (as init_position + objective) and the initial global best
     uint64_t mykey = KEY_NONE;
     for (int64_t i = threadIdx.x; i < n; i += BLOCK) {
-      float x[D], z[D];
-      init_position<D>(x, (uint32_t)i, 0u, k0, k1, D, P);
-#pragma unroll
-      for (int j = 0; j < D; ++j) z[j] = 0.0f;
-      const float c = objective<D, OBJ>(x, D, P);
-      st<D>(x, X, np, i);
-      st<D>(z, V, np, i);
-      st<D>(x, PB, np, i);
-      PC[i] = c;
-      mykey = umin64(mykey, make_key(c, (uint32_t)i));
+      Accum<OBJ> acc;
+      acc.start(d);
+      u4 r{};
+      for (int j = 0; j < d; ++j) {
+        if ((j & 3) == 0)
+          r = philox4x32_10(u4{(uint32_t)i, 0u, (uint32_t)(j >> 2) | (PPS_TAG_INIT << 16), 0u},
+                            k0, k1);
+        const uint32_t rr = (j & 3) == 0 ? r.x : (j & 3) == 1 ? r.y : (j & 3) == 2 ? r.z : r.w;
+        const float x = u01(rr) * (L->ub[j] - L->lb[j]) + L->lb[j];
+        X[(int64_t)j * np + i] = x;
+        V[(int64_t)j * np + i] = 0.0f;
+        PB[(int64_t)j * np + i] = x;
+        acc.add(j, has_shift ? x - L->shift[j] : x, p1, p2);
+      }
+      PC[i] = acc.res;
+      mykey = umin64(mykey, make_key(acc.res, (uint32_t)i));
     }
     const uint64_t bk = block_min<BLOCK>(mykey, s_red);
     if (mykey == bk) {
       const int64_t wi = (int64_t)(uint32_t)bk;
-#pragma unroll
-      for (int j = 0; j < D; ++j) s_gb[j] = PB[(int64_t)j * np + wi];
+      for (int j = 0; j < d; ++j) L->gb[j] = PB[(int64_t)j * np + wi];
     }
     __syncthreads();
-    if (threadIdx.x < D) B->gb[threadIdx.x] = s_gb[threadIdx.x];
+    if (threadIdx.x < d) B->gb[threadIdx.x] = L->gb[threadIdx.x];
     if (threadIdx.x == 0) {
       B->gb_cost = key_cost(bk);
       B->gb_idx = (uint32_t)bk;
@@ -87,124 +128,119 @@ __device__ void run_problem(BatchProb* B, int32_t chunk, uint64_t* s_red, float*
   }
   const int32_t steps = chunk < B->remaining ? chunk : B->remaining;
   if (steps <= 0) return;
-  if (threadIdx.x < D) s_gb[threadIdx.x] = B->gb[threadIdx.x];
+  if (threadIdx.x < d) L->gb[threadIdx.x] = B->gb[threadIdx.x];
   const uint32_t t0 = B->iter;
   float w = B->w;
   const float wdamp = B->wdamp, c1 = B->c1, c2 = B->c2;
+  if (threadIdx.x == 0) *s_stop = 0;
   __syncthreads();
   uint64_t bk = KEY_NONE;
-  if (n <= BLOCK) {
-    // swarm in registers for the whole chunk
-    const int64_t i = threadIdx.x;
-    const bool act = i < n;
-    float x[D], v[D], pb[D], pc = 0.0f;
+  int32_t done = 0;
+  while (done < steps) {
+    const uint32_t t = t0 + (uint32_t)(++done);
+    uint64_t mykey = KEY_NONE;
+    // PPT particles per thread in flight (i, i + BLOCK, ...): their loads of
+    // one coordinate pair are issued together, which hides memory latency
+    // when a workgroup streams a large swarm.
+    for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)PPT * BLOCK) {
+      Accum<OBJ> acc[PPT];
 #pragma unroll
-    for (int j = 0; j < D; ++j) x[j] = v[j] = pb[j] = 0.0f;
-    if (act) {
-      ld<D>(x, X, np, i);
-      ld<D>(v, V, np, i);
-      ld<D>(pb, PB, np, i);
-      pc = PC[i];
-    }
-    for (int32_t k = 1; k <= steps; ++k) {
-      float gb[D];
+      for (int q = 0; q < PPT; ++q) acc[q].start(d);
+      for (int k = 0; 2 * k < d; ++k) {
+        float x[PPT][2], v[PPT][2], pbv[PPT][2];
 #pragma unroll
-      for (int j = 0; j < D; ++j) gb[j] = s_gb[j];
-      uint64_t mykey = KEY_NONE;
-      if (act) {
-        velocity_position<D>(x, v, pb, gb, w, c1, c2, (uint32_t)i, t0 + (uint32_t)k, 0u, k0, k1,
-                             D, P);
-        const float c = objective<D, OBJ>(x, D, P);
-        if (c < pc) {
+        for (int q = 0; q < PPT; ++q) {
+          const int64_t i = i0 + (int64_t)q * BLOCK;
 #pragma unroll
-          for (int j = 0; j < D; ++j) pb[j] = x[j];
-          pc = c;
+          for (int h = 0; h < 2; ++h) {
+            const int j = 2 * k + h;
+            const int64_t o = (int64_t)j * np + i;
+            const bool ok = i < n && j < d;
+            x[q][h] = ok ? X[o] : 0.0f;
+            v[q][h] = ok ? V[o] : 0.0f;
+            pbv[q][h] = ok ? PB[o] : 0.0f;
+          }
         }
-        mykey = make_key(pc, (uint32_t)i);
-      }
-      bk = block_min<BLOCK>(mykey, s_red);
-      if (act && mykey == bk) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) s_gb[j] = pb[j];
-      }
-      __syncthreads();
-      w = w * wdamp;
-    }
-    if (act) {
-      st<D>(x, X, np, i);
-      st<D>(v, V, np, i);
-      st<D>(pb, PB, np, i);
-      PC[i] = pc;
-    }
-  } else {
-    // swarm streamed from the arena (L2/MALL resident for these sizes)
-    for (int32_t k = 1; k <= steps; ++k) {
-      float gb[D];
+        for (int q = 0; q < PPT; ++q) {
+          const int64_t i = i0 + (int64_t)q * BLOCK;
+          if (i >= n) continue;
+          const u4 r = philox4x32_10(u4{(uint32_t)i, t, (uint32_t)k | (PPS_TAG_STEP << 16), 0u}, k0, k1);
 #pragma unroll
-      for (int j = 0; j < D; ++j) gb[j] = s_gb[j];
-      uint64_t mykey = KEY_NONE;
-      for (int64_t i = threadIdx.x; i < n; i += BLOCK) {
-        float x[D], v[D], pb[D];
-        ld<D>(x, X, np, i);
-        ld<D>(v, V, np, i);
-        ld<D>(pb, PB, np, i);
+          for (int h = 0; h < 2; ++h) {
+            const int j = 2 * k + h;
+            if (j < d) {
+              const float r1 = u01(h == 0 ? r.x : r.z);
+              const float r2 = u01(h == 0 ? r.y : r.w);
+              const float a = w * v[q][h];
+              const float b = (c1 * r1) * (pbv[q][h] - x[q][h]);
+              const float c = (c2 * r2) * (L->gb[j] - x[q][h]);
+              const float vn = (a + b) + c;
+              float xn = x[q][h] + vn;
+              xn = jl_min(jl_max(xn, L->lb[j]), L->ub[j]);
+              const int64_t o = (int64_t)j * np + i;
+              X[o] = xn;
+              V[o] = vn;
+              acc[q].add(j, has_shift ? xn - L->shift[j] : xn, p1, p2);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        const int64_t i = i0 + (int64_t)q * BLOCK;
+        if (i >= n) continue;
         float pc = PC[i];
-        velocity_position<D>(x, v, pb, gb, w, c1, c2, (uint32_t)i, t0 + (uint32_t)k, 0u, k0, k1,
-                             D, P);
-        const float c = objective<D, OBJ>(x, D, P);
-        st<D>(x, X, np, i);
-        st<D>(v, V, np, i);
-        if (c < pc) {
-          st<D>(x, PB, np, i);
-          pc = c;
-          PC[i] = c;
+        if (acc[q].res < pc) {
+          for (int j = 0; j < d; ++j) PB[(int64_t)j * np + i] = X[(int64_t)j * np + i];
+          pc = acc[q].res;
+          PC[i] = pc;
         }
         mykey = umin64(mykey, make_key(pc, (uint32_t)i));
       }
-      bk = block_min<BLOCK>(mykey, s_red);
-      if (mykey == bk) {
-        const int64_t wi = (int64_t)(uint32_t)bk;
-#pragma unroll
-        for (int j = 0; j < D; ++j) s_gb[j] = PB[(int64_t)j * np + wi];
-      }
-      __syncthreads();
-      w = w * wdamp;
     }
+    bk = block_min<BLOCK>(mykey, s_red);
+    if (mykey == bk) {
+      const int64_t wi = (int64_t)(uint32_t)bk;
+      for (int j = 0; j < d; ++j) L->gb[j] = PB[(int64_t)j * np + wi];
+    }
+    if (threadIdx.x == 0) *s_stop = wall_clock64() > deadline;
+    __syncthreads();
+    w = w * wdamp;
+    if (*s_stop) break;
   }
-  if (threadIdx.x < D) B->gb[threadIdx.x] = s_gb[threadIdx.x];
+  if (threadIdx.x < d) B->gb[threadIdx.x] = L->gb[threadIdx.x];
   if (threadIdx.x == 0) {
     B->gb_cost = key_cost(bk);
     B->gb_idx = (uint32_t)bk;
-    B->iter = t0 + (uint32_t)steps;
-    B->remaining -= steps;
+    B->iter = t0 + (uint32_t)done;
+    B->remaining -= done;
     B->w = w;
   }
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_batch(BatchProb* probs, const int32_t* ids,
-                                                 int32_t chunk, BatchOut* out) {
+__global__ __launch_bounds__(BLOCK) void k_batch(BatchProb* probs, const BatchProb* fresh,
+                                                 const uint32_t* fresh_in, uint32_t launch_no,
+                                                 const int32_t* ids, int32_t chunk,
+                                                 uint64_t budget_ticks, BatchOut* out) {
   __shared__ uint64_t s_red[BLOCK / 64];
-  __shared__ float s_gb[PPS_BMAXD];
+  __shared__ ProbLds s_lds;
+  __shared__ int s_stop;
+  const uint64_t deadline = wall_clock64() + budget_ticks;
   const int32_t id = ids[blockIdx.x];
   BatchProb* B = probs + id;
-  const int sel = B->P.obj * 64 + B->P.d;
-  switch (sel) {
-#define PPS_CASE(OBJ, D) \
-  case OBJ * 64 + D: run_problem<D, OBJ, BLOCK>(B, chunk, s_red, s_gb); break;
-    PPS_CASE(PPS_OBJ_ROSENBROCK, 2)
-    PPS_CASE(PPS_OBJ_ROSENBROCK, 4)
-    PPS_CASE(PPS_OBJ_ROSENBROCK, 8)
-    PPS_CASE(PPS_OBJ_ROSENBROCK, 16)
-    PPS_CASE(PPS_OBJ_ROSENBROCK, 32)
-    PPS_CASE(PPS_OBJ_RASTRIGIN, 2)
-    PPS_CASE(PPS_OBJ_RASTRIGIN, 4)
-    PPS_CASE(PPS_OBJ_RASTRIGIN, 8)
-    PPS_CASE(PPS_OBJ_RASTRIGIN, 16)
-    PPS_CASE(PPS_OBJ_RASTRIGIN, 32)
-#undef PPS_CASE
-    default: break;
+  if (fresh && fresh_in[id] == launch_no) {
+    // newly admitted: pull the descriptor from host-visible memory
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(fresh + id);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(B);
+    for (int k = threadIdx.x; k < (int)(sizeof(BatchProb) / 4); k += BLOCK) dst[k] = src[k];
+    __syncthreads();
   }
+  if (B->P.obj == PPS_OBJ_RASTRIGIN)
+    run_problem<PPS_OBJ_RASTRIGIN, BLOCK, 4>(B, chunk, deadline, s_red, &s_lds, &s_stop);
+  else
+    run_problem<PPS_OBJ_ROSENBROCK, BLOCK, 4>(B, chunk, deadline, s_red, &s_lds, &s_stop);
   if (out) {
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -228,14 +264,19 @@ __global__ __launch_bounds__(BLOCK) void k_batch(BatchProb* probs, const int32_t
 
 bool batch_supported(int obj, int d) {
   if (obj != PPS_OBJ_ROSENBROCK && obj != PPS_OBJ_RASTRIGIN) return false;
-  return d == 2 || d == 4 || d == 8 || d == 16 || d == 32;
+  return d >= 1 && d <= PPS_BMAXD;
 }
 
-hipError_t launch_batch(BatchProb* probs, const int32_t* ids, int32_t count, int32_t chunk,
-                        BatchOut* out, hipStream_t st) {
+hipError_t launch_batch(BatchProb* probs, const BatchProb* fresh, const uint32_t* fresh_in,
+                        uint32_t launch_no, const int32_t* ids, int32_t count, int32_t chunk,
+                        uint64_t budget_ticks, BatchOut* out, int block, hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_batch<BATCH_BLOCK>, dim3(count), dim3(BATCH_BLOCK), 0, st, probs, ids,
-                     chunk, out);
+  if (block == BATCH_BLOCK_LARGE)
+    hipLaunchKernelGGL(k_batch<BATCH_BLOCK_LARGE>, dim3(count), dim3(BATCH_BLOCK_LARGE), 0, st,
+                       probs, fresh, fresh_in, launch_no, ids, chunk, budget_ticks, out);
+  else
+    hipLaunchKernelGGL(k_batch<BATCH_BLOCK>, dim3(count), dim3(BATCH_BLOCK), 0, st, probs, fresh,
+                       fresh_in, launch_no, ids, chunk, budget_ticks, out);
   return hipGetLastError();
 }
 

@@ -17,6 +17,8 @@ struct pps_ctx {
 
 namespace pps {
 
+// Message of the last failure on this thread (pps_last_error).
+extern thread_local std::string g_err;
 // Record a failure message for pps_last_error() and return `code`.
 int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what, const char* file, int line);

@@ -60,6 +60,17 @@ class pps_lbfgs_params(C.Structure):
                 ("max_backtracks", C.c_int32)]
 
 
+class pps_server_config(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("max_active", C.c_int32),
+                ("slice_us", C.c_int32), ("reserved", C.c_int32)]
+
+
+class pps_result(C.Structure):
+    _fields_ = [("gbest_cost", C.c_float), ("gbest_cost_f64", C.c_double),
+                ("gbest_idx", C.c_int64), ("d", C.c_int32), ("gbest_pos", C.c_float * 64),
+                ("t_submit_s", C.c_double), ("t_admit_s", C.c_double), ("t_done_s", C.c_double)]
+
+
 _lib = None
 
 
@@ -100,6 +111,17 @@ def lib():
     _sig(L, "pps_eval_f64", [C.POINTER(pps_problem), C.c_int64, dp, dp])
     _sig(L, "pps_lbfgs_params_default", [C.POINTER(pps_lbfgs_params)], None)
     _sig(L, "pps_swarm_lbfgs", [vp, C.POINTER(pps_lbfgs_params), fp, fp, fp, fp, i64p])
+    _sig(L, "pps_server_create", [vp, C.POINTER(pps_server_config), C.POINTER(vp)])
+    _sig(L, "pps_server_submit", [vp, C.POINTER(pps_problem), C.POINTER(pps_swarm_params),
+                                  C.c_int32, C.c_float, C.c_float, u64p])
+    _sig(L, "pps_server_poll", [vp, C.c_uint64, C.POINTER(pps_result)])
+    _sig(L, "pps_server_wait", [vp, C.c_uint64, C.c_double, C.POINTER(pps_result)])
+    _sig(L, "pps_server_cancel", [vp, C.c_uint64])
+    _sig(L, "pps_server_stats", [vp, i64p, i64p, i64p, i64p])
+    _sig(L, "pps_server_destroy", [vp])
+    _sig(L, "pps_batch_solve", [vp, C.c_int32, C.POINTER(pps_problem),
+                                C.POINTER(pps_swarm_params), C.POINTER(C.c_int32), fp, fp,
+                                C.POINTER(pps_result), fp])
     _lib = L
     return L
 

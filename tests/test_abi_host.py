@@ -97,3 +97,23 @@ def test_seed_sequence_is_reproducible(pps):
     pps.seed_(123)
     b = [api._next_seed() for _ in range(3)]
     assert a == b and len(set(a)) == 3
+
+
+def test_served_mix_generator(pps):
+    reqs, arr = pps.served_mix(2000, seed=11, qps=1000.0)
+    again, arr2 = pps.served_mix(2000, seed=11, qps=1000.0)
+    assert [q.seed for q in reqs] == [q.seed for q in again]
+    np.testing.assert_array_equal(arr, arr2)
+    ds = np.array([q.prob.d for q in reqs])
+    ns = np.array([q.n for q in reqs])
+    ts = np.array([q.maxiters for q in reqs])
+    assert set(ds) == {2, 4, 8, 16, 32}
+    assert ns.min() >= 256 and ns.max() <= 16384
+    # log-uniform: the median is near sqrt(256 * 16384) = 2048
+    assert 1500 < np.median(ns) < 2800
+    assert ts.min() >= 100 and ts.max() <= 1000
+    fams = {q.prob.f.name for q in reqs}
+    assert fams == {"rosenbrock", "rastrigin"}
+    assert all((np.abs(q.prob.shift) <= 1).all() for q in reqs)
+    # Poisson arrivals: mean gap 1/qps
+    assert abs(np.diff(arr).mean() - 1e-3) < 1e-4
