@@ -23,6 +23,11 @@ Also reported on the same line:
                 results bit for bit.
   served        the same mix arriving as a Poisson stream at a fixed rate
                 below saturation: solves/s and p50/p99 request latency.
+  hybrid        HybridPSO: sync PSO, 16-D Rastrigin, 2^16 particles, 200
+                iterations, then projected L-BFGS (m = 10, 100 iterations,
+                Armijo) from every particle; final minimum re-evaluated in FP64.
+  ode           Lotka-Volterra parameter estimation: 2^14 particles, 300
+                iterations, in-kernel FP32 RK4 (dt = 0.01, 101 observations).
   cpu_baseline  the CPU oracle (reference-faithful AoS restatement, OpenMP
                 over particles + serial argmin per iteration) on a bounded
                 sample of the same workload on this host.
@@ -149,6 +154,46 @@ def c4_mix(P, L, ctx, count=4096, plain_sample=256, served_qps=None):
     return batched, served
 
 
+def hybrid_c2(P, L, ctx):
+    d, n = 16, 1 << 16
+    prob = P.OptimizationProblem(P.Rastrigin(), np.zeros(d, np.float32), lb=np.full(d, -5.12),
+                                 ub=np.full(d, 5.12))
+    sw = P.Swarm(ctx, prob, P.swarm_params(n, L.MODE_SYNC, seed=16, launch=L.LAUNCH_GRAPH))
+    sw.run(5)  # warm-up (graph capture)
+    sw.reinit()
+    t0 = time.perf_counter()
+    sw.run(200)
+    pso_ms = sw.elapsed_ms()
+    g = sw.gbest()
+    _, _, bx, bf, bi = sw.lbfgs(maxiters=100, history=10, want_all=False)
+    wall = time.perf_counter() - t0
+    lb_ms = sw.elapsed_ms()
+    best = min(float(P.eval_f64(prob, g.position[None, :])[0]),
+               float(P.eval_f64(prob, bx[None, :])[0]))
+    sw.free()
+    return {"wall_s": wall, "pso_device_ms": pso_ms, "lbfgs_device_ms": lb_ms,
+            "pso_particle_iters_per_s": n * 200 / (pso_ms / 1e3),
+            "lbfgs_particle_iters_per_s": n * 100 / (lb_ms / 1e3),
+            "pso_best_f32": g.cost, "lbfgs_best_f32": bf, "final_objective_f64": best}
+
+
+def ode_c3(P):
+    import json as _json
+    doc = _json.load(open(os.path.join(ROOT, "tests", "golden", "lv_obs.json")))
+    obs = np.array(doc["obs"], np.float32)
+    P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 1 << 14, 5, seed=1)  # warm-up
+    t0 = time.perf_counter()
+    sol, sw = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 1 << 14, 300, seed=3)
+    wall = time.perf_counter() - t0
+    ms = sw.elapsed_ms()
+    sw.free()
+    rk4_steps = 1000  # per objective evaluation (100 intervals x 10 steps)
+    return {"wall_s": wall, "device_ms": ms,
+            "particle_iters_per_s": (1 << 14) * 300 / (ms / 1e3),
+            "rk4_steps_per_s": (1 << 14) * 300 * rk4_steps / (ms / 1e3),
+            "theta": [float(v) for v in sol.u], "loss_f64": sol.objective}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,6 +292,10 @@ def main():
     if not args.no_extras:
         out["batched"], out["served"] = c4_mix(P, L, ctx)
         log("[bench] c4 done")
+        out["hybrid"] = hybrid_c2(P, L, ctx)
+        log("[bench] hybrid done")
+        out["ode"] = ode_c3(P)
+        log("[bench] ode done")
 
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -13,3 +13,4 @@ from .api import (  # noqa: F401
     swarm_params)
 from .serve import Request, Result, Server, batch_solve, solve_alone  # noqa: F401
 from .workloads import served_mix  # noqa: F401
+from .ode import parameter_estim_ode  # noqa: F401

@@ -1,0 +1,41 @@
+"""ODE parameter estimation with PSO (reference parameter_estim_ode!,
+src/ode_pso.jl:113-170, fixed time step).
+
+The reference steps every particle (ode_update_particle_states!, with its
+whole-vector max/min clamp, src/ode_pso.jl:18-19), solves one ODE per particle
+with DiffEqGPU, reduces the squared error against the data
+(_reduce_losses!, :47-52), updates best costs and takes the argmin of the best
+costs as the next global best.  Here the Lotka-Volterra right-hand side is
+integrated with a fixed-step FP32 RK4 inside the fused PSO kernel, so one
+launch per iteration does all of it.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _lib as L
+from . import api
+
+
+def parameter_estim_ode(obs, lb, ub, num_particles=1 << 14, maxiters=300, *, u0=(1.0, 1.0),
+                        dt=0.01, steps_per_obs=10, w=0.7298, wdamp=1.0, seed=None,
+                        clamp="lexicographic", launch=L.LAUNCH_AUTO):
+    """Estimate (alpha, beta, gamma, delta) of Lotka-Volterra from `obs`
+    (n_obs x 2).  Returns (solution, swarm)."""
+    f = api.lotka_volterra_sse(obs, u0=u0, dt=dt, steps_per_obs=steps_per_obs)
+    prob = api.OptimizationProblem(f, np.zeros(4, np.float32), lb=np.asarray(lb, np.float32),
+                                   ub=np.asarray(ub, np.float32))
+    seed = api._next_seed() if seed is None else int(seed)
+    cl = L.CLAMP_LEXICOGRAPHIC if clamp == "lexicographic" else L.CLAMP_COMPONENT
+    sw = api.Swarm(api.context(), prob, api.swarm_params(num_particles, L.MODE_SYNC, seed=seed,
+                                                         clamp=cl, launch=launch))
+    t0 = time.perf_counter()
+    sw.run(maxiters, w, wdamp)
+    t1 = time.perf_counter()
+    g = sw.gbest()
+    obj = float(api.eval_f64(prob, g.position[None, :])[0])
+    sol = api.OptimizationSolution(g.position, obj, g.cost, None,
+                                   api.Stats(t1 - t0, sw.elapsed_ms()), "Default", g.index)
+    return sol, sw

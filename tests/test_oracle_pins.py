@@ -310,3 +310,19 @@ def test_reference_thresholds_gpu_suite(oracle, d):
     s = oracle.Swarm(P, 5000, seed=3234 + d)
     s.run_sync(500)
     assert s.gbest()[1] < 6e-4
+
+
+def test_lv_fixture_loss_at_true_parameters(oracle):
+    # the committed observations (tests/golden/lv_obs.json): FP32 in-kernel
+    # RK4 loss at the true parameters equals the FP64 noise energy
+    import json
+    import os
+
+    doc = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "lv_obs.json")))
+    obs = np.array(doc["obs"])
+    traj = np.array(doc["trajectory"])
+    sse64 = float(np.sum((obs - traj) ** 2))
+    sse32 = float(_lv(oracle, obs, doc["theta"], u0=doc["u0"], dt=doc["dt"], spo=doc["steps_per_obs"]))
+    assert abs(sse32 - sse64) < 1e-3 * sse64
+    # and the noise has the stated scale
+    assert abs(np.std(obs - traj) - doc["sigma"]) < 0.005
