@@ -1,0 +1,260 @@
+# PPSMI355X.jl — Julia binding of libpps for ParallelParticleSwarms.jl.
+#
+# Loaded as a package extension of ParallelParticleSwarms (v1.4.0), it adds the
+# backend type `MI355XBackend` and the SciMLBase methods the reference defines
+# for its own backends:
+#   init(prob, ParallelSyncPSOKernel|ParallelPSOKernel{<:MI355XBackend})
+#       src/init.jl:56-115
+#   solve!(cache, alg)      src/solve.jl:11-59
+#   reinit!(cache)          src/init.jl:117-141
+#   solve(prob, alg)        src/solve.jl:61-67
+#   solve!(::HybridPSOCache, ::HybridPSO{<:MI355XBackend,<:SimpleLBFGS})
+#                           src/hybrid.jl:100-158
+# Every call crosses into libpps through `ccall`; the C declarations are in
+# include/pps.h.  User code changes only the backend:
+#     solve(prob, ParallelSyncPSOKernel(1000; backend = MI355XBackend()), maxiters = 500)
+# The objective must be one of the markers below (an arbitrary Julia closure
+# cannot run inside a precompiled HIP kernel); anything else throws.
+module PPSMI355X
+
+using SciMLBase, StaticArrays
+import ParallelParticleSwarms: ParallelPSOKernel, ParallelSyncPSOKernel, HybridPSO,
+    SimpleLBFGS
+import SciMLBase: init, solve!, reinit!, solve, OptimizationProblem, OptimizationStats
+
+const libpps = get(ENV, "PPS_LIBPPS", joinpath(@__DIR__, "..", "lib", "libpps.so"))
+
+# ---- C ABI mirrors (include/pps.h) -----------------------------------------
+const PPS_OK = Int32(0)
+const PPS_OBJ = Dict(:rosenbrock => 1, :rastrigin => 2, :sphere => 3, :beale => 4,
+    :booth => 5, :himmelblau => 6, :quadratic => 7, :boundary_quadratic => 8,
+    :rosen2 => 9)
+const PPS_MODE_SYNC, PPS_MODE_ASYNC, PPS_MODE_QLOCK = Int32(0), Int32(1), Int32(2)
+const PPS_GBEST_PBEST, PPS_GBEST_CURRENT = Int32(0), Int32(1)
+
+struct CProblem            # pps_problem
+    objective::Int32
+    d::Int32
+    lb::Ptr{Float32}
+    ub::Ptr{Float32}
+    params::Ptr{Float32}
+    n_params::Int32
+    shift::Ptr{Float32}
+    obs::Ptr{Float32}
+    n_obs::Int32
+end
+
+struct CSwarmParams        # pps_swarm_params
+    n::Int64
+    mode::Int32
+    gbest_from::Int32
+    clamp::Int32
+    launch::Int32
+    seed::UInt64
+    c1::Float32
+    c2::Float32
+    qlock_min_improvers::Int32
+    block::Int32
+end
+
+struct CLbfgsParams        # pps_lbfgs_params
+    maxiters::Int32
+    history::Int32
+    abstol::Float32
+    armijo_c::Float32
+    armijo_rho::Float32
+    max_backtracks::Int32
+end
+
+function check(rc)
+    rc == PPS_OK && return nothing
+    msg = unsafe_string(ccall((:pps_last_error, libpps), Cstring, ()))
+    error("libpps error $rc: $msg")
+end
+
+# ---- backend and objective markers -----------------------------------------
+struct MI355XBackend
+    device::Int
+end
+MI355XBackend() = MI355XBackend(0)
+
+"Marker objective: `OptimizationProblem(PPSObjective(:rosenbrock), u0, p; lb, ub)`."
+struct PPSObjective{S}
+    shift::Union{Nothing, Vector{Float32}}
+end
+PPSObjective(s::Symbol; shift = nothing) = PPSObjective{s}(shift === nothing ? nothing :
+                                                           Float32.(collect(shift)))
+(f::PPSObjective{S})(x, p) where {S} = eval_f64(f, x, p)
+
+objective_code(::PPSObjective{S}) where {S} = Int32(PPS_OBJ[S])
+objective_code(f) = throw(ArgumentError(
+    "the MI355X backend runs a closed objective set; wrap it as PPSObjective(:name), got $(typeof(f))"))
+
+const CTX = Ref{Ptr{Cvoid}}(C_NULL)
+function context(b::MI355XBackend)
+    if CTX[] == C_NULL
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:pps_ctx_create, libpps), Cint, (Cint, Ptr{Ptr{Cvoid}}), b.device, h))
+        CTX[] = h[]
+    end
+    return CTX[]
+end
+
+# Keeps the Julia arrays that a CProblem points into alive.
+struct ProblemArrays
+    lb::Vector{Float32}
+    ub::Vector{Float32}
+    p::Vector{Float32}
+    shift::Union{Nothing, Vector{Float32}}
+end
+
+function cproblem(prob::OptimizationProblem)
+    f = prob.f.f
+    d = length(prob.u0)
+    (prob.lb === nothing || prob.ub === nothing) &&
+        throw(ArgumentError("the MI355X backend samples the initial swarm in [lb, ub]"))
+    prob.f.cons === nothing || throw(ArgumentError("constraints are not supported"))
+    a = ProblemArrays(Float32.(collect(prob.lb)), Float32.(collect(prob.ub)),
+        prob.p === nothing || prob.p isa SciMLBase.NullParameters ? Float32[] :
+        Float32.(collect(prob.p)), f isa PPSObjective ? f.shift : nothing)
+    c = CProblem(objective_code(f), Int32(d), pointer(a.lb), pointer(a.ub),
+        isempty(a.p) ? Ptr{Float32}(C_NULL) : pointer(a.p), Int32(length(a.p)),
+        a.shift === nothing ? Ptr{Float32}(C_NULL) : pointer(a.shift),
+        Ptr{Float32}(C_NULL), Int32(0))
+    return c, a
+end
+
+function eval_f64(f::PPSObjective, x, p)
+    d = length(x)
+    prob = (; f = (; f = f, cons = nothing), u0 = x, p = p,
+        lb = fill(-Inf32, d), ub = fill(Inf32, d))
+    a = ProblemArrays(Float32[], Float32[], p === nothing ? Float32[] : Float32.(collect(p)),
+        f.shift)
+    c = CProblem(objective_code(f), Int32(d), C_NULL, C_NULL,
+        isempty(a.p) ? Ptr{Float32}(C_NULL) : pointer(a.p), Int32(length(a.p)),
+        a.shift === nothing ? Ptr{Float32}(C_NULL) : pointer(a.shift), C_NULL, Int32(0))
+    xs = Float64.(collect(x))
+    out = Ref{Float64}(0.0)
+    GC.@preserve a xs begin
+        check(ccall((:pps_eval_f64, libpps), Cint,
+            (Ref{CProblem}, Int64, Ptr{Float64}, Ref{Float64}), c, 1, xs, out))
+    end
+    return out[]
+end
+
+# ---- cache (the role of the reference's PSOCache, src/init.jl:1-7) ---------
+mutable struct MI355XCache{P, A}
+    prob::P
+    alg::A
+    handle::Ptr{Cvoid}
+    gbest_pos::Vector{Float32}   # the reference's cache.gbest
+    gbest_cost::Float32
+end
+
+function free!(c::MI355XCache)
+    if c.handle != C_NULL
+        ccall((:pps_swarm_free, libpps), Cint, (Ptr{Cvoid},), c.handle)
+        c.handle = C_NULL
+    end
+end
+
+mode_of(::ParallelSyncPSOKernel) = PPS_MODE_SYNC
+mode_of(a::ParallelPSOKernel) = a.global_update ? PPS_MODE_QLOCK : PPS_MODE_ASYNC
+
+function read_gbest(h, d)
+    pos = zeros(Float32, d)
+    cost = Ref{Float32}(0)
+    idx = Ref{Int64}(0)
+    check(ccall((:pps_swarm_get_gbest, libpps), Cint,
+        (Ptr{Cvoid}, Ptr{Float32}, Ref{Float32}, Ref{Int64}), h, pos, cost, idx))
+    return pos, cost[], idx[]
+end
+
+function init(prob::OptimizationProblem,
+        alg::Union{ParallelSyncPSOKernel{<:MI355XBackend}, ParallelPSOKernel{<:MI355XBackend}},
+        args...; seed::Integer = rand(UInt64), kwargs...)
+    c, arrays = cproblem(prob)
+    prm = CSwarmParams(alg.num_particles, mode_of(alg), PPS_GBEST_PBEST, Int32(0), Int32(0),
+        UInt64(seed), 1.4962f0, 1.4962f0, Int32(2), Int32(0))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve arrays begin
+        check(ccall((:pps_swarm_init, libpps), Cint,
+            (Ptr{Cvoid}, Ref{CProblem}, Ref{CSwarmParams}, Ptr{Ptr{Cvoid}}),
+            context(alg.backend), c, prm, h))
+    end
+    pos, cost, _ = read_gbest(h[], length(prob.u0))
+    cache = MI355XCache(prob, alg, h[], pos, cost)
+    finalizer(free!, cache)
+    return cache
+end
+
+function solve!(cache::MI355XCache, args...; maxiters = 100, w = 0.7298f0, wdamp = 1.0f0,
+        kwargs...)
+    h, d = cache.handle, length(cache.prob.u0)
+    if cache.alg isa ParallelSyncPSOKernel
+        # the reference's sync solve! starts from cache.gbest every call
+        check(ccall((:pps_swarm_set_gbest, libpps), Cint,
+            (Ptr{Cvoid}, Ptr{Float32}, Float32), h, cache.gbest_pos, cache.gbest_cost))
+    end
+    t0 = time()
+    check(ccall((:pps_swarm_run, libpps), Cint, (Ptr{Cvoid}, Int32, Float32, Float32),
+        h, Int32(maxiters), Float32(w), Float32(wdamp)))
+    t1 = time()
+    pos, cost, _ = read_gbest(h, d)
+    if !(cache.alg isa ParallelSyncPSOKernel)
+        cache.gbest_pos, cache.gbest_cost = pos, cost
+    end
+    n = cache.alg.num_particles
+    aos = zeros(Float32, d * n)
+    check(ccall((:pps_swarm_get_positions, libpps), Cint, (Ptr{Cvoid}, Ptr{Float32}), h, aos))
+    u0 = cache.prob.u0
+    original = [typeof(u0)(view(aos, ((i - 1) * d + 1):(i * d))) for i in 1:n]
+    u = typeof(u0)(pos)
+    return SciMLBase.build_solution(
+        SciMLBase.DefaultOptimizationCache(cache.prob.f, cache.prob.p), cache.alg,
+        u, cache.prob.f(u, cache.prob.p); original = original,
+        stats = OptimizationStats(; time = t1 - t0))
+end
+
+function reinit!(cache::MI355XCache)
+    check(ccall((:pps_swarm_reinit, libpps), Cint, (Ptr{Cvoid},), cache.handle))
+    cache.gbest_pos, cache.gbest_cost, _ = read_gbest(cache.handle, length(cache.prob.u0))
+    return nothing
+end
+
+function solve(prob::OptimizationProblem,
+        alg::Union{ParallelSyncPSOKernel{<:MI355XBackend}, ParallelPSOKernel{<:MI355XBackend}},
+        args...; maxiters = 100, kwargs...)
+    return solve!(init(prob, alg; kwargs...); maxiters, kwargs...)
+end
+
+# HybridPSO{MI355XBackend, SimpleLBFGS} (src/hybrid.jl:100-158): PSO, then
+# projected L-BFGS from every particle's best position, keep the better.
+function solve(prob::OptimizationProblem, alg::HybridPSO{<:MI355XBackend, <:SimpleLBFGS},
+        args...; maxiters = 100, local_maxiters = 50, abstol = nothing, kwargs...)
+    cache = init(prob, alg.pso; kwargs...)
+    sol_pso = solve!(cache; maxiters)
+    d = length(prob.u0)
+    lp = CLbfgsParams(Int32(local_maxiters), Int32(10),
+        abstol === nothing ? 0.0f0 : Float32(abstol), 1.0f-4, 0.5f0, Int32(20))
+    bx = zeros(Float32, d)
+    bf = Ref{Float32}(0)
+    bi = Ref{Int64}(0)
+    t0 = time()
+    check(ccall((:pps_swarm_lbfgs, libpps), Cint,
+        (Ptr{Cvoid}, Ref{CLbfgsParams}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32},
+            Ref{Float32}, Ref{Int64}),
+        cache.handle, lp, C_NULL, C_NULL, bx, bf, bi))
+    best_u, best_obj = sol_pso.u, sol_pso.objective
+    local_obj = prob.f(typeof(prob.u0)(bx), prob.p)
+    if local_obj < best_obj
+        best_u, best_obj = typeof(prob.u0)(bx), local_obj
+    end
+    return SciMLBase.build_solution(
+        SciMLBase.DefaultOptimizationCache(prob.f, prob.p), alg, best_u, best_obj;
+        stats = OptimizationStats(; time = (time() - t0) + sol_pso.stats.time))
+end
+
+export MI355XBackend, PPSObjective
+
+end # module

@@ -10,7 +10,7 @@ import os
 import re
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "lib", "libpps.so")
+LIB_PATH = os.environ.get("PPS_LIBPPS") or os.path.join(_PKG, "lib", "libpps.so")
 HEADER = os.path.join(os.path.dirname(_PKG), "include", "pps.h")
 
 # status codes
