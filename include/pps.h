@@ -133,7 +133,10 @@ typedef struct pps_swarm_params {
   int32_t qlock_min_improvers; /* QLOCK: block publishes only when at least
                                   this many particles beat the global best;
                                   the reference uses 2 (src/kernels.jl:97)   */
-  int32_t block;             /* threads per workgroup, 0 = tuned             */
+  int32_t grid_per_cu;       /* workgroups per CU of the multi-workgroup sync
+                                step; 0 = autotuned value if one is cached
+                                for (objective, d, N), else 2. Results do
+                                not depend on it.                            */
 } pps_swarm_params;
 
 /* Fill p with the reference defaults (c1 = c2 = 1.4962, SYNC, PBEST,
@@ -204,6 +207,19 @@ void pps_lbfgs_params_default(pps_lbfgs_params* p);
 int pps_swarm_lbfgs(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x,
                     float* out_f, float* best_x, float* best_f,
                     int64_t* best_idx);
+
+/* ---- autotuning ------------------------------------------------------- */
+/* Time the multi-workgroup sync step for grid_per_cu in {1, 2, 3, 4, 6, 8} on a
+ * scratch swarm of `n` particles (`iters` timed iterations each), remember the
+ * fastest for (objective, d, ceil(log2 n)) for later pps_swarm_init calls in
+ * this process, and append it to the file named by $PPS_TUNE_CACHE if set
+ * (read back at the first lookup).  best_grid_per_cu and best_us_per_iter may
+ * be NULL. */
+int pps_autotune_sync(pps_ctx* ctx, const pps_problem* prob, int64_t n,
+                      int32_t iters, int32_t* best_grid_per_cu,
+                      float* best_us_per_iter);
+/* The cached choice for (objective, d, n), or 0 if none. */
+int pps_tuned_grid_per_cu(int32_t objective, int32_t d, int64_t n);
 
 /* ---- host FP64 re-evaluation -------------------------------------------- */
 int pps_eval_f64(const pps_problem* prob, int64_t n, const double* x_aos,

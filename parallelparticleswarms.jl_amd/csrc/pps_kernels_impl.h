@@ -142,9 +142,21 @@ __global__ __launch_bounds__(BLOCK) void k_rekey(SwarmDev S, ProbArgs P, uint32_
   publish_block<BLOCK>(S, par, d, mykey, gbest_from == PPS_GBEST_CURRENT ? S.x : S.pb, s_red);
 }
 
+// Occupancy target of the HBM-streaming sync step (waves per SIMD); 0 leaves
+// register allocation to the compiler.
+#ifndef PPS_SYNC_WAVES_PER_EU
+#define PPS_SYNC_WAVES_PER_EU 0
+#endif
+#if PPS_SYNC_WAVES_PER_EU > 0
+#define PPS_SYNC_ATTR __attribute__((amdgpu_waves_per_eu(PPS_SYNC_WAVES_PER_EU)))
+#else
+#define PPS_SYNC_ATTR
+#endif
+
 template <int MAXD, bool EXACT, int OBJ, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_sync_step(SwarmDev S, ProbArgs P, StepArgs A,
-                                                     float w, uint32_t k) {
+__global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, ProbArgs P,
+                                                                   StepArgs A, float w,
+                                                                   uint32_t k) {
   __shared__ uint64_t s_red[BLOCK / 64];
   const uint32_t t = *S.tbase + k;
   __shared__ float s_gb[MAXD];

@@ -8,6 +8,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -134,7 +138,7 @@ void pps_swarm_params_default(pps_swarm_params* p) {
   p->c1 = 1.4962f;
   p->c2 = 1.4962f;
   p->qlock_min_improvers = 2;
-  p->block = 0;
+  p->grid_per_cu = 0;
 }
 
 int pps_ctx_create(int device, pps_ctx** out) {
@@ -188,6 +192,52 @@ int pps_ctx_info(pps_ctx* c, int32_t* num_cus, int32_t* clock_khz, int64_t* tota
 // ---------------------------------------------------------------------------
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Default workgroups per CU of the multi-workgroup sync step (each reduces all
+// G block keys at its start, so G stays a small multiple of the CU count).
+#ifndef PPS_SYNC_GRID_PER_CU
+#define PPS_SYNC_GRID_PER_CU 2
+#endif
+
+namespace {
+
+// Autotuned grid choices keyed by (objective, d, ceil(log2 n)).
+struct TuneTable {
+  std::mutex mu;
+  std::map<std::tuple<int, int, int>, int> best;
+  bool loaded = false;
+};
+TuneTable& tune_table() {
+  static TuneTable t;
+  return t;
+}
+int log2_bucket(int64_t n) {
+  int b = 0;
+  while (((int64_t)1 << b) < n) ++b;
+  return b;
+}
+void tune_load_locked(TuneTable& t) {
+  if (t.loaded) return;
+  t.loaded = true;
+  const char* path = getenv("PPS_TUNE_CACHE");
+  if (!path) return;
+  FILE* f = fopen(path, "r");
+  if (!f) return;
+  int obj, d, b, g;
+  while (fscanf(f, "%d %d %d %d", &obj, &d, &b, &g) == 4)
+    if (g > 0 && g <= 64) t.best[std::make_tuple(obj, d, b)] = g;
+  fclose(f);
+}
+
+}  // namespace
+
+extern "C" int pps_tuned_grid_per_cu(int32_t obj, int32_t d, int64_t n) {
+  TuneTable& t = tune_table();
+  std::lock_guard<std::mutex> lk(t.mu);
+  tune_load_locked(t);
+  auto it = t.best.find(std::make_tuple((int)obj, (int)d, log2_bucket(n)));
+  return it == t.best.end() ? 0 : it->second;
+}
+
 static int swarm_prepare_launch(pps_swarm* s) {
   const int64_t n = s->prm.n;
   const int mode = s->prm.mode;
@@ -207,9 +257,16 @@ static int swarm_prepare_launch(pps_swarm* s) {
   } else {
     s->S.block = 256;
     const int64_t tiles = (n + 255) / 256;
-    int64_t gmax = s->prm.block > 0 ? s->prm.block : 0;  // reserved: tuned grid
-    (void)gmax;
-    const int64_t cap = (int64_t)s->ctx->num_cus * (mode == PPS_MODE_ASYNC ? 8 : 2);
+    int per_cu = PPS_SYNC_GRID_PER_CU;
+    if (mode == PPS_MODE_ASYNC) {
+      per_cu = 8;
+    } else if (s->prm.grid_per_cu > 0) {
+      per_cu = s->prm.grid_per_cu;
+    } else {
+      const int tuned = pps_tuned_grid_per_cu(s->P.obj, s->P.d, n);
+      if (tuned > 0) per_cu = tuned;
+    }
+    const int64_t cap = (int64_t)s->ctx->num_cus * per_cu;
     s->S.G = (int32_t)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
   }
   return PPS_OK;
@@ -288,6 +345,8 @@ int pps_swarm_init(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_params
     return fail(PPS_E_INVALID, "unknown gbest_from %d", prm->gbest_from);
   if (prm->clamp != PPS_CLAMP_COMPONENT && prm->clamp != PPS_CLAMP_LEXICOGRAPHIC)
     return fail(PPS_E_INVALID, "unknown clamp %d", prm->clamp);
+  if (prm->grid_per_cu < 0 || prm->grid_per_cu > 64)
+    return fail(PPS_E_INVALID, "grid_per_cu %d outside 0..64", prm->grid_per_cu);
   pps_swarm* s = new pps_swarm();
   s->ctx = ctx;
   s->prm = *prm;
@@ -507,6 +566,56 @@ int pps_swarm_free(pps_swarm* s) {
   if (s->mem) hipFree(s->mem);
   if (s->lx) hipFree(s->lx);
   delete s;
+  return PPS_OK;
+}
+
+int pps_autotune_sync(pps_ctx* ctx, const pps_problem* prob, int64_t n, int32_t iters,
+                      int32_t* best_grid_per_cu, float* best_us_per_iter) {
+  if (!ctx || !prob) return fail(PPS_E_INVALID, "NULL argument");
+  if (n <= 1024) return fail(PPS_E_INVALID, "swarms of at most 1024 particles run in one workgroup");
+  if (iters < 1) return fail(PPS_E_INVALID, "iters < 1");
+  static const int cand[] = {1, 2, 3, 4, 6, 8};
+  int best_g = 0;
+  float best_us = 0.0f;
+  for (int g : cand) {
+    pps_swarm_params prm;
+    pps_swarm_params_default(&prm);
+    prm.n = n;
+    prm.launch = PPS_LAUNCH_GRAPH;
+    prm.grid_per_cu = g;
+    prm.seed = 0x7E57u;
+    pps_swarm* s = nullptr;
+    int rc = pps_swarm_init(ctx, prob, &prm, &s);
+    if (rc != PPS_OK) return rc;
+    float ms = 0.0f, best_ms = 1e30f;
+    rc = pps_swarm_run(s, iters, 0.7298f, 1.0f);  // warm-up + graph capture
+    for (int rep = 0; rep < 3 && rc == PPS_OK; ++rep) {
+      rc = pps_swarm_run(s, iters, 0.7298f, 1.0f);
+      if (rc == PPS_OK) rc = pps_swarm_elapsed_ms(s, &ms);
+      best_ms = std::min(best_ms, ms);
+    }
+    pps_swarm_free(s);
+    if (rc != PPS_OK) return rc;
+    const float us = best_ms * 1e3f / (float)iters;
+    if (best_g == 0 || us < best_us) {
+      best_g = g;
+      best_us = us;
+    }
+  }
+  {
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> lk(t.mu);
+    tune_load_locked(t);
+    t.best[std::make_tuple((int)prob->objective, (int)prob->d, log2_bucket(n))] = best_g;
+    if (const char* path = getenv("PPS_TUNE_CACHE")) {
+      if (FILE* f = fopen(path, "a")) {
+        fprintf(f, "%d %d %d %d\n", prob->objective, prob->d, log2_bucket(n), best_g);
+        fclose(f);
+      }
+    }
+  }
+  if (best_grid_per_cu) *best_grid_per_cu = best_g;
+  if (best_us_per_iter) *best_us_per_iter = best_us;
   return PPS_OK;
 }
 

@@ -54,7 +54,7 @@ struct CSwarmParams        # pps_swarm_params
     c1::Float32
     c2::Float32
     qlock_min_improvers::Int32
-    block::Int32
+    grid_per_cu::Int32
 end
 
 struct CLbfgsParams        # pps_lbfgs_params

@@ -9,7 +9,7 @@ from .api import (  # noqa: F401
     Beale, Booth, BoundaryQuadratic, Context, GBest, Himmelblau, HybridPSO, MI355XBackend,
     Objective, OptimizationProblem, OptimizationSolution, ParallelPSOKernel,
     ParallelSyncPSOKernel, PSOCache, Quadratic, Rastrigin, Rosen2, Rosenbrock, SimpleLBFGS,
-    Sphere, Swarm, context, eval_f64, init, lotka_volterra_sse, reinit_, seed_, solve, solve_,
+    Sphere, Swarm, autotune_sync, context, eval_f64, init, lotka_volterra_sse, reinit_, seed_, solve, solve_,
     swarm_params)
 from .serve import Request, Result, Server, batch_solve, solve_alone  # noqa: F401
 from .workloads import served_mix  # noqa: F401

@@ -165,3 +165,35 @@ def test_shifted_objective_parity():
     o.run_sync(10)
     assert_same_state(g, o)
     assert_same_gbest(g, o)
+
+
+@pytest.mark.parametrize("grid_per_cu", [1, 3, 8])
+def test_grid_size_does_not_change_results(grid_per_cu):
+    # the autotuner may pick any grid: results must not depend on it
+    note(f"grid_per_cu={grid_per_cu}")
+    import ppswarm as P
+
+    d, n = 10, 300000
+    lb, ub = np.full(d, -5.12, np.float32), np.full(d, 5.12, np.float32)
+    prob = P.OptimizationProblem(P.Rastrigin(), np.zeros(d, np.float32), lb=lb, ub=ub)
+    g = P.Swarm(P.context(), prob, P.swarm_params(n, L.MODE_SYNC, seed=9, launch=L.LAUNCH_GRAPH,
+                                                  grid_per_cu=grid_per_cu))
+    o = O.Swarm(O.Problem(O.RASTRIGIN, d, lb, ub), n, 9)
+    g.run(6)
+    o.run_sync(6)
+    assert_same_state(g, o)
+    assert_same_gbest(g, o)
+
+
+def test_autotune_records_a_choice():
+    note("autotune")
+    import ppswarm as P
+    from ppswarm import _lib as LL
+
+    d, n = 8, 1 << 16
+    prob = P.OptimizationProblem(P.Sphere(), np.zeros(d, np.float32), lb=np.full(d, -1.0),
+                                 ub=np.full(d, 1.0))
+    g, us = P.autotune_sync(prob, n, iters=5)
+    assert g in (1, 2, 3, 4, 6, 8) and us > 0
+    assert LL.lib().pps_tuned_grid_per_cu(LL.OBJ_SPHERE, d, n) == g
+    assert LL.lib().pps_tuned_grid_per_cu(LL.OBJ_SPHERE, d, 4 * n) == 0
