@@ -8,8 +8,8 @@
 
 namespace pps {
 
-constexpr int BATCH_BLOCK = 256;         // workgroup of a small problem
-constexpr int BATCH_BLOCK_LARGE = 1024;  // workgroup of a large problem
+constexpr int BATCH_BLOCK = 256;   // threads per work item
+constexpr int BATCH_MAXPARTS = 64; // workgroups per iteration of a large problem
 
 // One admitted problem.  Written by the host at admission, then owned by the
 // device until the problem finishes (remaining == 0).
@@ -44,16 +44,37 @@ struct BatchOut {
   uint32_t pad_[3];
 };
 
+// Exchange area of a large problem: the key (order(cost) << 32 | index) and
+// best position of each part's best particle, by iteration parity.
+struct BatchXchg {
+  uint64_t key[2][BATCH_MAXPARTS];
+  float cand[2][BATCH_MAXPARTS][PPS_BMAXD];
+};
+
+enum { BATCH_SMALL = 0, BATCH_PART = 1, BATCH_FINAL = 2 };
+
+// One workgroup of a slice.
+struct BatchItem {
+  int32_t slot;
+  int32_t kind;    // BATCH_*
+  int32_t part;    // PART: q in 0..nparts-1
+  int32_t nparts;  // PART/FINAL: k
+  uint32_t t;      // PART: iteration (0 = init); FINAL: last iteration
+  float w;         // PART: inertia of iteration t
+  int32_t fresh;   // SMALL: copy the descriptor from host memory first
+  int32_t pad_;
+};
+
 bool batch_supported(int obj, int d);
-// Problems probs[ids[0..count)] each run in one workgroup until finished, or
-// `chunk` iterations, or `budget_ticks` of the device wall clock (checked
-// after every iteration).  When fresh_in[id] == launch_no the workgroup first
-// copies its descriptor fresh[id] (host-visible memory) -> probs[id].  If
-// `out` is non-null each problem's result and done flag are published to
-// out[id] when its remaining count reaches zero.
-// `block` selects the workgroup size (BATCH_BLOCK or BATCH_BLOCK_LARGE).
-hipError_t launch_batch(BatchProb* probs, const BatchProb* fresh, const uint32_t* fresh_in,
-                        uint32_t launch_no, const int32_t* ids, int32_t count, int32_t chunk,
-                        uint64_t budget_ticks, BatchOut* out, int block, hipStream_t st);
+// Parts of a large problem of n particles (tiles of 1024 particles, at most
+// BATCH_MAXPARTS).
+int batch_parts(int64_t n);
+// One slice: items[0..count) each run in one workgroup of BATCH_BLOCK threads.
+// SMALL items stop after the first iteration that ends past budget_ticks of
+// the device wall clock.  Finished problems publish result + done flag to
+// out[slot] (host-visible memory).
+hipError_t launch_slice(BatchProb* probs, const BatchProb* fresh, const BatchItem* items,
+                        int32_t count, uint64_t budget_ticks, BatchOut* out, BatchXchg* xchg,
+                        hipStream_t st);
 
 }  // namespace pps

@@ -1,17 +1,23 @@
 // pps_batch.hip — packed launches of many independent sync-PSO problems.
 //
-// Each admitted problem has a device descriptor (BatchProb): its objective,
-// bounds, Philox key, iteration counter, damped inertia, the attractor of its
-// next iteration, and pointers to its SoA state in a shared arena.  One launch
-// ("chunk") gives every listed problem one workgroup, which runs up to `chunk`
-// iterations of that problem with the workgroup barrier as the per-iteration
-// synchronisation; the winner's position is handed over through LDS.  A
-// problem's result depends only on its own seed, particle indices and
-// iteration numbers, so it is the same whether it runs alone (the per-problem
-// launch loop of pps_swarm_run) or packed with any other problems, in any
-// chunking.  A launch is time-sliced: each workgroup stops its problem after
-// the first iteration that ends past a device-clock deadline, so the
-// workgroups of one launch finish together whatever their problem sizes.
+// A launch ("slice") is a list of work items, one workgroup each:
+//   SMALL  one problem in one workgroup: its iteration loop runs until the
+//          problem ends or the slice's device-clock budget is spent (checked
+//          after every iteration by thread 0, broadcast through LDS); the
+//          winner's position is handed over through LDS.
+//   PART   part q of k of a large problem: one iteration t over the particle
+//          tiles q, q+k, q+2k, ... (1024 particles per tile).  The global best
+//          of iteration t-1 is the minimum of the k part keys the previous
+//          slice wrote to parity (t-1)&1 of the problem's exchange area; the
+//          part writes its own key and candidate position to parity t&1, so
+//          no workgroup reads what another writes within one launch.
+//   FINAL  reduce the k part keys of a large problem's last iteration into
+//          its result.
+// Results and done flags go to host-visible pinned memory.  A problem's random
+// stream is keyed by its own seed, particle indices and iteration numbers, and
+// every path below applies the same per-particle arithmetic as the plain sync
+// kernel, so a result is the same alone (pps_swarm_init + pps_swarm_run) or
+// packed with anything, in any slicing.
 #include <hip/hip_runtime.h>
 
 #include "pps_batch.h"
@@ -22,6 +28,8 @@ namespace pps {
 namespace {
 
 constexpr uint64_t KEY_NONE = ~0ull;
+constexpr int PPT = 4;  // particles in flight per thread
+constexpr int TILE = BATCH_BLOCK * PPT;
 
 template <int BLOCK>
 __device__ __forceinline__ uint64_t block_min(uint64_t k, uint64_t* s_red) {
@@ -65,60 +73,167 @@ struct Accum {
   }
 };
 
-// Run iterations of one problem in this workgroup until it is finished, or
-// `chunk` iterations are done, or the device clock passes `deadline` (checked
-// by thread 0 once per iteration and broadcast through LDS, so the whole
-// workgroup stops after the same iteration).
-//
-// The swarm is streamed one coordinate pair at a time (one Philox call per
-// pair, as in velocity_position()), so register use does not grow with d; a
-// particle whose cost improves re-reads its new position to store its best.
-template <int OBJ, int BLOCK, int PPT>
-__device__ void run_problem(BatchProb* B, int32_t chunk, uint64_t deadline, uint64_t* s_red,
-                            ProbLds* L, int* s_stop) {
-  const int d = B->P.d;
-  const int64_t n = B->n, np = B->npad;
-  const uint32_t k0 = B->key0, k1 = B->key1;
-  const float p1 = B->P.par[0], p2 = B->P.par[1];
-  const bool has_shift = B->P.has_shift != 0;
-  float* __restrict__ X = B->x;
-  float* __restrict__ V = B->v;
-  float* __restrict__ PB = B->pb;
-  float* __restrict__ PC = B->pc;
-  if (threadIdx.x < d) {
+struct Swarm {
+  int d;
+  int64_t n, np;
+  uint32_t k0, k1;
+  float p1, p2, c1, c2;
+  bool has_shift;
+  float* __restrict__ X;
+  float* __restrict__ V;
+  float* __restrict__ PB;
+  float* __restrict__ PC;
+};
+
+__device__ __forceinline__ Swarm load_swarm(const BatchProb* B, ProbLds* L) {
+  Swarm S;
+  S.d = B->P.d;
+  S.n = B->n;
+  S.np = B->npad;
+  S.k0 = B->key0;
+  S.k1 = B->key1;
+  S.p1 = B->P.par[0];
+  S.p2 = B->P.par[1];
+  S.c1 = B->c1;
+  S.c2 = B->c2;
+  S.has_shift = B->P.has_shift != 0;
+  S.X = B->x;
+  S.V = B->v;
+  S.PB = B->pb;
+  S.PC = B->pc;
+  if ((int)threadIdx.x < S.d) {
     L->lb[threadIdx.x] = B->P.lb[threadIdx.x];
     L->ub[threadIdx.x] = B->P.ub[threadIdx.x];
-    L->shift[threadIdx.x] = has_shift ? B->P.shift[threadIdx.x] : 0.0f;
+    L->shift[threadIdx.x] = S.has_shift ? B->P.shift[threadIdx.x] : 0.0f;
   }
-  __syncthreads();
-  if (!B->initialized) {
-    // uniform init (as init_position + objective) and the initial global best
-    uint64_t mykey = KEY_NONE;
-    for (int64_t i = threadIdx.x; i < n; i += BLOCK) {
+  return S;
+}
+
+// Uniform init (as init_position + objective) of the tiles first,
+// first + stride, ...; returns this thread's minimum key.
+template <int OBJ>
+__device__ uint64_t init_sweep(const Swarm& S, const ProbLds* L, int64_t first, int64_t stride) {
+  uint64_t mykey = KEY_NONE;
+  for (int64_t tile = first; tile * TILE < S.n; tile += stride) {
+    for (int q = 0; q < PPT; ++q) {
+      const int64_t i = tile * TILE + (int64_t)q * BATCH_BLOCK + threadIdx.x;
+      if (i >= S.n) continue;
       Accum<OBJ> acc;
-      acc.start(d);
+      acc.start(S.d);
       u4 r{};
-      for (int j = 0; j < d; ++j) {
+      for (int j = 0; j < S.d; ++j) {
         if ((j & 3) == 0)
           r = philox4x32_10(u4{(uint32_t)i, 0u, (uint32_t)(j >> 2) | (PPS_TAG_INIT << 16), 0u},
-                            k0, k1);
+                            S.k0, S.k1);
         const uint32_t rr = (j & 3) == 0 ? r.x : (j & 3) == 1 ? r.y : (j & 3) == 2 ? r.z : r.w;
         const float x = u01(rr) * (L->ub[j] - L->lb[j]) + L->lb[j];
-        X[(int64_t)j * np + i] = x;
-        V[(int64_t)j * np + i] = 0.0f;
-        PB[(int64_t)j * np + i] = x;
-        acc.add(j, has_shift ? x - L->shift[j] : x, p1, p2);
+        const int64_t o = (int64_t)j * S.np + i;
+        S.X[o] = x;
+        S.V[o] = 0.0f;
+        S.PB[o] = x;
+        acc.add(j, S.has_shift ? x - L->shift[j] : x, S.p1, S.p2);
       }
-      PC[i] = acc.res;
+      S.PC[i] = acc.res;
       mykey = umin64(mykey, make_key(acc.res, (uint32_t)i));
     }
-    const uint64_t bk = block_min<BLOCK>(mykey, s_red);
-    if (mykey == bk) {
-      const int64_t wi = (int64_t)(uint32_t)bk;
-      for (int j = 0; j < d; ++j) L->gb[j] = PB[(int64_t)j * np + wi];
+  }
+  return mykey;
+}
+
+// One iteration t (inertia w, attractor L->gb) over the tiles first,
+// first + stride, ...; returns this thread's minimum key.  Coordinates are
+// streamed pair by pair (one Philox call per pair, as velocity_position()),
+// PPT particles per thread in flight; an improving particle re-reads its new
+// position to store its best.
+template <int OBJ>
+__device__ uint64_t step_sweep(const Swarm& S, const ProbLds* L, int64_t first, int64_t stride,
+                               uint32_t t, float w) {
+  uint64_t mykey = KEY_NONE;
+  const int d = S.d;
+  for (int64_t tile = first; tile * TILE < S.n; tile += stride) {
+    const int64_t i0 = tile * TILE + threadIdx.x;
+    Accum<OBJ> acc[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) acc[q].start(d);
+    for (int k = 0; 2 * k < d; ++k) {
+      float x[PPT][2], v[PPT][2], pbv[PPT][2];
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        const int64_t i = i0 + (int64_t)q * BATCH_BLOCK;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int j = 2 * k + h;
+          const int64_t o = (int64_t)j * S.np + i;
+          const bool ok = i < S.n && j < d;
+          x[q][h] = ok ? S.X[o] : 0.0f;
+          v[q][h] = ok ? S.V[o] : 0.0f;
+          pbv[q][h] = ok ? S.PB[o] : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        const int64_t i = i0 + (int64_t)q * BATCH_BLOCK;
+        if (i >= S.n) continue;
+        const u4 r = philox4x32_10(u4{(uint32_t)i, t, (uint32_t)k | (PPS_TAG_STEP << 16), 0u},
+                                   S.k0, S.k1);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int j = 2 * k + h;
+          if (j < d) {
+            const float r1 = u01(h == 0 ? r.x : r.z);
+            const float r2 = u01(h == 0 ? r.y : r.w);
+            const float a = w * v[q][h];
+            const float b = (S.c1 * r1) * (pbv[q][h] - x[q][h]);
+            const float c = (S.c2 * r2) * (L->gb[j] - x[q][h]);
+            const float vn = (a + b) + c;
+            float xn = x[q][h] + vn;
+            xn = jl_min(jl_max(xn, L->lb[j]), L->ub[j]);
+            const int64_t o = (int64_t)j * S.np + i;
+            S.X[o] = xn;
+            S.V[o] = vn;
+            acc[q].add(j, S.has_shift ? xn - L->shift[j] : xn, S.p1, S.p2);
+          }
+        }
+      }
     }
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      const int64_t i = i0 + (int64_t)q * BATCH_BLOCK;
+      if (i >= S.n) continue;
+      float pc = S.PC[i];
+      if (acc[q].res < pc) {
+        for (int j = 0; j < d; ++j) S.PB[(int64_t)j * S.np + i] = S.X[(int64_t)j * S.np + i];
+        pc = acc[q].res;
+        S.PC[i] = pc;
+      }
+      mykey = umin64(mykey, make_key(pc, (uint32_t)i));
+    }
+  }
+  return mykey;
+}
+
+// The owning thread of the workgroup minimum copies that particle's best
+// position (which it wrote itself, or which is unchanged) to dst[0..d).
+__device__ __forceinline__ void copy_winner(const Swarm& S, uint64_t mykey, uint64_t bk,
+                                            float* dst) {
+  if (bk != KEY_NONE && mykey == bk) {
+    const int64_t wi = (int64_t)(uint32_t)bk;
+    for (int j = 0; j < S.d; ++j) dst[j] = S.PB[(int64_t)j * S.np + wi];
+  }
+}
+
+// SMALL item: run iterations until the problem ends or the deadline passes.
+template <int OBJ>
+__device__ void run_small(BatchProb* B, uint64_t deadline, uint64_t* s_red, ProbLds* L,
+                          int* s_stop) {
+  const Swarm S = load_swarm(B, L);
+  __syncthreads();
+  if (!B->initialized) {
+    const uint64_t mykey = init_sweep<OBJ>(S, L, 0, 1);
+    const uint64_t bk = block_min<BATCH_BLOCK>(mykey, s_red);
+    copy_winner(S, mykey, bk, L->gb);
     __syncthreads();
-    if (threadIdx.x < d) B->gb[threadIdx.x] = L->gb[threadIdx.x];
+    if ((int)threadIdx.x < S.d) B->gb[threadIdx.x] = L->gb[threadIdx.x];
     if (threadIdx.x == 0) {
       B->gb_cost = key_cost(bk);
       B->gb_idx = (uint32_t)bk;
@@ -126,90 +241,27 @@ __device__ void run_problem(BatchProb* B, int32_t chunk, uint64_t deadline, uint
     }
     __syncthreads();
   }
-  const int32_t steps = chunk < B->remaining ? chunk : B->remaining;
+  const int32_t steps = B->remaining;
   if (steps <= 0) return;
-  if (threadIdx.x < d) L->gb[threadIdx.x] = B->gb[threadIdx.x];
+  if ((int)threadIdx.x < S.d) L->gb[threadIdx.x] = B->gb[threadIdx.x];
   const uint32_t t0 = B->iter;
   float w = B->w;
-  const float wdamp = B->wdamp, c1 = B->c1, c2 = B->c2;
+  const float wdamp = B->wdamp;
   if (threadIdx.x == 0) *s_stop = 0;
   __syncthreads();
   uint64_t bk = KEY_NONE;
   int32_t done = 0;
   while (done < steps) {
     const uint32_t t = t0 + (uint32_t)(++done);
-    uint64_t mykey = KEY_NONE;
-    // PPT particles per thread in flight (i, i + BLOCK, ...): their loads of
-    // one coordinate pair are issued together, which hides memory latency
-    // when a workgroup streams a large swarm.
-    for (int64_t i0 = threadIdx.x; i0 < n; i0 += (int64_t)PPT * BLOCK) {
-      Accum<OBJ> acc[PPT];
-#pragma unroll
-      for (int q = 0; q < PPT; ++q) acc[q].start(d);
-      for (int k = 0; 2 * k < d; ++k) {
-        float x[PPT][2], v[PPT][2], pbv[PPT][2];
-#pragma unroll
-        for (int q = 0; q < PPT; ++q) {
-          const int64_t i = i0 + (int64_t)q * BLOCK;
-#pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const int j = 2 * k + h;
-            const int64_t o = (int64_t)j * np + i;
-            const bool ok = i < n && j < d;
-            x[q][h] = ok ? X[o] : 0.0f;
-            v[q][h] = ok ? V[o] : 0.0f;
-            pbv[q][h] = ok ? PB[o] : 0.0f;
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < PPT; ++q) {
-          const int64_t i = i0 + (int64_t)q * BLOCK;
-          if (i >= n) continue;
-          const u4 r = philox4x32_10(u4{(uint32_t)i, t, (uint32_t)k | (PPS_TAG_STEP << 16), 0u}, k0, k1);
-#pragma unroll
-          for (int h = 0; h < 2; ++h) {
-            const int j = 2 * k + h;
-            if (j < d) {
-              const float r1 = u01(h == 0 ? r.x : r.z);
-              const float r2 = u01(h == 0 ? r.y : r.w);
-              const float a = w * v[q][h];
-              const float b = (c1 * r1) * (pbv[q][h] - x[q][h]);
-              const float c = (c2 * r2) * (L->gb[j] - x[q][h]);
-              const float vn = (a + b) + c;
-              float xn = x[q][h] + vn;
-              xn = jl_min(jl_max(xn, L->lb[j]), L->ub[j]);
-              const int64_t o = (int64_t)j * np + i;
-              X[o] = xn;
-              V[o] = vn;
-              acc[q].add(j, has_shift ? xn - L->shift[j] : xn, p1, p2);
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < PPT; ++q) {
-        const int64_t i = i0 + (int64_t)q * BLOCK;
-        if (i >= n) continue;
-        float pc = PC[i];
-        if (acc[q].res < pc) {
-          for (int j = 0; j < d; ++j) PB[(int64_t)j * np + i] = X[(int64_t)j * np + i];
-          pc = acc[q].res;
-          PC[i] = pc;
-        }
-        mykey = umin64(mykey, make_key(pc, (uint32_t)i));
-      }
-    }
-    bk = block_min<BLOCK>(mykey, s_red);
-    if (mykey == bk) {
-      const int64_t wi = (int64_t)(uint32_t)bk;
-      for (int j = 0; j < d; ++j) L->gb[j] = PB[(int64_t)j * np + wi];
-    }
+    const uint64_t mykey = step_sweep<OBJ>(S, L, 0, 1, t, w);
+    bk = block_min<BATCH_BLOCK>(mykey, s_red);
+    copy_winner(S, mykey, bk, L->gb);
     if (threadIdx.x == 0) *s_stop = wall_clock64() > deadline;
     __syncthreads();
     w = w * wdamp;
     if (*s_stop) break;
   }
-  if (threadIdx.x < d) B->gb[threadIdx.x] = L->gb[threadIdx.x];
+  if ((int)threadIdx.x < S.d) B->gb[threadIdx.x] = L->gb[threadIdx.x];
   if (threadIdx.x == 0) {
     B->gb_cost = key_cost(bk);
     B->gb_idx = (uint32_t)bk;
@@ -219,64 +271,113 @@ __device__ void run_problem(BatchProb* B, int32_t chunk, uint64_t deadline, uint
   }
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_batch(BatchProb* probs, const BatchProb* fresh,
-                                                 const uint32_t* fresh_in, uint32_t launch_no,
-                                                 const int32_t* ids, int32_t chunk,
-                                                 uint64_t budget_ticks, BatchOut* out) {
-  __shared__ uint64_t s_red[BLOCK / 64];
-  __shared__ ProbLds s_lds;
-  __shared__ int s_stop;
-  const uint64_t deadline = wall_clock64() + budget_ticks;
-  const int32_t id = ids[blockIdx.x];
-  BatchProb* B = probs + id;
-  if (fresh && fresh_in[id] == launch_no) {
-    // newly admitted: pull the descriptor from host-visible memory
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(fresh + id);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(B);
-    for (int k = threadIdx.x; k < (int)(sizeof(BatchProb) / 4); k += BLOCK) dst[k] = src[k];
-    __syncthreads();
+// Global best of a large problem after iteration `t`: minimum of its part keys
+// of parity t & 1; the winning part is the one owning the winner's tile.
+__device__ __forceinline__ uint64_t part_gbest(const BatchXchg* XC, int k, uint32_t t, int d,
+                                               float* dst, uint64_t* s_red) {
+  const uint32_t par = t & 1u;
+  uint64_t key = KEY_NONE;
+  for (int q = threadIdx.x; q < k; q += BATCH_BLOCK) key = umin64(key, XC->key[par][q]);
+  key = block_min<BATCH_BLOCK>(key, s_red);
+  const int64_t wi = (int64_t)(uint32_t)key;
+  const int wpart = (int)((wi / TILE) % k);
+  if ((int)threadIdx.x < d) dst[threadIdx.x] = XC->cand[par][wpart][threadIdx.x];
+  __syncthreads();
+  return key;
+}
+
+// PART item: iteration t (t = 0: init) of part q of k.
+template <int OBJ>
+__device__ void run_part(BatchProb* B, const BatchItem& it, BatchXchg* XC, uint64_t* s_red,
+                         ProbLds* L) {
+  const Swarm S = load_swarm(B, L);
+  __syncthreads();
+  uint64_t mykey;
+  if (it.t == 0) {
+    mykey = init_sweep<OBJ>(S, L, it.part, it.nparts);
+  } else {
+    part_gbest(XC, it.nparts, it.t - 1u, S.d, L->gb, s_red);
+    mykey = step_sweep<OBJ>(S, L, it.part, it.nparts, it.t, it.w);
   }
-  if (B->P.obj == PPS_OBJ_RASTRIGIN)
-    run_problem<PPS_OBJ_RASTRIGIN, BLOCK, 4>(B, chunk, deadline, s_red, &s_lds, &s_stop);
-  else
-    run_problem<PPS_OBJ_ROSENBROCK, BLOCK, 4>(B, chunk, deadline, s_red, &s_lds, &s_stop);
-  if (out) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      // visible to the host after the state writes of this workgroup
-      __threadfence();
-      if (B->remaining <= 0) {
-        BatchOut* o = out + id;
-        o->gb_cost = B->gb_cost;
-        o->gb_idx = B->gb_idx;
-        o->iter = B->iter;
-        o->remaining = B->remaining;
-        for (int j = 0; j < B->P.d && j < PPS_BMAXD; ++j) o->gb[j] = B->gb[j];
-        __threadfence_system();
-        __hip_atomic_store(&o->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-      }
-    }
-  }
+  const uint64_t bk = block_min<BATCH_BLOCK>(mykey, s_red);
+  const uint32_t par = it.t & 1u;
+  copy_winner(S, mykey, bk, XC->cand[par][it.part]);
+  if (threadIdx.x == 0) XC->key[par][it.part] = bk;
 }
 
 }  // namespace
+
+__global__ __launch_bounds__(BATCH_BLOCK) void k_slice(BatchProb* probs, const BatchProb* fresh,
+                                                       const BatchItem* items,
+                                                       uint64_t budget_ticks, BatchOut* out,
+                                                       BatchXchg* xchg) {
+  __shared__ uint64_t s_red[BATCH_BLOCK / 64];
+  __shared__ ProbLds s_lds;
+  __shared__ int s_stop;
+  const uint64_t deadline = wall_clock64() + budget_ticks;
+  const BatchItem it = items[blockIdx.x];
+  BatchProb* B = probs + it.slot;
+  if (it.kind == BATCH_SMALL) {
+    if (it.fresh) {
+      // newly admitted: pull the descriptor from host-visible memory
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(fresh + it.slot);
+      uint32_t* dst = reinterpret_cast<uint32_t*>(B);
+      for (int k = threadIdx.x; k < (int)(sizeof(BatchProb) / 4); k += BATCH_BLOCK)
+        dst[k] = src[k];
+      __syncthreads();
+    }
+    if (B->P.obj == PPS_OBJ_RASTRIGIN)
+      run_small<PPS_OBJ_RASTRIGIN>(B, deadline, s_red, &s_lds, &s_stop);
+    else
+      run_small<PPS_OBJ_ROSENBROCK>(B, deadline, s_red, &s_lds, &s_stop);
+    __syncthreads();
+    if (threadIdx.x == 0 && B->remaining <= 0) {
+      BatchOut* o = out + it.slot;
+      o->gb_cost = B->gb_cost;
+      o->gb_idx = B->gb_idx;
+      o->iter = B->iter;
+      o->remaining = 0;
+      for (int j = 0; j < B->P.d; ++j) o->gb[j] = B->gb[j];
+      __threadfence_system();
+      __hip_atomic_store(&o->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  } else if (it.kind == BATCH_PART) {
+    if (B->P.obj == PPS_OBJ_RASTRIGIN)
+      run_part<PPS_OBJ_RASTRIGIN>(B, it, xchg + it.slot, s_red, &s_lds);
+    else
+      run_part<PPS_OBJ_ROSENBROCK>(B, it, xchg + it.slot, s_red, &s_lds);
+  } else {  // BATCH_FINAL: it.t = last iteration
+    const int d = B->P.d;
+    const uint64_t key = part_gbest(xchg + it.slot, it.nparts, it.t, d, s_lds.gb, s_red);
+    if (threadIdx.x == 0) {
+      BatchOut* o = out + it.slot;
+      o->gb_cost = key_cost(key);
+      o->gb_idx = (uint32_t)key;
+      o->iter = it.t;
+      o->remaining = 0;
+      for (int j = 0; j < d; ++j) o->gb[j] = s_lds.gb[j];
+      __threadfence_system();
+      __hip_atomic_store(&o->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+}
 
 bool batch_supported(int obj, int d) {
   if (obj != PPS_OBJ_ROSENBROCK && obj != PPS_OBJ_RASTRIGIN) return false;
   return d >= 1 && d <= PPS_BMAXD;
 }
 
-hipError_t launch_batch(BatchProb* probs, const BatchProb* fresh, const uint32_t* fresh_in,
-                        uint32_t launch_no, const int32_t* ids, int32_t count, int32_t chunk,
-                        uint64_t budget_ticks, BatchOut* out, int block, hipStream_t st) {
+int batch_parts(int64_t n) {
+  const int64_t tiles = (n + TILE - 1) / TILE;
+  return (int)(tiles < BATCH_MAXPARTS ? tiles : BATCH_MAXPARTS);
+}
+
+hipError_t launch_slice(BatchProb* probs, const BatchProb* fresh, const BatchItem* items,
+                        int32_t count, uint64_t budget_ticks, BatchOut* out, BatchXchg* xchg,
+                        hipStream_t st) {
   if (count <= 0) return hipSuccess;
-  if (block == BATCH_BLOCK_LARGE)
-    hipLaunchKernelGGL(k_batch<BATCH_BLOCK_LARGE>, dim3(count), dim3(BATCH_BLOCK_LARGE), 0, st,
-                       probs, fresh, fresh_in, launch_no, ids, chunk, budget_ticks, out);
-  else
-    hipLaunchKernelGGL(k_batch<BATCH_BLOCK>, dim3(count), dim3(BATCH_BLOCK), 0, st, probs, fresh,
-                       fresh_in, launch_no, ids, chunk, budget_ticks, out);
+  hipLaunchKernelGGL(k_slice, dim3(count), dim3(BATCH_BLOCK), 0, st, probs, fresh, items,
+                     budget_ticks, out, xchg);
   return hipGetLastError();
 }
 

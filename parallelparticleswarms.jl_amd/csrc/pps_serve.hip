@@ -44,8 +44,12 @@ double now_s() {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Swarms with n * d at or above this stream through 1024-thread workgroups.
+// Swarms with n * d at or above this run as `batch_parts(n)` workgroups, one
+// iteration per slice; smaller ones run their own loop in one workgroup.
 constexpr int64_t kLargeWork = 32768;
+// Device-clock budget of SMALL items while large problems are resident, so a
+// slice (and with it one iteration of every large problem) stays short.
+constexpr int kSliceUsWithLarge = 100;
 
 enum { R_PENDING = 0, R_RUNNING = 1, R_DONE = 2, R_CANCELLED = 3, R_FAILED = 4 };
 
@@ -61,6 +65,12 @@ struct Req {
   int state = R_PENDING;
   int slot = -1;
   size_t off = 0, bytes = 0;
+  // large problems: parts per iteration, next iteration to issue (0 = init),
+  // inertia of that iteration (iterated FP32 product, as pps_swarm_run)
+  bool large = false;
+  int parts = 0;
+  uint32_t t_next = 0;
+  float w_next = 0.0f;
   pps_result res{};
 };
 
@@ -117,14 +127,24 @@ struct pps_server {
   BatchProb* d_desc = nullptr;  // [max_active] device
   BatchProb* h_desc = nullptr;  // [max_active] pinned staging
   BatchOut* h_out = nullptr;    // [max_active] pinned results
-  int32_t* h_ids = nullptr;     // [max_active] pinned slice list
-  uint32_t* h_fresh = nullptr;  // [max_active] launch number whose slice uploads the slot
+  BatchItem* h_items[2] = {};    // [max_items] pinned work lists (two slices in flight)
+  int64_t max_items = 0;
+  hipEvent_t slice_done[2] = {};
+  uint64_t slices_enqueued = 0, slices_completed = 0;
+  // retired slots/arena ranges wait until every slice that may still list
+  // them has completed
+  struct Quarantined {
+    uint64_t after;  // free once slices_completed >= after
+    int slot;
+    size_t off, bytes;
+  };
+  std::deque<Quarantined> quarantine;
+  BatchXchg* d_xchg = nullptr;   // [max_active] device exchange areas
   std::vector<int> free_slots;
   std::vector<Req*> slot_req;
-  hipStream_t stream = nullptr;   // slice of small problems (256-thread workgroups)
-  hipStream_t stream2 = nullptr;  // slice of large problems (1024-thread workgroups)
-  uint32_t launch_no = 0;
-  uint64_t budget_ticks = 0;
+  std::vector<char> slot_fresh;  // SMALL slot whose descriptor the next slice uploads
+  hipStream_t stream = nullptr;
+  uint64_t budget_ticks = 0, budget_ticks_large = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Req*> pending;
@@ -174,9 +194,8 @@ bool pps_server::retire_locked() {
     pps_eval_f64(&pp, 1, xd, &f64);
     r->res.gbest_cost_f64 = f64;
     r->state = R_DONE;
-    arena.release(r->off, r->bytes);
+    quarantine.push_back({slices_enqueued, (int)s, r->off, r->bytes});
     slot_req[s] = nullptr;
-    free_slots.push_back((int)s);
     --running;
     ++completed;
     any = true;
@@ -186,15 +205,22 @@ bool pps_server::retire_locked() {
 
 void pps_server::loop() {
   (void)hipSetDevice(ctx->device);
-  std::vector<int32_t> ids, ids_large;
+  std::vector<BatchItem> items;
+  std::vector<int32_t> slots_in_slice;
   while (true) {
-    ids.clear();
-    ids_large.clear();
+    items.clear();
+    slots_in_slice.clear();
+    bool any_large = false;
     {
       std::unique_lock<std::mutex> lk(mu);
       if (retire_locked()) cv_done.notify_all();
+      while (!quarantine.empty() && quarantine.front().after <= slices_completed) {
+        const Quarantined& qd = quarantine.front();
+        arena.release(qd.off, qd.bytes);
+        free_slots.push_back(qd.slot);
+        quarantine.pop_front();
+      }
       if (stop && pending.empty() && running == 0) break;
-      const uint32_t next = launch_no + 1;
       while (!pending.empty() && !free_slots.empty()) {
         Req* r = pending.front();
         size_t off = 0;
@@ -226,51 +252,117 @@ void pps_server::loop() {
         B.v = (float*)(base + row);
         B.pb = (float*)(base + 2 * row);
         B.pc = (float*)(base + 3 * row);
-        h_fresh[s] = next;
+        r->large = r->n * r->P.d >= kLargeWork;
+        if (r->large) {
+          // parts read the device copy; upload it ahead of the slice
+          r->parts = batch_parts(r->n);
+          r->t_next = 0;
+          r->w_next = r->w;
+          (void)hipMemcpyAsync(d_desc + s, &B, sizeof(BatchProb), hipMemcpyHostToDevice, stream);
+        } else {
+          slot_fresh[s] = 1;
+        }
         slot_req[s] = r;
         ++running;
       }
       for (size_t s = 0; s < slot_req.size(); ++s) {
-        const Req* r = slot_req[s];
+        Req* r = slot_req[s];
         if (!r) continue;
-        // large swarms get 1024-thread workgroups (more memory parallelism
-        // for the one workgroup that streams them)
-        if (r->n * r->P.d >= kLargeWork) ids_large.push_back((int32_t)s);
-        else ids.push_back((int32_t)s);
+        if (!r->large) {
+          if (!slot_fresh[s] && __atomic_load_n(&h_out[s].done, __ATOMIC_ACQUIRE)) continue;
+          BatchItem it{};
+          it.slot = (int32_t)s;
+          it.kind = BATCH_SMALL;
+          it.fresh = slot_fresh[s];
+          slot_fresh[s] = 0;
+          items.push_back(it);
+          continue;
+        }
+        any_large = true;
+        if (r->t_next > (uint32_t)r->maxiters) {
+          if (r->t_next == (uint32_t)r->maxiters + 1) {
+            BatchItem it{};
+            it.slot = (int32_t)s;
+            it.kind = BATCH_FINAL;
+            it.nparts = r->parts;
+            it.t = (uint32_t)r->maxiters;
+            items.push_back(it);
+            r->t_next++;  // FINAL issued; retired when its done flag appears
+          }
+          continue;
+        }
+        for (int q = 0; q < r->parts; ++q) {
+          BatchItem it{};
+          it.slot = (int32_t)s;
+          it.kind = BATCH_PART;
+          it.part = q;
+          it.nparts = r->parts;
+          it.t = r->t_next;
+          it.w = r->w_next;
+          items.push_back(it);
+        }
+        if (r->t_next > 0) r->w_next = r->w_next * r->wdamp;
+        r->t_next++;
       }
-      if (ids.empty() && ids_large.empty()) {
-        cv_work.wait_for(lk, std::chrono::milliseconds(stop ? 1 : 5));
+      if (items.empty()) {
+        if (slices_completed < slices_enqueued) {
+          // nothing new to issue: let the last slice finish (frees quarantine)
+          lk.unlock();
+          (void)hipStreamSynchronize(stream);
+          lk.lock();
+          slices_completed = slices_enqueued;
+          continue;
+        }
+        cv_work.wait_for(lk, std::chrono::microseconds(running > 0 ? 30 : (stop ? 1000 : 5000)));
         continue;
       }
-      launch_no = next;
     }
-    const int32_t ks = (int32_t)ids.size(), kl = (int32_t)ids_large.size();
-    memcpy(h_ids, ids.data(), sizeof(int32_t) * ks);
-    memcpy(h_ids + ks, ids_large.data(), sizeof(int32_t) * kl);
-    hipError_t e = launch_batch(d_desc, h_desc, h_fresh, launch_no, h_ids, ks, 0x7FFFFFFF,
-                                budget_ticks, h_out, BATCH_BLOCK, stream);
-    if (e == hipSuccess)
-      e = launch_batch(d_desc, h_desc, h_fresh, launch_no, h_ids + ks, kl, 0x7FFFFFFF,
-                       budget_ticks, h_out, BATCH_BLOCK_LARGE, stream2);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream2);
-    std::lock_guard<std::mutex> lk(mu);
-    ++launches;
+    // Two slices in flight: work list i of slice n is list n & 1, rewritten
+    // only after slice n - 2 (its previous user) has completed.
+    hipError_t e = hipSuccess;
+    if (items.size() > (size_t)max_items) items.resize((size_t)max_items);  // rest next slice
+    const int buf = (int)(slices_enqueued & 1u);
+    if (slices_enqueued >= 2 && slices_completed + 2 <= slices_enqueued) {
+      e = hipEventSynchronize(slice_done[buf]);
+      if (e == hipSuccess) slices_completed = slices_enqueued - 1;
+    }
+    if (e == hipSuccess) {
+      memcpy(h_items[buf], items.data(), sizeof(BatchItem) * items.size());
+      e = launch_slice(d_desc, h_desc, h_items[buf], (int32_t)items.size(),
+                       any_large ? budget_ticks_large : budget_ticks, h_out, d_xchg, stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(slice_done[buf], stream);
+    std::unique_lock<std::mutex> lk(mu);
+    if (e == hipSuccess) {
+      ++slices_enqueued;
+      ++launches;
+      // wait for the older slice so the next work list (and the flags of the
+      // problems it finished) are available
+      if (slices_enqueued >= 2) {
+        const int older = (int)((slices_enqueued - 2) & 1u);
+        lk.unlock();
+        e = hipEventSynchronize(slice_done[older]);
+        lk.lock();
+        if (e == hipSuccess) slices_completed = slices_enqueued - 1;
+      }
+    }
     if (e != hipSuccess) {
       fatal = hipGetErrorString(e);
-      ids.insert(ids.end(), ids_large.begin(), ids_large.end());
-      for (int32_t s : ids) {
+      (void)hipStreamSynchronize(stream);
+      slices_completed = slices_enqueued;
+      for (size_t s = 0; s < slot_req.size(); ++s) {
         Req* r = slot_req[s];
         if (!r) continue;
         r->state = R_FAILED;
         arena.release(r->off, r->bytes);
         slot_req[s] = nullptr;
-        free_slots.push_back(s);
+        free_slots.push_back((int)s);
         --running;
       }
       cv_done.notify_all();
     }
   }
+  (void)hipStreamSynchronize(stream);
 }
 
 #define SVCHK(expr)                                                        \
@@ -285,17 +377,19 @@ void pps_server::loop() {
 
 static void destroy_partial(pps_server* sv) {
   if (!sv) return;
-  for (hipStream_t st : {sv->stream, sv->stream2})
-    if (st) {
-      (void)hipStreamSynchronize(st);
-      (void)hipStreamDestroy(st);
-    }
+  if (sv->stream) {
+    (void)hipStreamSynchronize(sv->stream);
+    (void)hipStreamDestroy(sv->stream);
+  }
   if (sv->arena_dev) (void)hipFree(sv->arena_dev);
   if (sv->d_desc) (void)hipFree(sv->d_desc);
   if (sv->h_desc) (void)hipHostFree(sv->h_desc);
   if (sv->h_out) (void)hipHostFree(sv->h_out);
-  if (sv->h_ids) (void)hipHostFree(sv->h_ids);
-  if (sv->h_fresh) (void)hipHostFree(sv->h_fresh);
+  for (int b = 0; b < 2; ++b) {
+    if (sv->h_items[b]) (void)hipHostFree(sv->h_items[b]);
+    if (sv->slice_done[b]) (void)hipEventDestroy(sv->slice_done[b]);
+  }
+  if (sv->d_xchg) (void)hipFree(sv->d_xchg);
   for (auto& kv : sv->reqs) delete kv.second;
   delete sv;
 }
@@ -325,19 +419,25 @@ int pps_server_create(pps_ctx* ctx, const pps_server_config* cfg, pps_server** o
   SVCHK(hipMalloc(&sv->d_desc, sizeof(BatchProb) * max_active));
   SVCHK(hipHostMalloc(&sv->h_desc, sizeof(BatchProb) * max_active, hipHostMallocCoherent | hipHostMallocMapped));
   SVCHK(hipHostMalloc(&sv->h_out, sizeof(BatchOut) * max_active, hipHostMallocCoherent | hipHostMallocMapped));
-  SVCHK(hipHostMalloc(&sv->h_ids, sizeof(int32_t) * max_active, hipHostMallocCoherent | hipHostMallocMapped));
-  SVCHK(hipHostMalloc(&sv->h_fresh, sizeof(uint32_t) * max_active, hipHostMallocCoherent | hipHostMallocMapped));
+  sv->max_items = 1 << 20;  // workgroups per launch
+  for (int b = 0; b < 2; ++b) {
+    SVCHK(hipHostMalloc(&sv->h_items[b], sizeof(BatchItem) * sv->max_items,
+                        hipHostMallocCoherent | hipHostMallocMapped));
+    SVCHK(hipEventCreateWithFlags(&sv->slice_done[b], hipEventDisableTiming));
+  }
+  SVCHK(hipMalloc(&sv->d_xchg, sizeof(BatchXchg) * max_active));
   memset(sv->h_out, 0, sizeof(BatchOut) * max_active);
-  memset(sv->h_fresh, 0, sizeof(uint32_t) * max_active);
   sv->slot_req.assign(max_active, nullptr);
+  sv->slot_fresh.assign(max_active, 0);
   for (int s = max_active - 1; s >= 0; --s) sv->free_slots.push_back(s);
   SVCHK(hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking));
-  SVCHK(hipStreamCreateWithFlags(&sv->stream2, hipStreamNonBlocking));
   int wclk_khz = 0;
   SVCHK(hipDeviceGetAttribute(&wclk_khz, hipDeviceAttributeWallClockRate, ctx->device));
   const int slice_us = sv->cfg.slice_us > 0 ? sv->cfg.slice_us : 1000;
   sv->cfg.slice_us = slice_us;
   sv->budget_ticks = (uint64_t)wclk_khz * (uint64_t)slice_us / 1000u;
+  sv->budget_ticks_large =
+      (uint64_t)wclk_khz * (uint64_t)std::min(slice_us, kSliceUsWithLarge) / 1000u;
   sv->th = std::thread([sv] { sv->loop(); });
   *out = sv;
   return PPS_OK;

@@ -40,6 +40,13 @@ def test_batch_equals_plain_per_problem_path():
                                        None if f.name == "rastrigin" else [1.0, 100.0],
                                        lb=np.full(d, -5.12), ub=np.full(d, 5.12),
                                        shift=np.linspace(-1, 1, d))
+    # and large swarms, which run as several workgroups per iteration
+    for q, (n, d) in zip(reqs[4:7], ((16000, 32), (9000, 8), (70000, 2))):
+        f = q.prob.f
+        q.prob = P.OptimizationProblem(f, np.zeros(d, np.float32),
+                                       None if f.name == "rastrigin" else [1.0, 100.0],
+                                       lb=np.full(d, -5.12), ub=np.full(d, 5.12))
+        q.n = n
     note("batch vs plain, 40 problems")
     res, ms = P.batch_solve(reqs)
     for q, r in zip(reqs, res):
