@@ -91,7 +91,8 @@ enum {
 enum {
   PPS_CLAMP_COMPONENT = 0,    /* min.(max.(x, lb), ub)  src/kernels.jl:15-17 */
   PPS_CLAMP_LEXICOGRAPHIC = 1 /* min(max(x, lb), ub) on whole vectors,
-                                 src/ode_pso.jl:18-19                        */
+                                 src/ode_pso.jl:18-19; the ODE estimation
+                                 path's, so PPS_OBJ_LOTKA_VOLTERRA only      */
 };
 
 /* Launch strategy for PPS_MODE_SYNC. */

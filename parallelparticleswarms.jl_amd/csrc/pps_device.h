@@ -189,7 +189,8 @@ __device__ __forceinline__ float objective(const float (&xin)[MAXD], const int d
                                            const ProbArgs& P) {
   float z[MAXD];
 #pragma unroll
-  for (int j = 0; j < MAXD; ++j) z[j] = P.has_shift ? (xin[j] - P.shift[j]) : xin[j];
+  // an absent shift is all +0.0f, and x - (+0.0f) is x bit for bit
+  for (int j = 0; j < MAXD; ++j) z[j] = xin[j] - P.shift[j];
   if constexpr (OBJ == PPS_OBJ_ROSENBROCK) {
     // res += p[2] * (x[i+1] - x[i]^2)^2 + (p[1] - x[i])^2   (test/gpu.jl:14-20)
     const float p1 = P.par[0], p2 = P.par[1];
@@ -262,7 +263,7 @@ __device__ __forceinline__ float objective(const float (&xin)[MAXD], const int d
 // philox(ctr = (i, t, k | TAG_STEP << 16, e), key = seed) for k = 0..ceil(d/2)-1
 // and use lanes (x, y, z, w) as (r1[2k], r2[2k], r1[2k+1], r2[2k+1]).
 // ---------------------------------------------------------------------------
-template <int MAXD>
+template <int MAXD, int OBJ>
 __device__ __forceinline__ void velocity_position(
     float (&x)[MAXD], float (&v)[MAXD], const float (&pb)[MAXD],
     const float* gb, float w, float c1, float c2, uint32_t i, uint32_t t,
@@ -286,7 +287,9 @@ __device__ __forceinline__ void velocity_position(
       }
     }
   }
-  if (P.clamp == PPS_CLAMP_COMPONENT) {
+  // The lexicographic clamp is the ODE path's (src/ode_pso.jl:18-19): it is
+  // compiled only into the Lotka-Volterra kernels.
+  if (OBJ != PPS_OBJ_LOTKA_VOLTERRA || P.clamp == PPS_CLAMP_COMPONENT) {
 #pragma unroll
     for (int j = 0; j < MAXD; ++j) {
       if (j < d) x[j] = jl_min(jl_max(x[j], P.lb[j]), P.ub[j]);

@@ -153,7 +153,32 @@ __global__ __launch_bounds__(BLOCK) void k_rekey(SwarmDev S, ProbArgs P, uint32_
 #define PPS_SYNC_ATTR
 #endif
 
-template <int MAXD, bool EXACT, int OBJ, int BLOCK>
+// Row addressing of the SoA state.  x, v and the best positions share one
+// row stride, so coordinate j of particle i is at the same byte offset in all
+// three.  With OFF32 (every array under 4 GiB) that offset is a 32-bit value
+// added to the array's base held in scalar registers, so a thread keeps one
+// 32-bit register per row instead of a 64-bit address per row and array.
+template <bool OFF32>
+struct RowAddr;
+template <>
+struct RowAddr<true> {
+  uint32_t i4, row4;
+  __device__ __forceinline__ RowAddr(int64_t i, int64_t npad)
+      : i4((uint32_t)i * 4u), row4((uint32_t)npad * 4u) {}
+  __device__ __forceinline__ float* at(float* base, int j) const {
+    return (float*)((char*)base + (i4 + (uint32_t)j * row4));
+  }
+};
+template <>
+struct RowAddr<false> {
+  int64_t i, npad;
+  __device__ __forceinline__ RowAddr(int64_t i_, int64_t np) : i(i_), npad(np) {}
+  __device__ __forceinline__ float* at(float* base, int j) const {
+    return base + (int64_t)j * npad + i;
+  }
+};
+
+template <int MAXD, bool EXACT, int OBJ, int BLOCK, bool OFF32>
 __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, ProbArgs P,
                                                                    StepArgs A, float w,
                                                                    uint32_t k) {
@@ -169,21 +194,43 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
   for (int64_t tile = blockIdx.x; tile * BLOCK < S.n; tile += gridDim.x) {
     const int64_t i = tile * BLOCK + threadIdx.x;
     if (i < S.n) {
+      const RowAddr<OFF32> R(i, S.npad);
       float x[MAXD], v[MAXD], pb[MAXD];
-      load_vec<MAXD>(x, S.x, S.npad, i, d);
-      load_vec<MAXD>(v, S.v, S.npad, i, d);
-      load_vec<MAXD>(pb, S.pb, S.npad, i, d);
-      float pc = S.pc[i];
-      velocity_position<MAXD>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch, S.key0,
-                              S.key1, d, P);
-      const float c = objective<MAXD, OBJ>(x, d, P);
-      store_vec<MAXD>(x, S.x, S.npad, i, d);
-      store_vec<MAXD>(v, S.v, S.npad, i, d);
-      if (c < pc) {
-        store_vec<MAXD>(x, S.pb, S.npad, i, d);
-        pc = c;
-        S.pc[i] = c;
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j) {
+        if (j < d) {
+          x[j] = *R.at(S.x, j);
+          v[j] = *R.at(S.v, j);
+          pb[j] = *R.at(S.pb, j);
+        }
       }
+      float pc = S.pc[i];
+      // Issue the whole tile's loads before any arithmetic: left alone, the
+      // scheduler interleaves them with the arithmetic pair by pair and the
+      // wave waits for memory several times per tile.
+      __builtin_amdgcn_sched_barrier(0);
+      velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch, S.key0,
+                                   S.key1, d, P);
+      const float c = objective<MAXD, OBJ>(x, d, P);
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j) {
+        if (j < d) {
+          *R.at(S.x, j) = x[j];
+          *R.at(S.v, j) = v[j];
+        }
+      }
+      // Best positions are written as whole wave rows whenever any particle
+      // of the wave improved (the others rewrite the values they loaded): a
+      // row written by a few lanes is a partial write, which HBM turns into a
+      // read-modify-write of the whole ECC word.
+      const bool imp = c < pc;
+      if (__any(imp)) {
+#pragma unroll
+        for (int j = 0; j < MAXD; ++j)
+          if (j < d) *R.at(S.pb, j) = imp ? x[j] : pb[j];
+        S.pc[i] = imp ? c : pc;
+      }
+      if (imp) pc = c;
       mykey = umin64(mykey, make_key(pc, (uint32_t)i));
     }
   }
@@ -219,7 +266,7 @@ __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepA
     for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
     uint64_t mykey = KEY_NONE;
     if (act) {
-      velocity_position<MAXD>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
+      velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
                               S.epoch, S.key0, S.key1, d, P);
       const float c = objective<MAXD, OBJ>(x, d, P);
       if (c < pc) {
@@ -285,7 +332,7 @@ __global__ __launch_bounds__(BLOCK) void k_async(SwarmDev S, ProbArgs P, StepArg
         float g[MAXD];
 #pragma unroll
         for (int j = 0; j < MAXD; ++j) g[j] = follow ? pb[j] : g0[j];
-        velocity_position<MAXD>(x, v, pb, g, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
+        velocity_position<MAXD, OBJ>(x, v, pb, g, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
                                 S.epoch, S.key0, S.key1, d, P);
         const float c = objective<MAXD, OBJ>(x, d, P);
         if (c < pc) {
@@ -339,7 +386,7 @@ __global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArg
     load_vec<MAXD>(v, S.v, S.npad, i, d);
     load_vec<MAXD>(pb, S.pb, S.npad, i, d);
     pc = S.pc[i];
-    velocity_position<MAXD>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch, S.key0,
+    velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch, S.key0,
                             S.key1, d, P);
     const float c = objective<MAXD, OBJ>(x, d, P);
     store_vec<MAXD>(x, S.x, S.npad, i, d);
@@ -407,10 +454,17 @@ struct Launch {
   }
   static hipError_t sync_step(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w,
                               uint32_t k, hipStream_t st) {
+    // 32-bit row offsets while every state array is below 4 GiB
+    const bool off32 = (uint64_t)S.npad * (uint64_t)(EXACT ? MAXD : P.d) * 4u < (1ull << 32);
     if (S.block == WGB && S.G == 1)
-      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, WGB>), dim3(1), dim3(WGB), 0, st, S, P, A, w, k);
+      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, WGB, true>), dim3(1), dim3(WGB), 0, st, S,
+                         P, A, w, k);
+    else if (off32)
+      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, BLK, true>), dim3(S.G), dim3(BLK), 0, st,
+                         S, P, A, w, k);
     else
-      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, BLK>), dim3(S.G), dim3(BLK), 0, st, S, P, A, w, k);
+      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, BLK, false>), dim3(S.G), dim3(BLK), 0, st,
+                         S, P, A, w, k);
     return hipGetLastError();
   }
   static hipError_t sync_wg(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w0,

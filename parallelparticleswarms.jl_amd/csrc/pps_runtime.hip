@@ -345,6 +345,10 @@ int pps_swarm_init(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_params
     return fail(PPS_E_INVALID, "unknown gbest_from %d", prm->gbest_from);
   if (prm->clamp != PPS_CLAMP_COMPONENT && prm->clamp != PPS_CLAMP_LEXICOGRAPHIC)
     return fail(PPS_E_INVALID, "unknown clamp %d", prm->clamp);
+  if (prm->clamp == PPS_CLAMP_LEXICOGRAPHIC && prob && prob->objective != PPS_OBJ_LOTKA_VOLTERRA)
+    return fail(PPS_E_INVALID,
+                "lexicographic clamping is the ODE estimation path's; it needs "
+                "PPS_OBJ_LOTKA_VOLTERRA (objective %d given)", prob->objective);
   if (prm->grid_per_cu < 0 || prm->grid_per_cu > 64)
     return fail(PPS_E_INVALID, "grid_per_cu %d outside 0..64", prm->grid_per_cu);
   pps_swarm* s = new pps_swarm();

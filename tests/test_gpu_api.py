@@ -6,6 +6,7 @@ import pytest
 
 import pps_oracle as O
 import ppswarm as P
+from ppswarm import _lib as L
 from parity_util import assert_same_state, note
 
 pytestmark = pytest.mark.gpu
@@ -82,3 +83,15 @@ def test_unsupported_dimension_is_rejected():
     prob = P.OptimizationProblem(P.Sphere(), np.zeros(65), lb=np.zeros(65), ub=np.ones(65))
     with pytest.raises(P.PPSError):
         P.init(prob, P.ParallelSyncPSOKernel(10))
+
+
+def test_lexicographic_clamp_is_the_ode_paths_only():
+    # whole-vector clamping belongs to parameter_estim_ode! (src/ode_pso.jl:18-19);
+    # the other objectives are compiled with component clamping only
+    note("lexicographic clamp refused outside the ODE objective")
+    d = 3
+    prob = P.OptimizationProblem(P.Rosenbrock(), np.zeros(d, np.float32), [1.0, 100.0],
+                                 lb=np.full(d, -1.0), ub=np.full(d, 10.0))
+    prm = P.swarm_params(64, L.MODE_SYNC, seed=1, clamp=L.CLAMP_LEXICOGRAPHIC)
+    with pytest.raises(P.PPSError, match="lexicographic"):
+        P.Swarm(P.context(), prob, prm)
