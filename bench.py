@@ -194,6 +194,34 @@ def ode_c3(P):
             "theta": [float(v) for v in sol.u], "loss_f64": sol.objective}
 
 
+def n_sweep(P, L):
+    """Sync step at 2^10..2^24 particles (10-D, both objectives): average launch
+    time over a timed graph replay after 1000 warm-up iterations (the
+    headline's timed range also starts after 1000 iterations).  The
+    state is 124 B per particle, so it outgrows the 256 MB Infinity Cache
+    from 2^22 particles on; below that the achieved rate includes cache hits."""
+    rows = []
+    for lg in range(10, 25, 2):
+        n = 1 << lg
+        iters = int(max(30, min(1000, 4e9 / (n * BYTES_PER_PARTICLE_ITER))))
+        us = []
+        for obj, seed in (("rastrigin", 11), ("rosenbrock", 12)):
+            s = make_swarm(P, obj, n, D, seed, L.MODE_SYNC)
+            s.run(1000)  # past the early phase where most particles still improve
+            s.run(iters)  # captures the graph of this length
+            s.run(iters)
+            us.append(s.elapsed_ms() * 1e3 / iters)
+            s.free()
+        avg = sum(us) / len(us)
+        rate = n * BYTES_PER_PARTICLE_ITER / (avg * 1e-6)
+        rows.append({"log2_n": lg, "iterations": iters, "us_per_launch": [round(u, 2) for u in us],
+                     "state_mb": round(n * 4 * (3 * D + 1) / 1e6, 1),
+                     "achieved_gbps": round(rate / 1e9, 1), "frac": round(rate / HBM_PEAK, 4),
+                     "particle_iters_per_s": n / (avg * 1e-6)})
+        log(f"[bench] sweep 2^{lg}: {avg:.2f} us/launch, {rate / 1e12:.2f} TB/s")
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,7 +229,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--extras", default="async,c0,c4,hybrid,ode",
+                    help="comma list of the legs to run besides the headline")
+    ap.add_argument("--sweep", action="store_true",
+                    help="also time the sync step at 2^10..2^24 particles (adds ~1 min)")
     args = ap.parse_args()
+    legs = set() if args.no_extras else {x.strip() for x in args.extras.split(",") if x.strip()}
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the swarm path has no multi-GPU launch", file=sys.stderr)
@@ -256,7 +289,7 @@ def main():
     for s in swarms:
         s.free()
 
-    if not args.no_extras:
+    if "async" in legs:
         # async (ParallelPSOKernel(global_update=false)) at the same shape
         a = [make_swarm(P, "rastrigin", N, D, 303, L.MODE_ASYNC),
              make_swarm(P, "rosenbrock", N, D, 404, L.MODE_ASYNC)]
@@ -273,6 +306,7 @@ def main():
         for s in a:
             s.free()
         log("[bench] async done")
+    if "c0" in legs:
         # C0: 3-D Rosenbrock, lb=-1, ub=10, 1000 particles, 500 iterations
         prob = P.OptimizationProblem(P.Rosenbrock(), np.zeros(3, np.float32), [1.0, 100.0],
                                      lb=np.full(3, -1.0), ub=np.full(3, 10.0))
@@ -289,11 +323,17 @@ def main():
                      "particle_iters_per_s": 1000 * 500 / float(np.median(times[1:]))}
         log("[bench] c0 done")
 
-    if not args.no_extras:
+    if args.sweep:
+        out["sweep"] = n_sweep(P, L)
+        log("[bench] sweep done")
+
+    if "c4" in legs:
         out["batched"], out["served"] = c4_mix(P, L, ctx)
         log("[bench] c4 done")
+    if "hybrid" in legs:
         out["hybrid"] = hybrid_c2(P, L, ctx)
         log("[bench] hybrid done")
+    if "ode" in legs:
         out["ode"] = ode_c3(P)
         log("[bench] ode done")
 
