@@ -31,7 +31,8 @@
 extern "C" {
 #endif
 
-#define PPS_ABI_VERSION 1
+#define PPS_ABI_VERSION 2
+#define PPS_MAXCONS 8  /* constraints per problem */
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -119,6 +120,20 @@ typedef struct pps_problem {
    * (int)params[3] steps. */
   const float* obs;
   int32_t n_obs;
+  /* Inequality constraints g_k(x) <= 0, k = 0..n_cons-1 (n_cons <= PPS_MAXCONS),
+   * the reference's `OptimizationFunction(f; cons)` for the separable quadratic
+   * family
+   *     g_k(x) = sum_j a_kj x_j + sum_j q_kj x_j^2 + c_k
+   * stored per constraint as 2d+1 floats [a_k0..a_k(d-1), q_k0..q_k(d-1), c_k].
+   * Handled as the reference does (src/kernels.jl:28-36, src/utils.jl:320-372):
+   * cost = f(x) + h(iter) * (sum_k theta(q_k) + sum_k theta(q_k) q_k^gamma(q_k)),
+   * q_k = max(g_k(x), 0), default theta/gamma schedules, h = sqrt, and `iter` as
+   * the reference passes it (init: 1; sync and queue-lock: particle index + 1;
+   * async: iteration of the launch + 1).  lcons/ucons are not used by the
+   * reference's penalty and have no field here.  Rosenbrock, Rastrigin and
+   * Sphere objectives; NULL / 0 for none. */
+  const float* cons;
+  int32_t n_cons;
 } pps_problem;
 
 /* Swarm/algorithm parameters (ParallelSyncPSOKernel / ParallelPSOKernel

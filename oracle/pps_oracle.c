@@ -48,6 +48,9 @@ typedef struct ora_problem {
   float lb[ORA_MAXD], ub[ORA_MAXD], shift[ORA_MAXD], par[ORA_MAXPAR];
   const float* obs;
   int32_t n_obs, steps_per_obs;
+  /* constraints g_k(x) <= 0: n_cons rows [a_0..a_{d-1}, q_0..q_{d-1}, c] */
+  const float* cons;
+  int32_t n_cons;
 } ora_problem;
 
 /* ---------------------------------------------------------------- Philox */
@@ -282,21 +285,75 @@ static void move_particle(float* p, const ora_problem* P, const float* gb, float
   }
 }
 
-/* cost = f(x); if cost < best_cost: best = x (src/kernels.jl:19-24). */
-static void evaluate_particle(float* p, const ora_problem* P) {
+/* Constraint penalty, calc_penalty (src/utils.jl:353-372) with the default
+ * schedules theta_default / gamma_default (src/utils.jl:320-336) and h = sqrt
+ * (src/algorithms.jl:88-89):
+ *   q = max.(cons(x), 0); penalty = sum(theta.(q));
+ *   for i: penalty += theta(q_i) * q_i^gamma(q_i);  penalty = h(iter) * penalty
+ * theta's thresholds are Float64 literals (0.001, 0.1, 1.0) compared after
+ * promotion; gamma's is the integer 1.  q^1 = q and q^2 = q*q (the correctly
+ * rounded power; the reference's @fastmath pow of a Float32 by 1f0 or 2f0).
+ * cons(x) is the separable quadratic family: nonzero linear terms in index
+ * order, then nonzero quadratic terms, then + c (for the reference's test
+ * constraints, test/constraints.jl:11-14, their own evaluation order). */
+static float theta_default(float q) {
+  double qd = (double)q;
+  if (qd < 0.001) return 10.0f;
+  if (qd <= 0.1) return 20.0f;
+  if (qd <= 1.0) return 100.0f;
+  return 300.0f;
+}
+
+float ora_constraint(const ora_problem* P, int k, const float* x) {
   int d = P->d;
-  COST(p, d) = ora_objective(P, POS(p, d));
+  const float* row = P->cons + (size_t)k * (2 * d + 1);
+  float g = 0.0f;
+  for (int j = 0; j < d; ++j)
+    if (row[j] != 0.0f) g = g + row[j] * x[j];
+  for (int j = 0; j < d; ++j)
+    if (row[d + j] != 0.0f) g = g + row[d + j] * (x[j] * x[j]);
+  return g + row[2 * d];
+}
+
+float ora_penalty(const ora_problem* P, const float* x, float iter) {
+  float q[16];
+  for (int k = 0; k < P->n_cons; ++k) q[k] = jl_max(ora_constraint(P, k, x), 0.0f);
+  float pen = theta_default(q[0]);
+  for (int k = 1; k < P->n_cons; ++k) pen = pen + theta_default(q[k]);
+  for (int k = 0; k < P->n_cons; ++k) {
+    float pw = q[k] < 1.0f ? q[k] : q[k] * q[k];
+    pen = pen + theta_default(q[k]) * pw;
+  }
+  return sqrtf(iter) * pen;
+}
+
+/* handle_constraints (src/kernels.jl:28-36): the objective, plus the penalty
+ * at h's argument `iter` when the problem has constraints. */
+float ora_cost(const ora_problem* P, const float* x, float iter) {
+  float c = ora_objective(P, x);
+  if (P->n_cons > 0) c = c + ora_penalty(P, x, iter);
+  return c;
+}
+
+/* cost = f(x); if cost < best_cost: best = x (src/kernels.jl:19-24). */
+static void evaluate_particle(float* p, const ora_problem* P, float iter) {
+  int d = P->d;
+  COST(p, d) = ora_cost(P, POS(p, d), iter);
   if (COST(p, d) < BCOST(p, d)) {
     memcpy(BPOS(p, d), POS(p, d), sizeof(float) * d);
     BCOST(p, d) = COST(p, d);
   }
 }
 
+/* `iter` is h's argument as the reference computes it at the call site:
+ * 1-based particle index + 1 in the sync and queue-lock kernels
+ * (src/kernels.jl:77,151,191 pass the index i as `iter`), 1-based iteration of
+ * the launch + 1 in the async kernel (src/kernels.jl:213-214). */
 static void step_particle(float* p, const ora_problem* P, const float* gb, float w,
                           float c1, float c2, uint32_t i, uint32_t t, uint32_t epoch,
-                          const uint32_t key[2]) {
+                          const uint32_t key[2], float iter) {
   move_particle(p, P, gb, w, c1, c2, i, t, epoch, key);
-  evaluate_particle(p, P);
+  evaluate_particle(p, P, iter);
 }
 
 /* Uniform init (src/utils.jl:1-5,57-84): x_j = rand() * (ub_j - lb_j) + lb_j,
@@ -321,7 +378,7 @@ void ora_init(const ora_problem* P, int64_t n, uint64_t seed, uint32_t epoch, fl
       POS(p, d)[j] = u01(r[j & 3]) * (P->ub[j] - P->lb[j]) + P->lb[j];
       VEL(p, d)[j] = 0.0f;
     }
-    COST(p, d) = ora_objective(P, POS(p, d));
+    COST(p, d) = ora_cost(P, POS(p, d), 1.0f); /* calc_penalty(.., 1, ..), src/utils.jl:72-73 */
     memcpy(BPOS(p, d), POS(p, d), sizeof(float) * d);
     BCOST(p, d) = COST(p, d);
   }
@@ -352,7 +409,8 @@ void ora_sync_run(const ora_problem* P, int64_t n, uint64_t seed, uint32_t epoch
     uint32_t t = t0 + (uint32_t)k;
 #pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
     for (int64_t i = 0; i < n; ++i)
-      step_particle(parts + i * s, P, gb, w, c1, c2, (uint32_t)i, t, epoch, key);
+      step_particle(parts + i * s, P, gb, w, c1, c2, (uint32_t)i, t, epoch, key,
+                    (float)(i + 2));
     int64_t b = ora_argmin(parts, n, d);
     const float* src = gbest_from == ORA_GBEST_CURRENT ? POS(parts + b * s, d)
                                                        : BPOS(parts + b * s, d);
@@ -387,7 +445,8 @@ void ora_async_run(const ora_problem* P, int64_t n, uint64_t seed, uint32_t epoc
     float gc = g0c;
     float w = w0;
     for (int32_t k = 1; k <= iters; ++k) {
-      step_particle(p, P, g, w, c1, c2, (uint32_t)i, t0 + (uint32_t)k, epoch, key);
+      step_particle(p, P, g, w, c1, c2, (uint32_t)i, t0 + (uint32_t)k, epoch, key,
+                    (float)(k + 1));
       if (BCOST(p, d) < gc) {
         memcpy(g, BPOS(p, d), sizeof(float) * d);
         gc = BCOST(p, d);
@@ -423,7 +482,8 @@ void ora_qlock_run(const ora_problem* P, int64_t n, uint64_t seed, uint32_t epoc
       float gb[ORA_MAXD];
       memcpy(gb, gb_pos, sizeof(float) * d);
       for (int64_t i = b0; i < b1; ++i)
-        step_particle(parts + i * s, P, gb, w, c1, c2, (uint32_t)i, t, epoch, key);
+        step_particle(parts + i * s, P, gb, w, c1, c2, (uint32_t)i, t, epoch, key,
+                      (float)(i + 2));
       int64_t count = 0, best = -1;
       for (int64_t i = b0; i < b1; ++i) {
         float bc = BCOST(parts + i * s, d);

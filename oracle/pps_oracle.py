@@ -35,7 +35,7 @@ class _Prob(C.Structure):
         ("obj", C.c_int32), ("d", C.c_int32), ("has_shift", C.c_int32), ("clamp", C.c_int32),
         ("lb", C.c_float * MAXD), ("ub", C.c_float * MAXD), ("shift", C.c_float * MAXD),
         ("par", C.c_float * MAXPAR), ("obs", C.POINTER(C.c_float)), ("n_obs", C.c_int32),
-        ("steps_per_obs", C.c_int32),
+        ("steps_per_obs", C.c_int32), ("cons", C.POINTER(C.c_float)), ("n_cons", C.c_int32),
     ]
 
 
@@ -69,6 +69,12 @@ def lib():
         L.ora_sin2pi.restype = C.c_float
         L.ora_objective.argtypes = [C.POINTER(_Prob), fp]
         L.ora_objective.restype = C.c_float
+        L.ora_constraint.argtypes = [C.POINTER(_Prob), C.c_int, fp]
+        L.ora_constraint.restype = C.c_float
+        L.ora_penalty.argtypes = [C.POINTER(_Prob), fp, C.c_float]
+        L.ora_penalty.restype = C.c_float
+        L.ora_cost.argtypes = [C.POINTER(_Prob), fp, C.c_float]
+        L.ora_cost.restype = C.c_float
         L.ora_lv_sse.argtypes = [fp, C.POINTER(_Prob)]
         L.ora_lv_sse.restype = C.c_float
         L.ora_eval.argtypes = [C.POINTER(_Prob), C.c_int64, fp, fp]
@@ -112,6 +118,7 @@ class Problem:
     shift: np.ndarray | None = None
     clamp: int = CLAMP_COMPONENT
     obs: np.ndarray | None = None  # Lotka-Volterra observations, (n_obs, 2)
+    cons: np.ndarray | None = None  # constraints, (n_cons, 2d + 1) rows [a, q, c]
 
     def c(self) -> _Prob:
         p = _Prob()
@@ -133,7 +140,26 @@ class Problem:
             p.obs = _fp(self._obs_keep)
             p.n_obs = len(self._obs_keep) // 2
             p.steps_per_obs = int(par[3])
+        if self.cons is not None:
+            self._cons_keep = np.ascontiguousarray(self.cons, np.float32).reshape(-1)
+            p.cons = _fp(self._cons_keep)
+            p.n_cons = len(self._cons_keep) // (2 * self.d + 1)
         return p
+
+
+def constraint(prob: Problem, k: int, x) -> np.float32:
+    x = np.ascontiguousarray(x, np.float32)
+    return np.float32(lib().ora_constraint(C.byref(prob.c()), int(k), _fp(x)))
+
+
+def penalty(prob: Problem, x, iter_arg: float) -> np.float32:
+    x = np.ascontiguousarray(x, np.float32)
+    return np.float32(lib().ora_penalty(C.byref(prob.c()), _fp(x), float(iter_arg)))
+
+
+def cost(prob: Problem, x, iter_arg: float) -> np.float32:
+    x = np.ascontiguousarray(x, np.float32)
+    return np.float32(lib().ora_cost(C.byref(prob.c()), _fp(x), float(iter_arg)))
 
 
 def philox(ctr, key):

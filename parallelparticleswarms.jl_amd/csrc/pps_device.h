@@ -259,6 +259,52 @@ __device__ __forceinline__ float objective(const float (&xin)[MAXD], const int d
 // stream:
 //   v' = (w*v + (c1*r1)*(pbest - x)) + (c2*r2)*(gbest - x)
 //   x' = clamp(x + v')
+// ---------------------------------------------------------------------------
+// Constraint penalty of the reference (src/utils.jl:320-372), called from
+// handle_constraints (src/kernels.jl:28-36):
+//   q_k = max(g_k(x), 0)                      (Julia max: NaN-propagating)
+//   theta(q) = 10 if q < 0.001, 20 if q <= 0.1, 100 if q <= 1, else 300
+//   gamma(q) = 1 if q < 1, else 2             (the thresholds are Float64
+//                                              literals compared after promotion)
+//   penalty = h * ((sum_k theta(q_k)) + theta(q_0) q_0^gamma(q_0) + ...)
+// with q^1 = q and q^2 = q*q (the correctly rounded power).  g_k sums its
+// nonzero linear terms in index order, then its nonzero quadratic terms, then
+// adds c_k: for the reference's test constraints (test/constraints.jl:11-14)
+// that is their own FP32 evaluation order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float penalty_theta(float q) {
+  const double qd = (double)q;
+  return qd < 0.001 ? 10.0f : (qd <= 0.1 ? 20.0f : (qd <= 1.0 ? 100.0f : 300.0f));
+}
+
+template <int MAXD>
+__device__ float constraint_penalty(const float (&x)[MAXD], const int d, const ProbArgs& P,
+                                    float h) {
+  float qs[PPS_MAXCONS];
+  float sum_theta = 0.0f;
+  for (int k = 0; k < P.n_cons; ++k) {
+    const float* row = P.cons + (int64_t)k * (2 * d + 1);
+    float g = 0.0f;
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j)
+      if (j < d && row[j] != 0.0f) g = g + row[j] * x[j];
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j)
+      if (j < d && row[d + j] != 0.0f) g = g + row[d + j] * (x[j] * x[j]);
+    g = g + row[2 * d];
+    const float q = jl_max(g, 0.0f);
+    qs[k] = q;
+    sum_theta = k == 0 ? penalty_theta(q) : sum_theta + penalty_theta(q);
+  }
+  float pen = 0.0f + sum_theta;
+  for (int k = 0; k < P.n_cons; ++k) {
+    const float q = qs[k];
+    const float pw = q < 1.0f ? q : q * q;
+    pen = pen + penalty_theta(q) * pw;
+  }
+  return h * pen;
+}
+
 // Random stream: for particle i at iteration t (1-based) of epoch e, call
 // philox(ctr = (i, t, k | TAG_STEP << 16, e), key = seed) for k = 0..ceil(d/2)-1
 // and use lanes (x, y, z, w) as (r1[2k], r2[2k], r1[2k+1], r2[2k+1]).

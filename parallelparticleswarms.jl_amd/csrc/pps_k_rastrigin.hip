@@ -2,5 +2,7 @@
 #include "pps_kernels_impl.h"
 
 namespace pps {
-const KernelSet* kernels_rastrigin(int d) { return generic_set<PPS_OBJ_RASTRIGIN>(d); }
+const KernelSet* kernels_rastrigin(int d, bool constrained) {
+  return generic_set<PPS_OBJ_RASTRIGIN>(d, constrained);
+}
 }  // namespace pps

@@ -2,5 +2,7 @@
 #include "pps_kernels_impl.h"
 
 namespace pps {
-const KernelSet* kernels_rosenbrock(int d) { return generic_set<PPS_OBJ_ROSENBROCK>(d); }
+const KernelSet* kernels_rosenbrock(int d, bool constrained) {
+  return generic_set<PPS_OBJ_ROSENBROCK>(d, constrained);
+}
 }  // namespace pps

@@ -2,5 +2,7 @@
 #include "pps_kernels_impl.h"
 
 namespace pps {
-const KernelSet* kernels_sphere(int d) { return generic_set<PPS_OBJ_SPHERE>(d); }
+const KernelSet* kernels_sphere(int d, bool constrained) {
+  return generic_set<PPS_OBJ_SPHERE>(d, constrained);
+}
 }  // namespace pps

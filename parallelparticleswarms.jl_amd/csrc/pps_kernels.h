@@ -37,8 +37,9 @@ struct KernelSet {
                            uint32_t k, hipStream_t);
 };
 
-// nullptr when (obj, d) has no compiled kernels.
-const KernelSet* find_kernels(int obj, int d);
+// nullptr when (obj, d) has no compiled kernels (constrained: no kernels that
+// evaluate the constraint penalty).
+const KernelSet* find_kernels(int obj, int d, bool constrained = false);
 
 // Reduce the block keys of parity `par` into swarm.gbrec (cost, idx, pos).
 hipError_t launch_finalize(const SwarmDev&, int d, uint32_t par, hipStream_t);

@@ -85,20 +85,20 @@ hipError_t launch_argmin_f32(const float* vals, int64_t n, uint64_t* out_key, hi
   return hipGetLastError();
 }
 
-const KernelSet* kernels_rosenbrock(int d);
-const KernelSet* kernels_rastrigin(int d);
-const KernelSet* kernels_sphere(int d);
+const KernelSet* kernels_rosenbrock(int d, bool constrained);
+const KernelSet* kernels_rastrigin(int d, bool constrained);
+const KernelSet* kernels_sphere(int d, bool constrained);
 const KernelSet* kernels_two_dim(int obj, int d);
 const KernelSet* kernels_lotka_volterra(int d);
 
-const KernelSet* find_kernels(int obj, int d) {
+const KernelSet* find_kernels(int obj, int d, bool constrained) {
   if (d < 1 || d > PPS_MAXD) return nullptr;
   switch (obj) {
-    case PPS_OBJ_ROSENBROCK: return kernels_rosenbrock(d);
-    case PPS_OBJ_RASTRIGIN: return kernels_rastrigin(d);
-    case PPS_OBJ_SPHERE: return kernels_sphere(d);
-    case PPS_OBJ_LOTKA_VOLTERRA: return kernels_lotka_volterra(d);
-    default: return kernels_two_dim(obj, d);
+    case PPS_OBJ_ROSENBROCK: return kernels_rosenbrock(d, constrained);
+    case PPS_OBJ_RASTRIGIN: return kernels_rastrigin(d, constrained);
+    case PPS_OBJ_SPHERE: return kernels_sphere(d, constrained);
+    case PPS_OBJ_LOTKA_VOLTERRA: return constrained ? nullptr : kernels_lotka_volterra(d);
+    default: return constrained ? nullptr : kernels_two_dim(obj, d);
   }
 }
 

@@ -105,6 +105,23 @@ __device__ __forceinline__ void publish_block(const SwarmDev& S, uint32_t par, i
   if (threadIdx.x == 0) S.bkey[(int64_t)par * S.G + b] = bk;
 }
 
+// Cost of a position: the objective, plus the constraint penalty when the
+// problem has constraints (src/kernels.jl:28-36).  `iter1` is h's argument
+// as the reference computes it at each call site (init: 1; sync and queue-lock
+// kernels: 1-based particle index + 1; async: 1-based iteration of the launch
+// + 1).  Constrained problems run on the runtime-dimension instantiations, so
+// the branch is compiled only there.
+template <int MAXD, bool EXACT, int OBJ>
+__device__ __forceinline__ float cost_at(const float (&x)[MAXD], const int d, const ProbArgs& P,
+                                         float iter1) {
+  float c = objective<MAXD, OBJ>(x, d, P);
+  if constexpr (!EXACT && (OBJ == PPS_OBJ_ROSENBROCK || OBJ == PPS_OBJ_RASTRIGIN ||
+                           OBJ == PPS_OBJ_SPHERE)) {
+    if (P.n_cons > 0) c = c + constraint_penalty<MAXD>(x, d, P, sqrtf(iter1));
+  }
+  return c;
+}
+
 // ---------------------------------------------------------------------------
 template <int MAXD, bool EXACT, int OBJ, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_init(SwarmDev S, ProbArgs P, uint32_t par) {
@@ -118,7 +135,7 @@ __global__ __launch_bounds__(BLOCK) void k_init(SwarmDev S, ProbArgs P, uint32_t
       init_position<MAXD>(x, (uint32_t)i, S.epoch, S.key0, S.key1, d, P);
 #pragma unroll
       for (int j = 0; j < MAXD; ++j) z[j] = 0.0f;
-      const float c = objective<MAXD, OBJ>(x, d, P);
+      const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, 1.0f);
       store_vec<MAXD>(x, S.x, S.npad, i, d);
       store_vec<MAXD>(z, S.v, S.npad, i, d);
       store_vec<MAXD>(x, S.pb, S.npad, i, d);
@@ -211,7 +228,7 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
       __builtin_amdgcn_sched_barrier(0);
       velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch, S.key0,
                                    S.key1, d, P);
-      const float c = objective<MAXD, OBJ>(x, d, P);
+      const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2));
 #pragma unroll
       for (int j = 0; j < MAXD; ++j) {
         if (j < d) {
@@ -268,7 +285,7 @@ __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepA
     if (act) {
       velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
                               S.epoch, S.key0, S.key1, d, P);
-      const float c = objective<MAXD, OBJ>(x, d, P);
+      const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2));
       if (c < pc) {
 #pragma unroll
         for (int j = 0; j < MAXD; ++j) pb[j] = x[j];
@@ -334,7 +351,7 @@ __global__ __launch_bounds__(BLOCK) void k_async(SwarmDev S, ProbArgs P, StepArg
         for (int j = 0; j < MAXD; ++j) g[j] = follow ? pb[j] : g0[j];
         velocity_position<MAXD, OBJ>(x, v, pb, g, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
                                 S.epoch, S.key0, S.key1, d, P);
-        const float c = objective<MAXD, OBJ>(x, d, P);
+        const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(k + 1));
         if (c < pc) {
 #pragma unroll
           for (int j = 0; j < MAXD; ++j) pb[j] = x[j];
@@ -388,7 +405,7 @@ __global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArg
     pc = S.pc[i];
     velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch, S.key0,
                             S.key1, d, P);
-    const float c = objective<MAXD, OBJ>(x, d, P);
+    const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2));
     store_vec<MAXD>(x, S.x, S.npad, i, d);
     store_vec<MAXD>(v, S.v, S.npad, i, d);
     if (c < pc) {
@@ -498,11 +515,12 @@ struct Launch {
 };
 
 // (objective, dimension) -> compiled kernel set.  Dimensions used by the
-// benchmark shapes get exact-size instantiations; others run in the next
-// power-of-two bucket with a runtime dimension.
+// benchmark shapes get exact-size instantiations; others, and every
+// constrained problem, run in the next power-of-two bucket with a runtime
+// dimension.
 template <int OBJ>
-const KernelSet* generic_set(int d) {
-  switch (d) {
+const KernelSet* generic_set(int d, bool constrained) {
+  if (!constrained) switch (d) {
     case 1: return Launch<1, true, OBJ>::set();
     case 2: return Launch<2, true, OBJ>::set();
     case 3: return Launch<3, true, OBJ>::set();

@@ -86,6 +86,20 @@ int fill_prob_args(const pps_problem* prob, ProbArgs* P) {
     P->n_obs = prob->n_obs;
     P->steps_per_obs = (int32_t)spo;
   }
+  if (prob->n_cons < 0 || prob->n_cons > PPS_MAXCONS)
+    return fail(PPS_E_INVALID, "n_cons %d outside 0..%d", prob->n_cons, PPS_MAXCONS);
+  if (prob->n_cons > 0) {
+    if (!prob->cons) return fail(PPS_E_INVALID, "cons is NULL with n_cons = %d", prob->n_cons);
+    if (prob->objective != PPS_OBJ_ROSENBROCK && prob->objective != PPS_OBJ_RASTRIGIN &&
+        prob->objective != PPS_OBJ_SPHERE)
+      return fail(PPS_E_INVALID,
+                  "constraints are supported with the Rosenbrock, Rastrigin and Sphere "
+                  "objectives (objective %d given)", prob->objective);
+    for (int k = 0; k < prob->n_cons * (2 * d + 1); ++k)
+      if (!std::isfinite(prob->cons[k]))
+        return fail(PPS_E_INVALID, "constraint coefficient %d is not finite", k);
+    P->n_cons = prob->n_cons;  // the coefficients are copied to device memory by the caller
+  }
   return PPS_OK;
 }
 
@@ -292,6 +306,7 @@ static int swarm_alloc(pps_swarm* s, const pps_problem* prob) {
   const size_t o_rec = take(sizeof(GbRec));
   const size_t o_t = take(sizeof(uint32_t) * 64);
   const size_t o_obs = take(sizeof(float) * 2 * (size_t)std::max(0, s->P.n_obs));
+  const size_t o_cons = take(sizeof(float) * (size_t)s->P.n_cons * (2 * d + 1));
   s->mem_bytes = off;
   HIPCHK(hipSetDevice(s->ctx->device));
   hipError_t e = hipMalloc(&s->mem, off);
@@ -319,6 +334,12 @@ static int swarm_alloc(pps_swarm* s, const pps_problem* prob) {
     HIPCHK(hipMemcpyAsync(obs, prob->obs, sizeof(float) * 2 * s->P.n_obs, hipMemcpyHostToDevice,
                           s->ctx->stream));
     s->P.obs = obs;
+  }
+  if (s->P.n_cons > 0) {
+    float* cons = (float*)(base + o_cons);
+    HIPCHK(hipMemcpyAsync(cons, prob->cons, sizeof(float) * s->P.n_cons * (2 * d + 1),
+                          hipMemcpyHostToDevice, s->ctx->stream));
+    s->P.cons = cons;
   }
   return PPS_OK;
 }
@@ -357,7 +378,7 @@ int pps_swarm_init(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_params
   int rc = fill_prob_args(prob, &s->P);
   if (rc == PPS_OK) {
     s->P.clamp = prm->clamp;
-    s->ks = find_kernels(s->P.obj, s->P.d);
+    s->ks = find_kernels(s->P.obj, s->P.d, s->P.n_cons > 0);
     if (!s->ks) rc = fail(PPS_E_INVALID, "no kernels for objective %d at d=%d", s->P.obj, s->P.d);
   }
   if (rc == PPS_OK) rc = swarm_prepare_launch(s);
@@ -638,6 +659,10 @@ int pps_swarm_lbfgs(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x, floa
   if (!s || !lp) return fail(PPS_E_INVALID, "NULL argument");
   if (!lbfgs_supported(s->P.obj, s->P.d))
     return fail(PPS_E_INVALID, "no analytic gradient for objective %d at d=%d", s->P.obj, s->P.d);
+  if (s->P.n_cons > 0)
+    return fail(PPS_E_INVALID,
+                "the local L-BFGS phase minimises the objective alone; it has no constraint "
+                "handling (the swarm has %d constraints)", s->P.n_cons);
   if (lp->history < 1 || lp->history > 16) return fail(PPS_E_INVALID, "history must be 1..16");
   if (lp->maxiters < 0 || lp->max_backtracks < 0) return fail(PPS_E_INVALID, "negative iteration count");
   if (!(lp->armijo_rho > 0.0f && lp->armijo_rho < 1.0f)) return fail(PPS_E_INVALID, "armijo_rho must be in (0,1)");

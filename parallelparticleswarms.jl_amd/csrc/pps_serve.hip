@@ -450,6 +450,8 @@ int pps_server_submit(pps_server* sv, const pps_problem* prob, const pps_swarm_p
     return fail(PPS_E_INVALID,
                 "served problems are Rosenbrock or Rastrigin with d in 1..32 (got objective %d, "
                 "d=%d)", prob->objective, prob->d);
+  if (prob->n_cons != 0)
+    return fail(PPS_E_INVALID, "served problems are unconstrained (n_cons = %d)", prob->n_cons);
   if (prm->mode != PPS_MODE_SYNC || prm->gbest_from != PPS_GBEST_PBEST ||
       prm->clamp != PPS_CLAMP_COMPONENT)
     return fail(PPS_E_INVALID, "served problems use sync PSO with best-position global best");

@@ -42,6 +42,10 @@ struct ProbArgs {
   const float* obs;  // Lotka-Volterra observations (device), 2*n_obs floats
   int32_t n_obs;
   int32_t steps_per_obs;
+  // Inequality constraints (device): n_cons rows of 2d+1 floats
+  // [a_0..a_{d-1}, q_0..q_{d-1}, c], g(x) = sum a_j x_j + sum q_j x_j^2 + c.
+  const float* cons;
+  int32_t n_cons;
 };
 
 // Global-best record in device memory (async and queue-lock modes keep their

@@ -42,6 +42,8 @@ struct CProblem            # pps_problem
     shift::Ptr{Float32}
     obs::Ptr{Float32}
     n_obs::Int32
+    cons::Ptr{Float32}
+    n_cons::Int32
 end
 
 struct CSwarmParams        # pps_swarm_params
@@ -91,8 +93,11 @@ objective_code(f) = throw(ArgumentError(
     "the MI355X backend runs a closed objective set; wrap it as PPSObjective(:name), got $(typeof(f))"))
 
 const CTX = Ref{Ptr{Cvoid}}(C_NULL)
+const ABI_VERSION = Int32(2)   # include/pps.h PPS_ABI_VERSION mirrored by CProblem & co.
 function context(b::MI355XBackend)
     if CTX[] == C_NULL
+        v = ccall((:pps_abi_version, libpps), Cint, ())
+        v == ABI_VERSION || error("$libpps has ABI version $v, this binding mirrors $ABI_VERSION")
         h = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:pps_ctx_create, libpps), Cint, (Cint, Ptr{Ptr{Cvoid}}), b.device, h))
         CTX[] = h[]
@@ -100,12 +105,43 @@ function context(b::MI355XBackend)
     return CTX[]
 end
 
+"""
+    PPSQuadraticConstraints(linear, quadratic, c)
+
+Constraints g_k(x) = sum_j linear[k,j] x_j + sum_j quadratic[k,j] x_j^2 + c[k] <= 0
+(k <= 8) for `OptimizationFunction(f; cons = PPSQuadraticConstraints(...))`.  Callable
+as `cons(x, p)` (an SVector of the g_k), so the same problem runs on the reference's
+own backends.  test/constraints.jl's `conss` is
+`PPSQuadraticConstraints([-1 2; 1 -2; 0 0], [0 0; 0 0; 0.25 1], [-1, 1, -1])`.
+"""
+struct PPSQuadraticConstraints{M <: AbstractMatrix, V <: AbstractVector}
+    linear::M
+    quadratic::M
+    c::V
+end
+function (g::PPSQuadraticConstraints)(x, p)
+    m = length(g.c)
+    return SVector{m}(ntuple(k -> begin
+        acc = zero(eltype(x))
+        for j in eachindex(x)
+            g.linear[k, j] != 0 && (acc += eltype(x)(g.linear[k, j]) * x[j])
+        end
+        for j in eachindex(x)
+            g.quadratic[k, j] != 0 && (acc += eltype(x)(g.quadratic[k, j]) * (x[j] * x[j]))
+        end
+        acc + eltype(x)(g.c[k])
+    end, m))
+end
+cons_rows(g::PPSQuadraticConstraints, d) = Float32.(vec(permutedims(
+    hcat(g.linear, g.quadratic, reshape(g.c, :, 1)))))   # row-major [a | q | c] per constraint
+
 # Keeps the Julia arrays that a CProblem points into alive.
 struct ProblemArrays
     lb::Vector{Float32}
     ub::Vector{Float32}
     p::Vector{Float32}
     shift::Union{Nothing, Vector{Float32}}
+    cons::Vector{Float32}
 end
 
 function cproblem(prob::OptimizationProblem)
@@ -113,14 +149,20 @@ function cproblem(prob::OptimizationProblem)
     d = length(prob.u0)
     (prob.lb === nothing || prob.ub === nothing) &&
         throw(ArgumentError("the MI355X backend samples the initial swarm in [lb, ub]"))
-    prob.f.cons === nothing || throw(ArgumentError("constraints are not supported"))
+    g = prob.f.cons
+    g === nothing || g isa PPSQuadraticConstraints || throw(ArgumentError(
+        "the MI355X backend evaluates constraints given as PPSQuadraticConstraints"))
+    # lcons/ucons are not read: the reference's penalty uses g(x) <= 0 (src/utils.jl:359-361)
     a = ProblemArrays(Float32.(collect(prob.lb)), Float32.(collect(prob.ub)),
         prob.p === nothing || prob.p isa SciMLBase.NullParameters ? Float32[] :
-        Float32.(collect(prob.p)), f isa PPSObjective ? f.shift : nothing)
+        Float32.(collect(prob.p)), f isa PPSObjective ? f.shift : nothing,
+        g === nothing ? Float32[] : cons_rows(g, d))
     c = CProblem(objective_code(f), Int32(d), pointer(a.lb), pointer(a.ub),
         isempty(a.p) ? Ptr{Float32}(C_NULL) : pointer(a.p), Int32(length(a.p)),
         a.shift === nothing ? Ptr{Float32}(C_NULL) : pointer(a.shift),
-        Ptr{Float32}(C_NULL), Int32(0))
+        Ptr{Float32}(C_NULL), Int32(0),
+        isempty(a.cons) ? Ptr{Float32}(C_NULL) : pointer(a.cons),
+        Int32(g === nothing ? 0 : length(g.c)))
     return c, a
 end
 
@@ -129,10 +171,11 @@ function eval_f64(f::PPSObjective, x, p)
     prob = (; f = (; f = f, cons = nothing), u0 = x, p = p,
         lb = fill(-Inf32, d), ub = fill(Inf32, d))
     a = ProblemArrays(Float32[], Float32[], p === nothing ? Float32[] : Float32.(collect(p)),
-        f.shift)
+        f.shift, Float32[])
     c = CProblem(objective_code(f), Int32(d), C_NULL, C_NULL,
         isempty(a.p) ? Ptr{Float32}(C_NULL) : pointer(a.p), Int32(length(a.p)),
-        a.shift === nothing ? Ptr{Float32}(C_NULL) : pointer(a.shift), C_NULL, Int32(0))
+        a.shift === nothing ? Ptr{Float32}(C_NULL) : pointer(a.shift), C_NULL, Int32(0),
+        C_NULL, Int32(0))
     xs = Float64.(collect(x))
     out = Ref{Float64}(0.0)
     GC.@preserve a xs begin
@@ -255,6 +298,6 @@ function solve(prob::OptimizationProblem, alg::HybridPSO{<:MI355XBackend, <:Simp
         stats = OptimizationStats(; time = (time() - t0) + sol_pso.stats.time))
 end
 
-export MI355XBackend, PPSObjective
+export MI355XBackend, PPSObjective, PPSQuadraticConstraints
 
 end # module

@@ -39,10 +39,14 @@ class PPSError(RuntimeError):
         self.code = code
 
 
+ABI_VERSION = 2  # include/pps.h PPS_ABI_VERSION mirrored by the structures below
+
+
 class pps_problem(C.Structure):
     _fields_ = [
         ("objective", C.c_int32), ("d", C.c_int32), ("lb", fp), ("ub", fp), ("params", fp),
         ("n_params", C.c_int32), ("shift", fp), ("obs", fp), ("n_obs", C.c_int32),
+        ("cons", fp), ("n_cons", C.c_int32),
     ]
 
 
@@ -125,6 +129,9 @@ def lib():
     _sig(L, "pps_batch_solve", [vp, C.c_int32, C.POINTER(pps_problem),
                                 C.POINTER(pps_swarm_params), C.POINTER(C.c_int32), fp, fp,
                                 C.POINTER(pps_result), fp])
+    if L.pps_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{LIB_PATH} has ABI version {L.pps_abi_version()}, this binding "
+                           f"mirrors version {ABI_VERSION}; rebuild the library")
     _lib = L
     return L
 

@@ -172,12 +172,61 @@ def lotka_volterra_sse(obs, u0=(1.0, 1.0), dt=0.01, steps_per_obs=10) -> LotkaVo
 
 # ---------------------------------------------------------------------------
 # problem
+MAXCONS = 8
+
+
+class QuadraticConstraints:
+    """Inequality constraints g_k(x) <= 0 for ``OptimizationFunction(f; cons)``,
+    in the separable quadratic family
+
+        g_k(x) = sum_j linear[k, j] x_j + sum_j quadratic[k, j] x_j^2 + const[k]
+
+    (k < 8).  The reference's test problem (test/constraints.jl:11-14),
+    ``[-x1 + 2 x2 - 1, x1 - 2 x2 + 1, x1^2/4 + x2^2 - 1]``, is
+    ``QuadraticConstraints(linear=[[-1, 2], [1, -2], [0, 0]],
+    quadratic=[[0, 0], [0, 0], [0.25, 1]], const=[-1, 1, -1])``.  The swarm
+    minimises f + the reference's penalty (src/kernels.jl:28-36,
+    src/utils.jl:320-372); the solution's objective is f alone, as the
+    reference's (src/solve.jl:29-31)."""
+
+    def __init__(self, *, linear=None, quadratic=None, const):
+        c = np.asarray(const, np.float32).reshape(-1)
+        m = c.size
+        if m < 1 or m > MAXCONS:
+            raise ValueError(f"1..{MAXCONS} constraints, got {m}")
+        self.const = c
+        self.linear = None if linear is None else np.asarray(linear, np.float32).reshape(m, -1)
+        self.quadratic = (None if quadratic is None
+                          else np.asarray(quadratic, np.float32).reshape(m, -1))
+
+    def rows(self, d: int) -> np.ndarray:
+        m = self.const.size
+        out = np.zeros((m, 2 * d + 1), np.float32)
+        for name, block, col in (("linear", self.linear, 0), ("quadratic", self.quadratic, d)):
+            if block is not None:
+                if block.shape[1] != d:
+                    raise ValueError(f"{name} coefficients must have {d} columns")
+                out[:, col:col + d] = block
+        out[:, 2 * d] = self.const
+        return out
+
+    def __call__(self, x) -> np.ndarray:
+        """g(x) in float64 (for checks and reporting)."""
+        x = np.asarray(x, np.float64).reshape(-1)
+        r = self.rows(x.size).astype(np.float64)
+        d = x.size
+        return r[:, :d] @ x + r[:, d:2 * d] @ (x * x) + r[:, 2 * d]
+
+
 class OptimizationProblem:
     """``OptimizationProblem(f, u0, p; lb, ub)``.  Bounds are required: the
     unbounded and QMC samplers of the reference (src/init.jl:15-36) are not part
-    of this backend."""
+    of this backend.  ``cons`` takes QuadraticConstraints (the reference's
+    ``OptimizationFunction(f; cons)``); ``lcons``/``ucons`` are accepted and,
+    as in the reference's penalty, not used."""
 
-    def __init__(self, f, u0, p=None, *, lb=None, ub=None, shift=None, _validate=True):
+    def __init__(self, f, u0, p=None, *, lb=None, ub=None, shift=None, cons=None, lcons=None,
+                 ucons=None, _validate=True):
         if not isinstance(f, Objective):
             raise ValueError(
                 "objective must be one of the ppswarm objective markers (Rosenbrock(), "
@@ -203,6 +252,10 @@ class OptimizationProblem:
         if _validate and (self.lb.size != self.d or self.ub.size != self.d):
             raise ValueError("lb/ub must have the dimension of u0")
         self.shift = None if shift is None else np.asarray(shift, np.float32).reshape(-1)
+        if cons is not None and not isinstance(cons, QuadraticConstraints):
+            raise ValueError("cons must be a QuadraticConstraints marker")
+        self.cons = cons
+        self.lcons, self.ucons = lcons, ucons
 
     def c(self):
         """pps_problem view; the returned tuple keeps the arrays alive."""
@@ -226,6 +279,9 @@ class OptimizationProblem:
         if isinstance(self.f, LotkaVolterraSSE):
             pr.obs = ptr(self.f.obs)
             pr.n_obs = int(np.asarray(self.f.obs).size // 2)
+        if self.cons is not None:
+            pr.cons = ptr(self.cons.rows(self.d))
+            pr.n_cons = int(self.cons.const.size)
         return pr, keep
 
 

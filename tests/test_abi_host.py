@@ -26,7 +26,7 @@ def test_abi_version_and_defaults(pps):
     from ppswarm import _lib
 
     L = _lib.lib()
-    assert L.pps_abi_version() == 1
+    assert L.pps_abi_version() == _lib.ABI_VERSION == 2
     p = _lib.pps_swarm_params()
     L.pps_swarm_params_default(C.byref(p))
     # reference defaults: c1 = c2 = 1.4962f0 (src/kernels.jl:41-42), queue
