@@ -134,6 +134,9 @@ typedef struct pps_problem {
    * Sphere objectives; NULL / 0 for none. */
   const float* cons;
   int32_t n_cons;
+  /* u0 (d floats, NULL = zeros): the unbounded sampler's centre
+   * (src/utils.jl:23-55). */
+  const float* u0;
 } pps_problem;
 
 /* Swarm/algorithm parameters (ParallelSyncPSOKernel / ParallelPSOKernel
@@ -153,7 +156,29 @@ typedef struct pps_swarm_params {
                                 step; 0 = autotuned value if one is cached
                                 for (objective, d, N), else 2. Results do
                                 not depend on it.                            */
+  int32_t sampler;           /* PPS_SAMPLER_*: initial positions (init and
+                                reinit); bounds that are infinite in every
+                                coordinate select PPS_SAMPLER_UNBOUNDED, as
+                                the reference's init does (src/init.jl:70-74) */
 } pps_swarm_params;
+
+/* Initial-position samplers (the reference's `sampler` keyword of init,
+ * src/init.jl:15-54,56-115). */
+enum {
+  PPS_SAMPLER_UNIFORM = 0,        /* GPUUniformSampler: r*(ub-lb)+lb (default) */
+  PPS_SAMPLER_UNBOUNDED = 1,      /* GPUUnboundedSampler: |u0_j| > 0 ?
+                                     u0_j + r*|u0_j| : r  (src/utils.jl:23-55)  */
+  PPS_SAMPLER_LATIN_HYPERCUBE = 2 /* Latin hypercube design on the device: in
+                                     each coordinate, particle i takes stratum
+                                     pi_j(i) of n equal strata of [lb, ub] plus
+                                     a uniform offset inside it; pi_j is a keyed
+                                     permutation (4-round Feistel network with a
+                                     Philox round function, cycle-walked to
+                                     [0, n)), re-keyed by every reinit.  The
+                                     stratification of QuasiMonteCarlo's
+                                     LatinHypercubeSample; any other design can
+                                     be passed as positions, see below.        */
+};
 
 /* Fill p with the reference defaults (c1 = c2 = 1.4962, SYNC, PBEST,
  * per-component clamp, qlock_min_improvers = 2, auto launch). */
@@ -181,6 +206,16 @@ int pps_swarm_init(pps_ctx* ctx, const pps_problem* prob,
                    const pps_swarm_params* params, pps_swarm** out);
 /* Draw a fresh initial swarm (next epoch of the random stream). */
 int pps_swarm_reinit(pps_swarm* s);
+/* init / reinit! with host-sampled initial positions: the reference's path
+ * for QuasiMonteCarlo samplers (src/init.jl:15-36: QuasiMonteCarlo.sample on
+ * the host, then gpu_init_particles! over the samples, src/utils.jl:86-112).
+ * positions: n x d floats, particle-major (the memory of the d x n matrix
+ * QuasiMonteCarlo.sample returns), finite; used as given (not clamped).
+ * params->sampler is ignored by these two calls. */
+int pps_swarm_init_positions(pps_ctx* ctx, const pps_problem* prob,
+                             const pps_swarm_params* params, const float* positions,
+                             pps_swarm** out);
+int pps_swarm_reinit_positions(pps_swarm* s, const float* positions);
 /* Run `maxiters` iterations; w_k = w * wdamp^k as an iterated FP32 product
  * (src/lowerlevel_solve.jl:23-33).  Resumable: the random stream continues. */
 int pps_swarm_run(pps_swarm* s, int32_t maxiters, float w, float wdamp);

@@ -51,7 +51,10 @@ typedef struct ora_problem {
   /* constraints g_k(x) <= 0: n_cons rows [a_0..a_{d-1}, q_0..q_{d-1}, c] */
   const float* cons;
   int32_t n_cons;
+  float u0[ORA_MAXD]; /* the unbounded sampler's centre */
 } ora_problem;
+
+enum { ORA_SAMPLER_UNIFORM = 0, ORA_SAMPLER_UNBOUNDED = 1, ORA_SAMPLER_LATIN_HYPERCUBE = 2 };
 
 /* ---------------------------------------------------------------- Philox */
 /* Philox4x32-10, Salmon et al. SC'11: round multipliers 0xD2511F53,
@@ -356,12 +359,70 @@ static void step_particle(float* p, const ora_problem* P, const float* gb, float
   evaluate_particle(p, P, iter);
 }
 
-/* Uniform init (src/utils.jl:1-5,57-84): x_j = rand() * (ub_j - lb_j) + lb_j,
- * rand() for dimension j from philox(ctr = (i, 0, floor(j/4) | 1 << 16,
- * epoch)) lane j mod 4; v = 0; cost = f(x); best = x.  The init global best is
- * (best_position, best_cost) of the argmin (src/init.jl:49-51). */
+/* Keyed permutation of [0, n) of the Latin hypercube sampler (the design's
+ * definition, restated by the library's lhs_permute): balanced 4-round
+ * Feistel network on b bits (b even, 2^b >= n), round function
+ * philox(ctr = (right half, round, j | 2 << 16, epoch)) word 0 masked to b/2
+ * bits, (L, R) -> (R, L ^ F(R)), iterated until the value is below n. */
+uint32_t ora_lhs_permute(uint32_t i, uint32_t n, uint32_t j, uint32_t epoch, uint64_t seed) {
+  uint32_t key[2];
+  seed_key(seed, key);
+  uint32_t b = 2;
+  while (b < 32 && ((uint64_t)1 << b) < (uint64_t)n) b += 2;
+  uint32_t h = b / 2, mask = (uint32_t)(((uint64_t)1 << h) - 1);
+  uint32_t x = i;
+  do {
+    uint32_t L = x >> h, R = x & mask;
+    for (uint32_t r = 0; r < 4; ++r) {
+      uint32_t ctr[4] = {R, r, j | (2u << 16), epoch}, out[4];
+      ora_philox(ctr, key, out);
+      uint32_t t = L ^ (out[0] & mask);
+      L = R;
+      R = t;
+    }
+    x = (L << h) | R;
+  } while (x >= n);
+  return x;
+}
+
+/* Initial position of particle i under a sampler, random words from
+ * philox(ctr = (i, 0, floor(j/4) | tag << 16, epoch)) lane j mod 4:
+ *   uniform (src/utils.jl:1-5,57-84), tag 1: x_j = r * (ub_j - lb_j) + lb_j;
+ *   unbounded (src/utils.jl:23-55), tag 1: x_j = u0_j + r * |u0_j| if
+ *     |u0_j| > 0, else r (the reference's ifelse draws twice; one counter-based
+ *     draw per coordinate here);
+ *   Latin hypercube, tag 3: t = (pi_j(i) + r) / n, x_j = t * (ub_j - lb_j) + lb_j. */
+static void sample_position(const ora_problem* P, int sampler, int64_t n, int64_t i,
+                            uint32_t epoch, uint64_t seed, const uint32_t key[2], float* x) {
+  int d = P->d;
+  uint32_t tag = sampler == ORA_SAMPLER_LATIN_HYPERCUBE ? 3u : 1u;
+  uint32_t r[4];
+  for (int j = 0; j < d; ++j) {
+    if ((j & 3) == 0) {
+      uint32_t ctr[4] = {(uint32_t)i, 0u, (uint32_t)(j >> 2) | (tag << 16), epoch};
+      ora_philox(ctr, key, r);
+    }
+    float u = u01(r[j & 3]);
+    if (sampler == ORA_SAMPLER_UNBOUNDED) {
+      float c = P->u0[j], a = fabsf(c);
+      x[j] = a > 0.0f ? c + u * a : u;
+    } else if (sampler == ORA_SAMPLER_LATIN_HYPERCUBE) {
+      uint32_t pi = ora_lhs_permute((uint32_t)i, (uint32_t)n, (uint32_t)j, epoch, seed);
+      float t = ((float)pi + u) / (float)n;
+      x[j] = t * (P->ub[j] - P->lb[j]) + P->lb[j];
+    } else {
+      x[j] = u * (P->ub[j] - P->lb[j]) + P->lb[j];
+    }
+  }
+}
+
+/* init (src/init.jl:15-54, src/utils.jl:23-112): positions from the sampler,
+ * or `positions` (n x d, particle-major; the QuasiMonteCarlo path) when not
+ * NULL; v = 0; cost = f(x) [+ penalty at h(1)]; best = x.  The init global
+ * best is (best_position, best_cost) of the argmin (src/init.jl:49-51). */
 void ora_init(const ora_problem* P, int64_t n, uint64_t seed, uint32_t epoch, float* parts,
-              float* gb_pos, float* gb_cost, int64_t* gb_idx, int nthreads) {
+              float* gb_pos, float* gb_cost, int64_t* gb_idx, int nthreads, int32_t sampler,
+              const float* positions) {
   int d = P->d;
   int64_t s = stride_of(d);
   uint32_t key[2];
@@ -369,15 +430,11 @@ void ora_init(const ora_problem* P, int64_t n, uint64_t seed, uint32_t epoch, fl
 #pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t i = 0; i < n; ++i) {
     float* p = parts + i * s;
-    uint32_t r[4];
-    for (int j = 0; j < d; ++j) {
-      if ((j & 3) == 0) {
-        uint32_t ctr[4] = {(uint32_t)i, 0u, (uint32_t)(j >> 2) | (1u << 16), epoch};
-        ora_philox(ctr, key, r);
-      }
-      POS(p, d)[j] = u01(r[j & 3]) * (P->ub[j] - P->lb[j]) + P->lb[j];
-      VEL(p, d)[j] = 0.0f;
-    }
+    if (positions)
+      memcpy(POS(p, d), positions + i * d, sizeof(float) * d);
+    else
+      sample_position(P, sampler, n, i, epoch, seed, key, POS(p, d));
+    for (int j = 0; j < d; ++j) VEL(p, d)[j] = 0.0f;
     COST(p, d) = ora_cost(P, POS(p, d), 1.0f); /* calc_penalty(.., 1, ..), src/utils.jl:72-73 */
     memcpy(BPOS(p, d), POS(p, d), sizeof(float) * d);
     BCOST(p, d) = COST(p, d);

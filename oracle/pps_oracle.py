@@ -27,6 +27,7 @@ ROSENBROCK, RASTRIGIN, SPHERE, BEALE, BOOTH, HIMMELBLAU, QUADRATIC, BOUNDARY_QUA
     1, 2, 3, 4, 5, 6, 7, 8, 9)
 LOTKA_VOLTERRA = 16
 GBEST_PBEST, GBEST_CURRENT = 0, 1
+SAMPLER_UNIFORM, SAMPLER_UNBOUNDED, SAMPLER_LATIN_HYPERCUBE = 0, 1, 2
 CLAMP_COMPONENT, CLAMP_LEXICOGRAPHIC = 0, 1
 
 
@@ -36,6 +37,7 @@ class _Prob(C.Structure):
         ("lb", C.c_float * MAXD), ("ub", C.c_float * MAXD), ("shift", C.c_float * MAXD),
         ("par", C.c_float * MAXPAR), ("obs", C.POINTER(C.c_float)), ("n_obs", C.c_int32),
         ("steps_per_obs", C.c_int32), ("cons", C.POINTER(C.c_float)), ("n_cons", C.c_int32),
+        ("u0", C.c_float * MAXD),
     ]
 
 
@@ -81,7 +83,9 @@ def lib():
         L.ora_argmin.argtypes = [fp, C.c_int64, C.c_int]
         L.ora_argmin.restype = C.c_int64
         L.ora_init.argtypes = [C.POINTER(_Prob), C.c_int64, C.c_uint64, C.c_uint32, fp, fp, fp,
-                               i64p, C.c_int]
+                               i64p, C.c_int, C.c_int32, fp]
+        L.ora_lhs_permute.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]
+        L.ora_lhs_permute.restype = C.c_uint32
         L.ora_sync_run.argtypes = [C.POINTER(_Prob), C.c_int64, C.c_uint64, C.c_uint32,
                                    C.c_uint32, C.c_int32, C.c_float, C.c_float, C.c_float,
                                    C.c_float, C.c_int32, fp, fp, fp, i64p, C.c_int]
@@ -119,6 +123,7 @@ class Problem:
     clamp: int = CLAMP_COMPONENT
     obs: np.ndarray | None = None  # Lotka-Volterra observations, (n_obs, 2)
     cons: np.ndarray | None = None  # constraints, (n_cons, 2d + 1) rows [a, q, c]
+    u0: np.ndarray | None = None  # centre of the unbounded sampler
 
     def c(self) -> _Prob:
         p = _Prob()
@@ -140,6 +145,10 @@ class Problem:
             p.obs = _fp(self._obs_keep)
             p.n_obs = len(self._obs_keep) // 2
             p.steps_per_obs = int(par[3])
+        if self.u0 is not None:
+            u0 = np.asarray(self.u0, np.float32).reshape(-1)
+            for j in range(self.d):
+                p.u0[j] = float(u0[j])
         if self.cons is not None:
             self._cons_keep = np.ascontiguousarray(self.cons, np.float32).reshape(-1)
             p.cons = _fp(self._cons_keep)
@@ -160,6 +169,10 @@ def penalty(prob: Problem, x, iter_arg: float) -> np.float32:
 def cost(prob: Problem, x, iter_arg: float) -> np.float32:
     x = np.ascontiguousarray(x, np.float32)
     return np.float32(lib().ora_cost(C.byref(prob.c()), _fp(x), float(iter_arg)))
+
+
+def lhs_permute(i: int, n: int, j: int, epoch: int, seed: int) -> int:
+    return int(lib().ora_lhs_permute(int(i), int(n), int(j), int(epoch), int(seed)))
 
 
 def philox(ctr, key):
@@ -195,7 +208,8 @@ def evaluate(prob: Problem, xs) -> np.ndarray:
 class Swarm:
     """Oracle swarm in AoS SPSOParticle layout ((3d+2) float32 per particle)."""
 
-    def __init__(self, prob: Problem, n: int, seed: int, nthreads: int = 0):
+    def __init__(self, prob: Problem, n: int, seed: int, nthreads: int = 0,
+                 sampler: int = SAMPLER_UNIFORM, positions=None):
         self.prob, self.n, self.seed = prob, int(n), int(seed)
         self.d = prob.d
         self.nthreads = nthreads or lib().ora_num_threads()
@@ -206,7 +220,8 @@ class Swarm:
         self.epoch = 0
         self.iter = 0
         self._p = prob.c()
-        self.init()
+        self.sampler = int(sampler)
+        self.init(positions)
 
     # -- views in the kernels' SoA convention
     @property
@@ -244,15 +259,18 @@ class Swarm:
         self.gb_pos[: self.d] = np.asarray(pos, np.float32)
         self.gb_cost.value = float(cost)
 
-    def init(self):
+    def init(self, positions=None):
+        pos = None
+        if positions is not None:
+            pos = np.ascontiguousarray(positions, np.float32).reshape(self.n, self.d)
         lib().ora_init(C.byref(self._p), self.n, self.seed, self.epoch, _fp(self.parts),
                        _fp(self.gb_pos), C.byref(self.gb_cost), C.byref(self.gb_idx),
-                       self.nthreads)
+                       self.nthreads, self.sampler, None if pos is None else _fp(pos))
         self.iter = 0
 
-    def reinit(self):
+    def reinit(self, positions=None):
         self.epoch += 1
-        self.init()
+        self.init(positions)
 
     def run_sync(self, iters, w=0.7298, wdamp=1.0, c1=1.4962, c2=1.4962, gbest_from=GBEST_PBEST):
         lib().ora_sync_run(C.byref(self._p), self.n, self.seed, self.epoch, self.iter, int(iters),

@@ -394,4 +394,80 @@ __device__ __forceinline__ void init_position(float (&x)[MAXD], uint32_t i,
   }
 }
 
+// GPUUnboundedSampler (src/utils.jl:23-55): x_j = u0_j + r * |u0_j| when
+// |u0_j| > 0, else r, with r drawn as in init_position.
+template <int MAXD>
+__device__ __forceinline__ void unbounded_position(float (&x)[MAXD], uint32_t i, uint32_t epoch,
+                                                   uint32_t k0, uint32_t k1, const int d,
+                                                   const float* u0) {
+#pragma unroll
+  for (int m = 0; 4 * m < MAXD; ++m) {
+    if (4 * m < d) {
+      const u4 r = philox4x32_10(u4{i, 0u, (uint32_t)m | (PPS_TAG_INIT << 16), epoch}, k0, k1);
+      const uint32_t rr[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int j = 4 * m + h;
+        if (j < MAXD && j < d) {
+          const float c = u0[j], a = __builtin_fabsf(c), u = u01(rr[h]);
+          x[j] = a > 0.0f ? c + u * a : u;
+        }
+      }
+    }
+  }
+}
+
+// Keyed permutation of [0, n) for the Latin hypercube sampler: a balanced
+// 4-round Feistel network on b bits (b even, 2^b >= n, 2^b < 4n for n > 4)
+// whose round function is Philox(ctr = (right half, round, j | TAG << 16,
+// epoch)) lane x masked to b/2 bits, iterated until the value lies below n
+// ("cycle walking"; the network is a bijection on [0, 2^b), so the orbit of
+// i < n returns below n, on average in fewer than 4 passes).
+__device__ __forceinline__ uint32_t lhs_permute(uint32_t i, uint32_t n, uint32_t j,
+                                                uint32_t epoch, uint32_t k0, uint32_t k1) {
+  uint32_t b = 2;
+  while (b < 32 && (1ull << b) < (uint64_t)n) b += 2;
+  const uint32_t h = b / 2, mask = (uint32_t)((1ull << h) - 1ull);
+  uint32_t x = i;
+  do {
+    uint32_t L = x >> h, R = x & mask;
+#pragma unroll
+    for (uint32_t r = 0; r < 4; ++r) {
+      const uint32_t f =
+          philox4x32_10(u4{R, r, j | (PPS_TAG_LHS_PERM << 16), epoch}, k0, k1).x & mask;
+      const uint32_t t = L ^ f;
+      L = R;
+      R = t;
+    }
+    x = (L << h) | R;
+  } while (x >= n);
+  return x;
+}
+
+// Latin hypercube position: t = (pi_j(i) + u) / n with u from
+// philox(ctr = (i, 0, m | TAG_LHS_JITTER << 16, e)) lanes (x, y, z, w) for
+// coordinates 4m..4m+3, then x_j = t * (ub_j - lb_j) + lb_j (FP32).
+template <int MAXD>
+__device__ __forceinline__ void lhs_position(float (&x)[MAXD], uint32_t i, uint32_t n,
+                                             uint32_t epoch, uint32_t k0, uint32_t k1,
+                                             const int d, const ProbArgs& P) {
+#pragma unroll
+  for (int m = 0; 4 * m < MAXD; ++m) {
+    if (4 * m < d) {
+      const u4 r =
+          philox4x32_10(u4{i, 0u, (uint32_t)m | (PPS_TAG_LHS_JITTER << 16), epoch}, k0, k1);
+      const uint32_t rr[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int j = 4 * m + h;
+        if (j < MAXD && j < d) {
+          const uint32_t pi = lhs_permute(i, n, (uint32_t)j, epoch, k0, k1);
+          const float t = ((float)pi + u01(rr[h])) / (float)n;
+          x[j] = t * (P.ub[j] - P.lb[j]) + P.lb[j];
+        }
+      }
+    }
+  }
+}
+
 }  // namespace pps

@@ -8,8 +8,9 @@
 // One work-item steps one particle per tile; workgroup b owns tiles b, b+G, ...
 //
 // Kernels (reference counterparts in brackets):
-//   k_init      uniform initialisation + block argmin   [gpu_init_particles!,
-//               src/utils.jl:57-84; minimum, src/init.jl:49-51]
+//   k_init      initial positions (uniform / unbounded / Latin hypercube /
+//               given) + cost + block argmin   [gpu_init_particles!,
+//               src/utils.jl:23-112; minimum, src/init.jl:49-51]
 //   k_sync_step fused velocity/position/clamp/objective/pbest + block argmin
 //               [update_particle_states! sync, src/kernels.jl:127-194 and the
 //               per-iteration minimum of src/lowerlevel_solve.jl:23-33]
@@ -132,7 +133,17 @@ __global__ __launch_bounds__(BLOCK) void k_init(SwarmDev S, ProbArgs P, uint32_t
     const int64_t i = tile * BLOCK + threadIdx.x;
     if (i < S.n) {
       float x[MAXD], z[MAXD];
-      init_position<MAXD>(x, (uint32_t)i, S.epoch, S.key0, S.key1, d, P);
+      // sampler: uniform (src/utils.jl:57-84), unbounded (src/utils.jl:23-55),
+      // Latin hypercube, or positions already in the x rows (the QMC path,
+      // src/utils.jl:86-112)
+      if (S.sampler == PPS_SAMPLER_POSITIONS)
+        load_vec<MAXD>(x, S.x, S.npad, i, d);
+      else if (S.sampler == PPS_SAMPLER_UNBOUNDED)
+        unbounded_position<MAXD>(x, (uint32_t)i, S.epoch, S.key0, S.key1, d, S.u0);
+      else if (S.sampler == PPS_SAMPLER_LATIN_HYPERCUBE)
+        lhs_position<MAXD>(x, (uint32_t)i, (uint32_t)S.n, S.epoch, S.key0, S.key1, d, P);
+      else
+        init_position<MAXD>(x, (uint32_t)i, S.epoch, S.key0, S.key1, d, P);
 #pragma unroll
       for (int j = 0; j < MAXD; ++j) z[j] = 0.0f;
       const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, 1.0f);

@@ -47,17 +47,28 @@ int fill_prob_args(const pps_problem* prob, ProbArgs* P) {
   memset(P, 0, sizeof *P);
   const int d = prob->d;
   if (d < 1 || d > PPS_MAXD) return fail(PPS_E_INVALID, "dimension %d outside 1..%d", d, PPS_MAXD);
-  if (!prob->lb || !prob->ub) return fail(PPS_E_INVALID, "bounds are required (lb/ub NULL)");
+  if (!prob->lb != !prob->ub) return fail(PPS_E_INVALID, "lb and ub are both given or both NULL");
   P->obj = prob->objective;
   P->d = d;
+  // No bounds = infinite bounds (check_init_bounds, src/utils.jl:310-318).
+  // Bounds are either finite in every coordinate or -Inf/+Inf in every one
+  // (the unbounded case, src/init.jl:70-74); a mix would make the uniform
+  // initialiser produce non-finite positions and is refused.
+  int n_inf = 0;
   for (int j = 0; j < d; ++j) {
-    const float lo = prob->lb[j], hi = prob->ub[j];
-    if (!std::isfinite(lo) || !std::isfinite(hi))
-      return fail(PPS_E_INVALID, "bounds must be finite (dimension %d)", j);
+    const float lo = prob->lb ? prob->lb[j] : -INFINITY, hi = prob->ub ? prob->ub[j] : INFINITY;
+    if (std::isnan(lo) || std::isnan(hi)) return fail(PPS_E_INVALID, "bound %d is NaN", j);
     if (!(lo <= hi)) return fail(PPS_E_INVALID, "lb[%d] > ub[%d]", j, j);
+    if (lo == -INFINITY && hi == INFINITY)
+      ++n_inf;
+    else if (!std::isfinite(lo) || !std::isfinite(hi))
+      return fail(PPS_E_INVALID, "bounds must be finite, or -Inf/+Inf in every dimension "
+                                 "(dimension %d)", j);
     P->lb[j] = lo;
     P->ub[j] = hi;
   }
+  if (n_inf != 0 && n_inf != d)
+    return fail(PPS_E_INVALID, "bounds must be finite, or -Inf/+Inf in every dimension");
   if (prob->n_params < 0 || prob->n_params > PPS_MAXPAR)
     return fail(PPS_E_INVALID, "n_params %d outside 0..%d", prob->n_params, PPS_MAXPAR);
   if (prob->n_params > 0 && !prob->params) return fail(PPS_E_INVALID, "params is NULL");
@@ -103,6 +114,8 @@ int fill_prob_args(const pps_problem* prob, ProbArgs* P) {
   return PPS_OK;
 }
 
+bool prob_unbounded(const ProbArgs& P) { return P.d > 0 && std::isinf(P.lb[0]); }
+
 }  // namespace pps
 
 // ---------------------------------------------------------------------------
@@ -123,6 +136,7 @@ struct pps_swarm {
   int32_t g_T = -1;
   float g_w = 0.0f, g_wdamp = 0.0f;
   StepArgs g_args{};
+  int32_t sampler = PPS_SAMPLER_UNIFORM;  // sampler of init / reinit
   // Local-search outputs (allocated on first use): x (d x npad), f, argmin.
   float* lx = nullptr;
   float* lf = nullptr;
@@ -153,6 +167,7 @@ void pps_swarm_params_default(pps_swarm_params* p) {
   p->c2 = 1.4962f;
   p->qlock_min_improvers = 2;
   p->grid_per_cu = 0;
+  p->sampler = PPS_SAMPLER_UNIFORM;
 }
 
 int pps_ctx_create(int device, pps_ctx** out) {
@@ -307,6 +322,7 @@ static int swarm_alloc(pps_swarm* s, const pps_problem* prob) {
   const size_t o_t = take(sizeof(uint32_t) * 64);
   const size_t o_obs = take(sizeof(float) * 2 * (size_t)std::max(0, s->P.n_obs));
   const size_t o_cons = take(sizeof(float) * (size_t)s->P.n_cons * (2 * d + 1));
+  const size_t o_u0 = take(sizeof(float) * d);
   s->mem_bytes = off;
   HIPCHK(hipSetDevice(s->ctx->device));
   hipError_t e = hipMalloc(&s->mem, off);
@@ -335,6 +351,10 @@ static int swarm_alloc(pps_swarm* s, const pps_problem* prob) {
                           s->ctx->stream));
     s->P.obs = obs;
   }
+  s->S.u0 = (const float*)(base + o_u0);
+  if (prob->u0)
+    HIPCHK(hipMemcpyAsync(base + o_u0, prob->u0, sizeof(float) * d, hipMemcpyHostToDevice,
+                          s->ctx->stream));
   if (s->P.n_cons > 0) {
     float* cons = (float*)(base + o_cons);
     HIPCHK(hipMemcpyAsync(cons, prob->cons, sizeof(float) * s->P.n_cons * (2 * d + 1),
@@ -345,19 +365,41 @@ static int swarm_alloc(pps_swarm* s, const pps_problem* prob) {
 }
 
 // Uniform init of the current epoch + global best of the initial swarm.
-static int swarm_do_init(pps_swarm* s) {
-  s->S.epoch = s->S.epoch;  // epoch is set by the caller
+// Initialise the swarm of the current epoch: sample (or take `positions`,
+// n x d particle-major), evaluate, global best of the initial swarm.
+static int swarm_do_init(pps_swarm* s, const float* positions) {
   s->iter = 0;
-  HIPCHK(s->ks->init(s->S, s->P, 0u, s->ctx->stream));
-  HIPCHK(launch_finalize(s->S, s->P.d, 0u, s->ctx->stream));
+  SwarmDev S = s->S;
+  S.sampler = s->sampler;
+  if (positions) {
+    const int d = s->P.d;
+    const int64_t n = s->prm.n, npad = s->S.npad;
+    for (int64_t k = 0; k < n * d; ++k)
+      if (!std::isfinite(positions[k]))
+        return fail(PPS_E_INVALID, "initial position %lld is not finite", (long long)k);
+    std::vector<float> rows((size_t)d * npad, 0.0f);
+    for (int64_t i = 0; i < n; ++i)
+      for (int j = 0; j < d; ++j) rows[(size_t)j * npad + i] = positions[i * d + j];
+    HIPCHK(hipMemcpyAsync(S.x, rows.data(), sizeof(float) * rows.size(), hipMemcpyHostToDevice,
+                          s->ctx->stream));
+    S.sampler = PPS_SAMPLER_POSITIONS;
+    HIPCHK(s->ks->init(S, s->P, 0u, s->ctx->stream));
+    HIPCHK(launch_finalize(S, s->P.d, 0u, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));  // `rows` is read by the copy
+    return PPS_OK;
+  }
+  HIPCHK(s->ks->init(S, s->P, 0u, s->ctx->stream));
+  HIPCHK(launch_finalize(S, s->P.d, 0u, s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
   return PPS_OK;
 }
 
-int pps_swarm_init(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_params* prm,
-                   pps_swarm** out) {
+static int swarm_create(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_params* prm,
+                        const float* positions, pps_swarm** out) {
   if (!ctx || !prob || !prm || !out) return fail(PPS_E_INVALID, "NULL argument");
   *out = nullptr;
+  if (prm->sampler < PPS_SAMPLER_UNIFORM || prm->sampler > PPS_SAMPLER_LATIN_HYPERCUBE)
+    return fail(PPS_E_INVALID, "unknown sampler %d", prm->sampler);
   if (prm->n < 1 || prm->n > (int64_t)0x7FFFFFFF)
     return fail(PPS_E_INVALID, "num_particles %lld outside 1..2^31-1", (long long)prm->n);
   if (prm->mode < PPS_MODE_SYNC || prm->mode > PPS_MODE_QLOCK)
@@ -378,6 +420,7 @@ int pps_swarm_init(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_params
   int rc = fill_prob_args(prob, &s->P);
   if (rc == PPS_OK) {
     s->P.clamp = prm->clamp;
+    s->sampler = prob_unbounded(s->P) ? PPS_SAMPLER_UNBOUNDED : prm->sampler;
     s->ks = find_kernels(s->P.obj, s->P.d, s->P.n_cons > 0);
     if (!s->ks) rc = fail(PPS_E_INVALID, "no kernels for objective %d at d=%d", s->P.obj, s->P.d);
   }
@@ -390,7 +433,7 @@ int pps_swarm_init(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_params
   }
   if (rc == PPS_OK) {
     s->S.epoch = 0;
-    rc = swarm_do_init(s);
+    rc = swarm_do_init(s, positions);
   }
   if (rc != PPS_OK) {
     std::string keep = g_err;
@@ -402,11 +445,29 @@ int pps_swarm_init(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_params
   return PPS_OK;
 }
 
+int pps_swarm_init(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_params* prm,
+                   pps_swarm** out) {
+  return swarm_create(ctx, prob, prm, nullptr, out);
+}
+
+int pps_swarm_init_positions(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_params* prm,
+                             const float* positions, pps_swarm** out) {
+  if (!positions) return fail(PPS_E_INVALID, "positions is NULL");
+  return swarm_create(ctx, prob, prm, positions, out);
+}
+
 int pps_swarm_reinit(pps_swarm* s) {
   if (!s) return fail(PPS_E_INVALID, "swarm is NULL");
   HIPCHK(hipSetDevice(s->ctx->device));
   s->S.epoch += 1;
-  return swarm_do_init(s);
+  return swarm_do_init(s, nullptr);
+}
+
+int pps_swarm_reinit_positions(pps_swarm* s, const float* positions) {
+  if (!s || !positions) return fail(PPS_E_INVALID, "NULL argument");
+  HIPCHK(hipSetDevice(s->ctx->device));
+  s->S.epoch += 1;
+  return swarm_do_init(s, positions);
 }
 
 static int run_sync_graph(pps_swarm* s, int32_t T, float w, float wdamp, const StepArgs& A) {

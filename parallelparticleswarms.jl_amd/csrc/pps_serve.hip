@@ -8,11 +8,13 @@
 //      host memory (their result is read there and their HBM freed);
 //   2. admit queued requests in arrival order while a descriptor slot is free
 //      and the problem's SoA state fits in the HBM arena (first fit);
-//   3. launch one time slice over every resident problem: one workgroup per
-//      problem runs its iteration loop until the problem ends or the slice's
-//      device-clock budget is spent, so a slice lasts about the budget
-//      whatever mix of sizes is resident, and requests that arrived during a
-//      slice start in the next one while the others keep running.
+//   3. launch one time slice over every resident problem (work items, one
+//      workgroup each): a small problem's workgroup runs its iteration loop
+//      until the problem ends or the slice's device-clock budget is spent; a
+//      large problem runs one iteration per slice split over several
+//      workgroups, then a final item publishes its result.  Requests that
+//      arrive during a slice start in the next one while the others keep
+//      running; two slices are in flight at a time.
 // A problem's random stream is keyed by its own seed and particle indices, so
 // its result does not depend on what it was packed with.
 #include <hip/hip_runtime.h>
@@ -20,6 +22,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -452,6 +455,8 @@ int pps_server_submit(pps_server* sv, const pps_problem* prob, const pps_swarm_p
                 "d=%d)", prob->objective, prob->d);
   if (prob->n_cons != 0)
     return fail(PPS_E_INVALID, "served problems are unconstrained (n_cons = %d)", prob->n_cons);
+  if (prm->sampler != PPS_SAMPLER_UNIFORM)
+    return fail(PPS_E_INVALID, "served problems use the uniform initialiser");
   if (prm->mode != PPS_MODE_SYNC || prm->gbest_from != PPS_GBEST_PBEST ||
       prm->clamp != PPS_CLAMP_COMPONENT)
     return fail(PPS_E_INVALID, "served problems use sync PSO with best-position global best");
@@ -464,6 +469,10 @@ int pps_server_submit(pps_server* sv, const pps_problem* prob, const pps_swarm_p
     return rc;
   }
   r->P.clamp = PPS_CLAMP_COMPONENT;
+  if (std::isinf(r->P.lb[0])) {
+    delete r;
+    return fail(PPS_E_INVALID, "served problems need finite bounds");
+  }
   r->lb.assign(prob->lb, prob->lb + prob->d);
   r->ub.assign(prob->ub, prob->ub + prob->d);
   if (prob->n_params > 0) r->par.assign(prob->params, prob->params + prob->n_params);

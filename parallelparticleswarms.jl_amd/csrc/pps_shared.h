@@ -14,6 +14,10 @@
 // Third counter word = block index | tag << 16.
 #define PPS_TAG_STEP 0u
 #define PPS_TAG_INIT 1u
+#define PPS_TAG_LHS_PERM 2u    // Latin hypercube: Feistel round function
+#define PPS_TAG_LHS_JITTER 3u  // Latin hypercube: offset inside the stratum
+// Internal sampler value: the x rows already hold the initial positions.
+#define PPS_SAMPLER_POSITIONS 3
 
 // Taylor coefficients (-1)^k (2 pi)^(2k) / (2k)! rounded to FP32, k = 1..7.
 #define COS2PI_C1 (-0x1.3bd3ccp+4f)
@@ -85,6 +89,8 @@ struct SwarmDev {
   uint32_t epoch;
   int32_t G;
   int32_t block;
+  int32_t sampler;   // PPS_SAMPLER_* of the next initialisation
+  const float* u0;   // d floats (device): the unbounded sampler's centre
 };
 
 // Local-search (projected L-BFGS) settings passed to the kernel.

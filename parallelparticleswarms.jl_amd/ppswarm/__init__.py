@@ -6,7 +6,8 @@ swarm resident in HBM and every iteration in the HIP kernels of libpps.
 """
 from ._lib import PPSError, declared_functions, lib  # noqa: F401
 from .api import (  # noqa: F401
-    Beale, Booth, BoundaryQuadratic, Context, GBest, Himmelblau, HybridPSO, MI355XBackend,
+    Beale, Booth, BoundaryQuadratic, Context, GBest, GPUUnboundedSampler, GPUUniformSampler,
+    Himmelblau, HybridPSO, LatinHypercubeSample, MI355XBackend,
     Objective, OptimizationProblem, OptimizationSolution, ParallelPSOKernel, QuadraticConstraints,
     ParallelSyncPSOKernel, PSOCache, Quadratic, Rastrigin, Rosen2, Rosenbrock, SimpleLBFGS,
     Sphere, Swarm, autotune_sync, context, eval_f64, init, lotka_volterra_sse, reinit_, seed_, solve, solve_,

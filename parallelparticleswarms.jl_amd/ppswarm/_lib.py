@@ -24,6 +24,7 @@ OBJ_QUADRATIC, OBJ_BOUNDARY_QUADRATIC, OBJ_ROSEN2, OBJ_LOTKA_VOLTERRA = 7, 8, 9,
 MODE_SYNC, MODE_ASYNC, MODE_QLOCK = 0, 1, 2
 GBEST_PBEST, GBEST_CURRENT = 0, 1
 CLAMP_COMPONENT, CLAMP_LEXICOGRAPHIC = 0, 1
+SAMPLER_UNIFORM, SAMPLER_UNBOUNDED, SAMPLER_LATIN_HYPERCUBE = 0, 1, 2
 LAUNCH_AUTO, LAUNCH_EAGER, LAUNCH_GRAPH, LAUNCH_WORKGROUP = 0, 1, 2, 3
 
 fp = C.POINTER(C.c_float)
@@ -46,7 +47,7 @@ class pps_problem(C.Structure):
     _fields_ = [
         ("objective", C.c_int32), ("d", C.c_int32), ("lb", fp), ("ub", fp), ("params", fp),
         ("n_params", C.c_int32), ("shift", fp), ("obs", fp), ("n_obs", C.c_int32),
-        ("cons", fp), ("n_cons", C.c_int32),
+        ("cons", fp), ("n_cons", C.c_int32), ("u0", fp),
     ]
 
 
@@ -55,6 +56,7 @@ class pps_swarm_params(C.Structure):
         ("n", C.c_int64), ("mode", C.c_int32), ("gbest_from", C.c_int32), ("clamp", C.c_int32),
         ("launch", C.c_int32), ("seed", C.c_uint64), ("c1", C.c_float), ("c2", C.c_float),
         ("qlock_min_improvers", C.c_int32), ("grid_per_cu", C.c_int32),
+        ("sampler", C.c_int32),
     ]
 
 
@@ -104,6 +106,9 @@ def lib():
     _sig(L, "pps_swarm_init", [vp, C.POINTER(pps_problem), C.POINTER(pps_swarm_params),
                                C.POINTER(vp)])
     _sig(L, "pps_swarm_reinit", [vp])
+    _sig(L, "pps_swarm_init_positions", [vp, C.POINTER(pps_problem), C.POINTER(pps_swarm_params),
+                                         fp, C.POINTER(vp)])
+    _sig(L, "pps_swarm_reinit_positions", [vp, fp])
     _sig(L, "pps_swarm_run", [vp, C.c_int32, C.c_float, C.c_float])
     _sig(L, "pps_swarm_get_gbest", [vp, fp, fp, i64p])
     _sig(L, "pps_swarm_set_gbest", [vp, fp, C.c_float])

@@ -218,12 +218,35 @@ class QuadraticConstraints:
         return r[:, :d] @ x + r[:, d:2 * d] @ (x * x) + r[:, 2 * d]
 
 
+class GPUUniformSampler:
+    """The default initialiser: x = r * (ub - lb) + lb (src/utils.jl:57-84)."""
+
+    code = L.SAMPLER_UNIFORM
+
+
+class GPUUnboundedSampler:
+    """x_j = u0_j + r |u0_j| if u0_j != 0, else r (src/utils.jl:23-55); chosen
+    automatically when lb/ub are absent or infinite in every coordinate."""
+
+    code = L.SAMPLER_UNBOUNDED
+
+
+class LatinHypercubeSample:
+    """Latin hypercube design drawn on the device (one particle per stratum
+    of each coordinate, keyed Feistel permutation + jitter, re-drawn by
+    reinit!).  For a host-side design (QuasiMonteCarlo's samplers in the
+    reference, src/init.jl:15-36) pass any object with a
+    ``sample(n, lb, ub) -> (d, n) array`` method as the sampler instead."""
+
+    code = L.SAMPLER_LATIN_HYPERCUBE
+
+
 class OptimizationProblem:
-    """``OptimizationProblem(f, u0, p; lb, ub)``.  Bounds are required: the
-    unbounded and QMC samplers of the reference (src/init.jl:15-36) are not part
-    of this backend.  ``cons`` takes QuadraticConstraints (the reference's
-    ``OptimizationFunction(f; cons)``); ``lcons``/``ucons`` are accepted and,
-    as in the reference's penalty, not used."""
+    """``OptimizationProblem(f, u0, p; lb, ub)``.  Without lb/ub (or with
+    infinite bounds) the swarm starts from the unbounded sampler around u0, as
+    the reference's init does.  ``cons`` takes QuadraticConstraints (the
+    reference's ``OptimizationFunction(f; cons)``); ``lcons``/``ucons`` are
+    accepted and, as in the reference's penalty, not used."""
 
     def __init__(self, f, u0, p=None, *, lb=None, ub=None, shift=None, cons=None, lcons=None,
                  ucons=None, _validate=True):
@@ -244,12 +267,11 @@ class OptimizationProblem:
             self.p = np.asarray(p, np.float32).reshape(-1)
         if _validate and self.p.size < f.n_params:
             raise ValueError(f"{f.name} needs {f.n_params} parameters, got {self.p.size}")
-        if lb is None or ub is None:
-            if _validate:
-                raise ValueError("lb and ub are required (unbounded sampling is not supported)")
+        if (lb is None) != (ub is None):
+            raise ValueError("give both lb and ub, or neither (unbounded)")
         self.lb = None if lb is None else np.asarray(lb, np.float32).reshape(-1)
         self.ub = None if ub is None else np.asarray(ub, np.float32).reshape(-1)
-        if _validate and (self.lb.size != self.d or self.ub.size != self.d):
+        if _validate and lb is not None and (self.lb.size != self.d or self.ub.size != self.d):
             raise ValueError("lb/ub must have the dimension of u0")
         self.shift = None if shift is None else np.asarray(shift, np.float32).reshape(-1)
         if cons is not None and not isinstance(cons, QuadraticConstraints):
@@ -276,6 +298,7 @@ class OptimizationProblem:
         pr.params = ptr(self.p) if self.p.size else None
         pr.n_params = int(self.p.size)
         pr.shift = ptr(self.shift)
+        pr.u0 = ptr(self.u0)
         if isinstance(self.f, LotkaVolterraSSE):
             pr.obs = ptr(self.f.obs)
             pr.n_obs = int(np.asarray(self.f.obs).size // 2)
@@ -362,13 +385,26 @@ class OptimizationSolution:
 class Swarm:
     """Owning wrapper of a pps_swarm handle."""
 
-    def __init__(self, ctx: Context, prob: OptimizationProblem, params: L.pps_swarm_params):
+    def __init__(self, ctx: Context, prob: OptimizationProblem, params: L.pps_swarm_params,
+                 positions=None):
+        """positions: optional (n, d) initial positions (the QMC path)."""
         self.ctx, self.prob, self.params = ctx, prob, params
+        self.d, self.n = prob.d, int(params.n)
         pr, keep = prob.c()
         h = C.c_void_p()
-        L.check(L.lib().pps_swarm_init(ctx.h, C.byref(pr), C.byref(params), C.byref(h)))
+        if positions is None:
+            L.check(L.lib().pps_swarm_init(ctx.h, C.byref(pr), C.byref(params), C.byref(h)))
+        else:
+            pos = self._positions(positions)
+            L.check(L.lib().pps_swarm_init_positions(ctx.h, C.byref(pr), C.byref(params),
+                                                     pos.ctypes.data_as(L.fp), C.byref(h)))
         self.h = h
-        self.d, self.n = prob.d, int(params.n)
+
+    def _positions(self, positions) -> np.ndarray:
+        pos = np.ascontiguousarray(positions, np.float32)
+        if pos.shape != (self.n, self.d):
+            raise ValueError(f"positions must be ({self.n}, {self.d}), got {pos.shape}")
+        return pos
 
     def __del__(self):
         self.free()
@@ -382,8 +418,12 @@ class Swarm:
         L.check(L.lib().pps_swarm_run(self.h, int(maxiters), float(np.float32(w)),
                                       float(np.float32(wdamp))))
 
-    def reinit(self):
-        L.check(L.lib().pps_swarm_reinit(self.h))
+    def reinit(self, positions=None):
+        if positions is None:
+            L.check(L.lib().pps_swarm_reinit(self.h))
+        else:
+            pos = self._positions(positions)
+            L.check(L.lib().pps_swarm_reinit_positions(self.h, pos.ctypes.data_as(L.fp)))
 
     def gbest(self) -> GBest:
         pos = np.zeros(self.d, np.float32)
@@ -445,11 +485,12 @@ def swarm_params(n: int, mode: int = L.MODE_SYNC, *, seed: int = 0,
                  gbest_from: int = L.GBEST_PBEST, clamp: int = L.CLAMP_COMPONENT,
                  launch: int = L.LAUNCH_AUTO, qlock_min_improvers: int = 2,
                  c1: float = 1.4962, c2: float = 1.4962,
-                 grid_per_cu: int = 0) -> L.pps_swarm_params:
+                 grid_per_cu: int = 0, sampler: int = L.SAMPLER_UNIFORM) -> L.pps_swarm_params:
     p = L.pps_swarm_params()
     L.lib().pps_swarm_params_default(C.byref(p))
     p.n, p.mode, p.seed = int(n), int(mode), int(seed) & 0xFFFFFFFFFFFFFFFF
     p.grid_per_cu = int(grid_per_cu)
+    p.sampler = int(sampler)
     p.gbest_from, p.clamp, p.launch = int(gbest_from), int(clamp), int(launch)
     p.qlock_min_improvers = int(qlock_min_improvers)
     p.c1, p.c2 = float(c1), float(c2)
@@ -477,6 +518,7 @@ class PSOCache:
     swarm: Swarm
     gbest: GBest
     seed: int
+    sampler: Any = None
 
 
 def _mode_of(alg) -> int:
@@ -487,18 +529,37 @@ def _mode_of(alg) -> int:
     raise ValueError(f"unsupported algorithm {type(alg).__name__}")
 
 
-def init(prob: OptimizationProblem, alg, *, seed: Optional[int] = None, **kw) -> PSOCache:
+def _host_sample(sampler, prob: OptimizationProblem, n: int) -> np.ndarray:
+    """QuasiMonteCarlo.sample(n, lb, ub, sampler) -> d x n; returned as n x d."""
+    x = np.asarray(sampler.sample(n, prob.lb, prob.ub), np.float32)
+    if x.shape != (prob.d, n):
+        raise ValueError(f"sampler.sample must return a ({prob.d}, {n}) array, got {x.shape}")
+    return np.ascontiguousarray(x.T)
+
+
+def init(prob: OptimizationProblem, alg, *, seed: Optional[int] = None, sampler=None,
+         **kw) -> PSOCache:
+    """``init(prob, alg; sampler)`` (src/init.jl:56-115): sampler is
+    GPUUniformSampler() (default), GPUUnboundedSampler(), LatinHypercubeSample()
+    or a host sampler with ``sample(n, lb, ub)``."""
     if isinstance(alg, HybridPSO):
-        return init(prob, alg.pso, seed=seed, **kw)
+        return init(prob, alg.pso, seed=seed, sampler=sampler, **kw)
     seed = _next_seed() if seed is None else int(seed)
     ctx = context(alg.backend)
-    params = swarm_params(alg.num_particles, _mode_of(alg), seed=seed, **kw)
-    sw = Swarm(ctx, prob, params)
-    return PSOCache(prob, alg, sw, sw.gbest(), seed)
+    code = getattr(sampler, "code", L.SAMPLER_UNIFORM)
+    params = swarm_params(alg.num_particles, _mode_of(alg), seed=seed, sampler=code, **kw)
+    host = sampler is not None and not hasattr(sampler, "code") and prob.lb is not None
+    pos = _host_sample(sampler, prob, alg.num_particles) if host else None
+    sw = Swarm(ctx, prob, params, positions=pos)
+    return PSOCache(prob, alg, sw, sw.gbest(), seed, sampler if host else None)
 
 
 def reinit_(cache: PSOCache) -> None:
-    cache.swarm.reinit()
+    """``reinit!``: a fresh initial swarm from the cache's sampler."""
+    if cache.sampler is not None:
+        cache.swarm.reinit(_host_sample(cache.sampler, cache.prob, cache.swarm.n))
+    else:
+        cache.swarm.reinit()
     cache.gbest = cache.swarm.gbest()
 
 

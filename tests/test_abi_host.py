@@ -72,7 +72,9 @@ def test_problem_validation(pps):
     with pytest.raises(ValueError):
         pps.OptimizationProblem(lambda x, p: sum(x), np.zeros(2), lb=[0, 0], ub=[1, 1])
     with pytest.raises(ValueError):
-        pps.OptimizationProblem(pps.Rosenbrock(), np.zeros(2), [1.0, 100.0])  # no bounds
+        pps.OptimizationProblem(pps.Rosenbrock(), np.zeros(2), [1.0, 100.0], lb=[0, 0])
+    # no bounds at all: the unbounded sampler's problem (src/init.jl:70-74)
+    assert pps.OptimizationProblem(pps.Rosenbrock(), np.zeros(2), [1.0, 100.0]).lb is None
     with pytest.raises(ValueError):
         pps.OptimizationProblem(pps.Beale(), np.zeros(3), lb=np.zeros(3), ub=np.ones(3))
     with pytest.raises(ValueError):
