@@ -17,7 +17,8 @@
 # cannot run inside a precompiled HIP kernel); anything else throws.
 module PPSMI355X
 
-using SciMLBase, StaticArrays
+using SciMLBase, StaticArrays, QuasiMonteCarlo
+import ParallelParticleSwarms
 import ParallelParticleSwarms: ParallelPSOKernel, ParallelSyncPSOKernel, HybridPSO,
     SimpleLBFGS
 import SciMLBase: init, solve!, reinit!, solve, OptimizationProblem, OptimizationStats
@@ -44,6 +45,7 @@ struct CProblem            # pps_problem
     n_obs::Int32
     cons::Ptr{Float32}
     n_cons::Int32
+    u0::Ptr{Float32}
 end
 
 struct CSwarmParams        # pps_swarm_params
@@ -57,6 +59,7 @@ struct CSwarmParams        # pps_swarm_params
     c2::Float32
     qlock_min_improvers::Int32
     grid_per_cu::Int32
+    sampler::Int32
 end
 
 struct CLbfgsParams        # pps_lbfgs_params
@@ -142,27 +145,30 @@ struct ProblemArrays
     p::Vector{Float32}
     shift::Union{Nothing, Vector{Float32}}
     cons::Vector{Float32}
+    u0::Vector{Float32}
 end
 
 function cproblem(prob::OptimizationProblem)
     f = prob.f.f
     d = length(prob.u0)
-    (prob.lb === nothing || prob.ub === nothing) &&
-        throw(ArgumentError("the MI355X backend samples the initial swarm in [lb, ub]"))
+    (prob.lb === nothing) == (prob.ub === nothing) ||
+        throw(ArgumentError("give both lb and ub, or neither"))
     g = prob.f.cons
     g === nothing || g isa PPSQuadraticConstraints || throw(ArgumentError(
         "the MI355X backend evaluates constraints given as PPSQuadraticConstraints"))
     # lcons/ucons are not read: the reference's penalty uses g(x) <= 0 (src/utils.jl:359-361)
-    a = ProblemArrays(Float32.(collect(prob.lb)), Float32.(collect(prob.ub)),
+    # no bounds: the library fills -Inf/+Inf and starts from the unbounded
+    # sampler around u0 (check_init_bounds + src/init.jl:70-74)
+    a = ProblemArrays(prob.lb === nothing ? Float32[] : Float32.(collect(prob.lb)),
+        prob.ub === nothing ? Float32[] : Float32.(collect(prob.ub)),
         prob.p === nothing || prob.p isa SciMLBase.NullParameters ? Float32[] :
         Float32.(collect(prob.p)), f isa PPSObjective ? f.shift : nothing,
-        g === nothing ? Float32[] : cons_rows(g, d))
-    c = CProblem(objective_code(f), Int32(d), pointer(a.lb), pointer(a.ub),
-        isempty(a.p) ? Ptr{Float32}(C_NULL) : pointer(a.p), Int32(length(a.p)),
-        a.shift === nothing ? Ptr{Float32}(C_NULL) : pointer(a.shift),
-        Ptr{Float32}(C_NULL), Int32(0),
-        isempty(a.cons) ? Ptr{Float32}(C_NULL) : pointer(a.cons),
-        Int32(g === nothing ? 0 : length(g.c)))
+        g === nothing ? Float32[] : cons_rows(g, d), Float32.(collect(prob.u0)))
+    ptr(v) = isempty(v) ? Ptr{Float32}(C_NULL) : pointer(v)
+    c = CProblem(objective_code(f), Int32(d), ptr(a.lb), ptr(a.ub), ptr(a.p),
+        Int32(length(a.p)), a.shift === nothing ? Ptr{Float32}(C_NULL) : pointer(a.shift),
+        Ptr{Float32}(C_NULL), Int32(0), ptr(a.cons), Int32(g === nothing ? 0 : length(g.c)),
+        pointer(a.u0))
     return c, a
 end
 
@@ -171,11 +177,11 @@ function eval_f64(f::PPSObjective, x, p)
     prob = (; f = (; f = f, cons = nothing), u0 = x, p = p,
         lb = fill(-Inf32, d), ub = fill(Inf32, d))
     a = ProblemArrays(Float32[], Float32[], p === nothing ? Float32[] : Float32.(collect(p)),
-        f.shift, Float32[])
+        f.shift, Float32[], Float32[])
     c = CProblem(objective_code(f), Int32(d), C_NULL, C_NULL,
         isempty(a.p) ? Ptr{Float32}(C_NULL) : pointer(a.p), Int32(length(a.p)),
         a.shift === nothing ? Ptr{Float32}(C_NULL) : pointer(a.shift), C_NULL, Int32(0),
-        C_NULL, Int32(0))
+        C_NULL, Int32(0), C_NULL)
     xs = Float64.(collect(x))
     out = Ref{Float64}(0.0)
     GC.@preserve a xs begin
@@ -186,13 +192,39 @@ function eval_f64(f::PPSObjective, x, p)
 end
 
 # ---- cache (the role of the reference's PSOCache, src/init.jl:1-7) ---------
-mutable struct MI355XCache{P, A}
+mutable struct MI355XCache{P, A, S}
     prob::P
     alg::A
     handle::Ptr{Cvoid}
     gbest_pos::Vector{Float32}   # the reference's cache.gbest
     gbest_cost::Float32
+    sampler::S                   # the reference's cache.sampler
 end
+
+"""
+    PPSLatinHypercube()
+
+Latin hypercube initial swarm drawn on the device (one particle per stratum of
+every coordinate; a fresh design on every reinit!).  QuasiMonteCarlo samplers
+(`LatinHypercubeSample()`, `SobolSample()`, ...) are sampled on the host exactly
+as the reference does (src/init.jl:15-36) and handed to the library.
+"""
+struct PPSLatinHypercube end
+
+const PPS_SAMPLER_UNIFORM, PPS_SAMPLER_UNBOUNDED, PPS_SAMPLER_LATIN_HYPERCUBE =
+    Int32(0), Int32(1), Int32(2)
+sampler_code(::ParallelParticleSwarms.GPUUniformSampler) = PPS_SAMPLER_UNIFORM
+sampler_code(::ParallelParticleSwarms.GPUUnboundedSampler) = PPS_SAMPLER_UNBOUNDED
+sampler_code(::PPSLatinHypercube) = PPS_SAMPLER_LATIN_HYPERCUBE
+sampler_code(::QuasiMonteCarlo.SamplingAlgorithm) = PPS_SAMPLER_UNIFORM  # host-sampled
+
+host_sampled(prob, sampler) = sampler isa QuasiMonteCarlo.SamplingAlgorithm &&
+    prob.lb !== nothing && !(all(isinf, prob.lb) && all(isinf, prob.ub))
+
+# QuasiMonteCarlo.sample(n, lb, ub, sampler) is d x n; its memory is the
+# particle-major n x d layout pps_swarm_init_positions takes.
+host_positions(prob, n, sampler) =
+    Float32.(QuasiMonteCarlo.sample(n, collect(prob.lb), collect(prob.ub), sampler))
 
 function free!(c::MI355XCache)
     if c.handle != C_NULL
@@ -215,18 +247,26 @@ end
 
 function init(prob::OptimizationProblem,
         alg::Union{ParallelSyncPSOKernel{<:MI355XBackend}, ParallelPSOKernel{<:MI355XBackend}},
-        args...; seed::Integer = rand(UInt64), kwargs...)
+        args...; seed::Integer = rand(UInt64),
+        sampler = ParallelParticleSwarms.GPUUniformSampler(), kwargs...)
     c, arrays = cproblem(prob)
     prm = CSwarmParams(alg.num_particles, mode_of(alg), PPS_GBEST_PBEST, Int32(0), Int32(0),
-        UInt64(seed), 1.4962f0, 1.4962f0, Int32(2), Int32(0))
+        UInt64(seed), 1.4962f0, 1.4962f0, Int32(2), Int32(0), sampler_code(sampler))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     GC.@preserve arrays begin
-        check(ccall((:pps_swarm_init, libpps), Cint,
-            (Ptr{Cvoid}, Ref{CProblem}, Ref{CSwarmParams}, Ptr{Ptr{Cvoid}}),
-            context(alg.backend), c, prm, h))
+        if host_sampled(prob, sampler)
+            x0 = host_positions(prob, alg.num_particles, sampler)
+            check(ccall((:pps_swarm_init_positions, libpps), Cint,
+                (Ptr{Cvoid}, Ref{CProblem}, Ref{CSwarmParams}, Ptr{Float32}, Ptr{Ptr{Cvoid}}),
+                context(alg.backend), c, prm, x0, h))
+        else
+            check(ccall((:pps_swarm_init, libpps), Cint,
+                (Ptr{Cvoid}, Ref{CProblem}, Ref{CSwarmParams}, Ptr{Ptr{Cvoid}}),
+                context(alg.backend), c, prm, h))
+        end
     end
     pos, cost, _ = read_gbest(h[], length(prob.u0))
-    cache = MI355XCache(prob, alg, h[], pos, cost)
+    cache = MI355XCache(prob, alg, h[], pos, cost, sampler)
     finalizer(free!, cache)
     return cache
 end
@@ -260,7 +300,13 @@ function solve!(cache::MI355XCache, args...; maxiters = 100, w = 0.7298f0, wdamp
 end
 
 function reinit!(cache::MI355XCache)
-    check(ccall((:pps_swarm_reinit, libpps), Cint, (Ptr{Cvoid},), cache.handle))
+    if host_sampled(cache.prob, cache.sampler)
+        x0 = host_positions(cache.prob, cache.alg.num_particles, cache.sampler)
+        check(ccall((:pps_swarm_reinit_positions, libpps), Cint, (Ptr{Cvoid}, Ptr{Float32}),
+            cache.handle, x0))
+    else
+        check(ccall((:pps_swarm_reinit, libpps), Cint, (Ptr{Cvoid},), cache.handle))
+    end
     cache.gbest_pos, cache.gbest_cost, _ = read_gbest(cache.handle, length(cache.prob.u0))
     return nothing
 end
@@ -298,6 +344,6 @@ function solve(prob::OptimizationProblem, alg::HybridPSO{<:MI355XBackend, <:Simp
         stats = OptimizationStats(; time = (time() - t0) + sol_pso.stats.time))
 end
 
-export MI355XBackend, PPSObjective, PPSQuadraticConstraints
+export MI355XBackend, PPSObjective, PPSQuadraticConstraints, PPSLatinHypercube
 
 end # module
