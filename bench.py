@@ -27,7 +27,8 @@ Also reported on the same line:
                 iterations, then projected L-BFGS (m = 10, 100 iterations,
                 Armijo) from every particle; final minimum re-evaluated in FP64.
   ode           Lotka-Volterra parameter estimation: 2^14 particles, 300
-                iterations, in-kernel FP32 RK4 (dt = 0.01, 101 observations).
+                iterations, in-kernel FP32 RK4 (dt = 0.01, 101 observations);
+                the same with Tsit5 at fixed and adaptive steps.
   cpu_baseline  the CPU oracle (reference-faithful AoS restatement, OpenMP
                 over particles + serial argmin per iteration) on a bounded
                 sample of the same workload on this host.
@@ -188,10 +189,20 @@ def ode_c3(P):
     ms = sw.elapsed_ms()
     sw.free()
     rk4_steps = 1000  # per objective evaluation (100 intervals x 10 steps)
-    return {"wall_s": wall, "device_ms": ms,
-            "particle_iters_per_s": (1 << 14) * 300 / (ms / 1e3),
-            "rk4_steps_per_s": (1 << 14) * 300 * rk4_steps / (ms / 1e3),
-            "theta": [float(v) for v in sol.u], "loss_f64": sol.objective}
+    out = {"wall_s": wall, "device_ms": ms,
+           "particle_iters_per_s": (1 << 14) * 300 / (ms / 1e3),
+           "rk4_steps_per_s": (1 << 14) * 300 * rk4_steps / (ms / 1e3),
+           "theta": [float(v) for v in sol.u], "loss_f64": sol.objective}
+    # the reference's default solver, GPUTsit5: fixed steps (dt = 0.01) and
+    # adaptive steps (abstol 1e-6, reltol 1e-3, initial dt 0.01), same shape
+    for key, adaptive in (("tsit5", False), ("tsit5_adaptive", True)):
+        sol, sw = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 1 << 14, 300, seed=3,
+                                        ode_alg="tsit5", adaptive=adaptive)
+        ms = sw.elapsed_ms()
+        sw.free()
+        out[key] = {"device_ms": ms, "particle_iters_per_s": (1 << 14) * 300 / (ms / 1e3),
+                    "theta": [float(v) for v in sol.u], "loss_f64": sol.objective}
+    return out
 
 
 def n_sweep(P, L):

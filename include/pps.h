@@ -65,8 +65,11 @@ enum {
   PPS_OBJ_BOUNDARY_QUADRATIC = 8, /* (x1+1)^2                            */
   PPS_OBJ_ROSEN2 = 9,             /* (p1-x1)^2 + p2 (x2-x1^2)^2          */
   /* Lotka-Volterra parameter estimation: sum of squared errors of an FP32
-   * RK4 trajectory against observations (reference src/ode_pso.jl:47-52). */
-  PPS_OBJ_LOTKA_VOLTERRA = 16
+   * trajectory against observations (reference src/ode_pso.jl:47-52), solved
+   * in the kernel by one of three methods. */
+  PPS_OBJ_LOTKA_VOLTERRA = 16,       /* fixed-step RK4                         */
+  PPS_OBJ_LOTKA_VOLTERRA_TSIT5 = 17, /* fixed-step Tsit5 (GPUTsit5, Val(false)) */
+  PPS_OBJ_LOTKA_VOLTERRA_ATSIT5 = 18 /* adaptive Tsit5 (GPUTsit5, Val(true))    */
 };
 
 /* Swarm update semantics. */
@@ -109,15 +112,19 @@ enum {
 typedef struct pps_problem {
   int32_t objective;     /* PPS_OBJ_*                                        */
   int32_t d;             /* dimension, 1..64 (LOTKA_VOLTERRA: 4)             */
-  const float* lb;       /* d floats, finite                                 */
-  const float* ub;       /* d floats, finite, lb <= ub                       */
+  const float* lb;       /* d floats, lb <= ub; finite, or -Inf/+Inf in every */
+  const float* ub;       /* coordinate; both NULL = unbounded (the unbounded
+                            sampler around u0, src/init.jl:70-74)            */
   const float* params;   /* n_params floats (Rosenbrock: [p1, p2])           */
   int32_t n_params;
   const float* shift;    /* d floats or NULL: f is evaluated at x - shift    */
-  /* PPS_OBJ_LOTKA_VOLTERRA only: observations u(t_k), k = 0..n_obs-1,
+  /* Lotka-Volterra objectives only: observations u(t_k), k = 0..n_obs-1,
    * interleaved [u1(t0), u2(t0), u1(t1), ...]; the trajectory starts at
-   * params[0..1] = u0, steps dt = params[2], and is sampled every
-   * (int)params[3] steps. */
+   * params[0..1] = u0.  RK4 and fixed-step Tsit5: steps dt = params[2],
+   * sampled every (int)params[3] steps.  Adaptive Tsit5: t_k = k * params[2],
+   * initial step params[3], abstol params[4], reltol params[5], optional step
+   * limit params[6] (default 10000; beyond it the loss is +Inf; controller
+   * documented at lv_sse_tsit5 in the device sources). */
   const float* obs;
   int32_t n_obs;
   /* Inequality constraints g_k(x) <= 0, k = 0..n_cons-1 (n_cons <= PPS_MAXCONS),

@@ -38,7 +38,8 @@
 enum {
   ORA_ROSENBROCK = 1, ORA_RASTRIGIN = 2, ORA_SPHERE = 3, ORA_BEALE = 4,
   ORA_BOOTH = 5, ORA_HIMMELBLAU = 6, ORA_QUADRATIC = 7,
-  ORA_BOUNDARY_QUADRATIC = 8, ORA_ROSEN2 = 9, ORA_LOTKA_VOLTERRA = 16
+  ORA_BOUNDARY_QUADRATIC = 8, ORA_ROSEN2 = 9, ORA_LOTKA_VOLTERRA = 16,
+  ORA_LOTKA_VOLTERRA_TSIT5 = 17, ORA_LOTKA_VOLTERRA_ATSIT5 = 18
 };
 enum { ORA_GBEST_PBEST = 0, ORA_GBEST_CURRENT = 1 };
 enum { ORA_CLAMP_COMPONENT = 0, ORA_CLAMP_LEXICOGRAPHIC = 1 };
@@ -144,6 +145,136 @@ static void lv_rhs(float u1, float u2, const float* p, float* f1, float* f2) {
   *f1 = p[0] * u1 - (p[1] * u1) * u2;
   *f2 = (-p[2]) * u2 + (p[3] * u1) * u2;
 }
+/* Tsitouras 5(4) (Comput. Math. Appl. 62 (2011) 770-775), the method of the
+ * reference's default ode_alg = GPUTsit5() (src/ode_pso.jl:54-170): stage
+ * weights a (row 6 = solution weights b, FSAL) and error weights
+ * btilde = b - bhat, as FP32. */
+static const float TS_A[7][6] = {
+    {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f},
+    {0.161f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f},
+    {-0.008480655492356989f, 0.335480655492357f, 0.0f, 0.0f, 0.0f, 0.0f},
+    {2.897153057105493f, -6.359448489975075f, 4.3622954328695815f, 0.0f, 0.0f, 0.0f},
+    {5.325864828439257f, -11.748883564062828f, 7.4955393428898365f, -0.09249506636175525f, 0.0f,
+     0.0f},
+    {5.86145544294642f, -12.92096931784711f, 8.159367898576159f, -0.071584973281401f,
+     -0.028269050394068383f, 0.0f},
+    {0.09646076681806523f, 0.01f, 0.4798896504144996f, 1.379008574103742f, -3.290069515436081f,
+     2.324710524099774f}};
+static const float TS_E[7] = {-0.00178001105222577714f, -0.0008164344596567469f,
+                              0.007880878010261995f,    -0.1447110071732629f,
+                              0.5823571654525552f,      -0.45808210592918697f,
+                              0.015151515151515152f};
+
+/* The tableau as used (for the order-condition checks). */
+void ora_tsit5_tableau(float* a /* 7 x 6 */, float* e /* 7 */) {
+  memcpy(a, TS_A, sizeof TS_A);
+  memcpy(e, TS_E, sizeof TS_E);
+}
+
+/* One Tsit5 step from u with k[0] = f(u): y_i = u + h * (a_i0 k_0 + ... ),
+ * sums left to right; k[i] = f(y_i); the new state is y_6; k[6] = f(y_6)
+ * (FSAL); err = h * (e_0 k_0 + ... + e_6 k_6). */
+static void lv_tsit5_step(const float u[2], float k[7][2], float h, const float* th,
+                          float un[2], float err[2]) {
+  for (int i = 1; i < 7; ++i) {
+    float y[2];
+    for (int c = 0; c < 2; ++c) {
+      float acc = TS_A[i][0] * k[0][c];
+      for (int j = 1; j < i; ++j) acc = acc + TS_A[i][j] * k[j][c];
+      y[c] = u[c] + h * acc;
+    }
+    lv_rhs(y[0], y[1], th, &k[i][0], &k[i][1]);
+    if (i == 6) un[0] = y[0], un[1] = y[1];
+  }
+  for (int c = 0; c < 2; ++c) {
+    float acc = TS_E[0] * k[0][c];
+    for (int j = 1; j < 7; ++j) acc = acc + TS_E[j] * k[j][c];
+    err[c] = h * acc;
+  }
+}
+
+/* x^(1/5) from IEEE basic operations: exponent-halving seed (the float's
+ * bits, as an int32, minus those of 1.0f, divided by 5, plus those of 1.0f)
+ * and four Newton steps y <- (4y + x/y^4)/5; 0 for x <= 0 or NaN. */
+float ora_root5(float x) {
+  if (!(x > 0.0f)) return 0.0f;
+  if (isinf(x)) return x;
+  int32_t bits;
+  memcpy(&bits, &x, 4);
+  bits = (bits - 0x3f800000) / 5 + 0x3f800000;
+  float y;
+  memcpy(&y, &bits, 4);
+  for (int it = 0; it < 4; ++it) {
+    float y2 = y * y;
+    y = (4.0f * y + x / (y2 * y2)) / 5.0f;
+  }
+  return y;
+}
+
+/* Lotka-Volterra SSE with Tsit5, fixed (dt = par[2], par[3] steps per
+ * observation; parameter_estim_ode! with Val(false)) or adaptive (observation
+ * interval par[2], initial step par[3], abstol par[4], reltol par[5];
+ * Val(true)).  DiffEqGPU's controller is not vendored; this one: scaled error
+ * s_i = err_i / (abstol + max(|u_i|, |unew_i|) reltol), EEst = sqrt((s_1^2 +
+ * s_2^2) * 0.5), accept when EEst <= 1, q = root5(EEst) / 0.9 in [0.1, 5]
+ * (5 when EEst is not finite), next step h / q; steps end exactly on the
+ * observation times (t_k = k * par[2] in FP32) and a step shortened to meet
+ * one keeps the larger of the old and new proposal; more steps than par[6]
+ * (10000 when par[6] <= 0), or a step that does not advance t, give +Inf. */
+float ora_lv_sse_tsit5(const float* th, const ora_problem* P, int adaptive) {
+  float u[2] = {P->par[0], P->par[1]}, k[7][2];
+  lv_rhs(u[0], u[1], th, &k[0][0], &k[0][1]);
+  float loss = 0.0f, t = 0.0f, dt = adaptive ? P->par[3] : P->par[2];
+  int32_t steps = 0;
+  for (int kk = 0; kk < P->n_obs; ++kk) {
+    if (kk > 0) {
+      if (!adaptive) {
+        for (int s = 0; s < P->steps_per_obs; ++s) {
+          float un[2], err[2];
+          lv_tsit5_step(u, k, dt, th, un, err);
+          u[0] = un[0], u[1] = un[1];
+          k[0][0] = k[6][0], k[0][1] = k[6][1];
+        }
+      } else {
+        float t_next = (float)kk * P->par[2];
+        float atol = P->par[4], rtol = P->par[5];
+        int32_t max_steps = P->par[6] > 0.0f ? (int32_t)P->par[6] : 10000;
+        while (t < t_next) {
+          int hit = !(t + dt < t_next);
+          float h = hit ? t_next - t : dt;
+          if (++steps > max_steps || !(t + h > t)) return INFINITY;
+          float un[2], err[2];
+          lv_tsit5_step(u, k, h, th, un, err);
+          float s[2];
+          for (int c = 0; c < 2; ++c) {
+            float a = fabsf(u[c]), b = fabsf(un[c]);
+            s[c] = err[c] / (atol + (a > b ? a : b) * rtol);
+          }
+          float eest = sqrtf((s[0] * s[0] + s[1] * s[1]) * 0.5f);
+          float q = 5.0f;
+          if (eest <= 3.0e38f) {
+            q = ora_root5(eest) / 0.9f;
+            q = q < 0.1f ? 0.1f : (q > 5.0f ? 5.0f : q);
+          }
+          float hnew = h / q;
+          if (eest <= 1.0f) {
+            t = hit ? t_next : t + h;
+            u[0] = un[0], u[1] = un[1];
+            k[0][0] = k[6][0], k[0][1] = k[6][1];
+            dt = hit ? (hnew > dt ? hnew : dt) : hnew;
+          } else {
+            dt = hnew;
+          }
+        }
+      }
+    }
+    float e1 = P->obs[2 * kk] - u[0];
+    float e2 = P->obs[2 * kk + 1] - u[1];
+    loss = loss + (e1 * e1 + e2 * e2);
+  }
+  return loss;
+}
+
 float ora_lv_sse(const float* th, const ora_problem* P) {
   float u1 = P->par[0], u2 = P->par[1];
   float dt = P->par[2];
@@ -224,6 +355,10 @@ float ora_objective(const ora_problem* P, const float* xin) {
     }
     case ORA_LOTKA_VOLTERRA:
       return ora_lv_sse(z, P);
+    case ORA_LOTKA_VOLTERRA_TSIT5:
+      return ora_lv_sse_tsit5(z, P, 0);
+    case ORA_LOTKA_VOLTERRA_ATSIT5:
+      return ora_lv_sse_tsit5(z, P, 1);
   }
   return NAN;
 }

@@ -26,6 +26,7 @@ MAXPAR = 8
 ROSENBROCK, RASTRIGIN, SPHERE, BEALE, BOOTH, HIMMELBLAU, QUADRATIC, BOUNDARY_QUADRATIC, ROSEN2 = (
     1, 2, 3, 4, 5, 6, 7, 8, 9)
 LOTKA_VOLTERRA = 16
+LOTKA_VOLTERRA_TSIT5, LOTKA_VOLTERRA_ATSIT5 = 17, 18
 GBEST_PBEST, GBEST_CURRENT = 0, 1
 SAMPLER_UNIFORM, SAMPLER_UNBOUNDED, SAMPLER_LATIN_HYPERCUBE = 0, 1, 2
 CLAMP_COMPONENT, CLAMP_LEXICOGRAPHIC = 0, 1
@@ -78,6 +79,9 @@ def lib():
         L.ora_cost.argtypes = [C.POINTER(_Prob), fp, C.c_float]
         L.ora_cost.restype = C.c_float
         L.ora_lv_sse.argtypes = [fp, C.POINTER(_Prob)]
+        L.ora_root5.argtypes = [C.c_float]
+        L.ora_root5.restype = C.c_float
+        L.ora_tsit5_tableau.argtypes = [fp, fp]
         L.ora_lv_sse.restype = C.c_float
         L.ora_eval.argtypes = [C.POINTER(_Prob), C.c_int64, fp, fp]
         L.ora_argmin.argtypes = [fp, C.c_int64, C.c_int]
@@ -169,6 +173,18 @@ def penalty(prob: Problem, x, iter_arg: float) -> np.float32:
 def cost(prob: Problem, x, iter_arg: float) -> np.float32:
     x = np.ascontiguousarray(x, np.float32)
     return np.float32(lib().ora_cost(C.byref(prob.c()), _fp(x), float(iter_arg)))
+
+
+def root5(x: float) -> np.float32:
+    return np.float32(lib().ora_root5(float(np.float32(x))))
+
+
+def tsit5_tableau():
+    """(a 7x6, btilde 7) as the oracle stores them (float32)."""
+    a = np.zeros((7, 6), np.float32)
+    e = np.zeros(7, np.float32)
+    lib().ora_tsit5_tableau(_fp(a), _fp(e))
+    return a, e
 
 
 def lhs_permute(i: int, n: int, j: int, epoch: int, seed: int) -> int:

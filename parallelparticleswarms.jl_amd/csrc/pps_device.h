@@ -12,6 +12,11 @@
 
 #include "pps_shared.h"
 
+constexpr bool is_lotka_volterra(int obj) {
+  return obj == PPS_OBJ_LOTKA_VOLTERRA || obj == PPS_OBJ_LOTKA_VOLTERRA_TSIT5 ||
+         obj == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5;
+}
+
 namespace pps {
 
 // ---------------------------------------------------------------------------
@@ -182,6 +187,168 @@ __device__ __forceinline__ float lv_sse(const float* th, const ProbArgs& P) {
   return loss;
 }
 
+// ---------------------------------------------------------------------------
+// Tsitouras 5(4) (Ch. Tsitouras, Comput. Math. Appl. 62 (2011) 770-775), the
+// method of the reference's default ode_alg = GPUTsit5() (src/ode_pso.jl:54-170):
+// nodes c, stage weights a, solution weights b (= row 7 of a, FSAL) and the
+// error weights btilde = b - bhat of the embedded 4th-order solution, as FP32.
+// ---------------------------------------------------------------------------
+#define TS_A21 0.161f
+#define TS_A31 -0.008480655492356989f
+#define TS_A32 0.335480655492357f
+#define TS_A41 2.897153057105493f
+#define TS_A42 -6.359448489975075f
+#define TS_A43 4.3622954328695815f
+#define TS_A51 5.325864828439257f
+#define TS_A52 -11.748883564062828f
+#define TS_A53 7.4955393428898365f
+#define TS_A54 -0.09249506636175525f
+#define TS_A61 5.86145544294642f
+#define TS_A62 -12.92096931784711f
+#define TS_A63 8.159367898576159f
+#define TS_A64 -0.071584973281401f
+#define TS_A65 -0.028269050394068383f
+#define TS_B1 0.09646076681806523f
+#define TS_B2 0.01f
+#define TS_B3 0.4798896504144996f
+#define TS_B4 1.379008574103742f
+#define TS_B5 -3.290069515436081f
+#define TS_B6 2.324710524099774f
+#define TS_E1 -0.00178001105222577714f
+#define TS_E2 -0.0008164344596567469f
+#define TS_E3 0.007880878010261995f
+#define TS_E4 -0.1447110071732629f
+#define TS_E5 0.5823571654525552f
+#define TS_E6 -0.45808210592918697f
+#define TS_E7 0.015151515151515152f
+
+// One Tsit5 step of the Lotka-Volterra system from (u1, u2) with k1 = f(u):
+// stage sums left to right, y_i = u + h * sum_j a_ij k_j; returns the new
+// state, k7 = f(new state) (the next step's k1) and, when ERR, the error
+// estimate h * sum_j btilde_j k_j.
+template <bool ERR>
+__device__ __forceinline__ void lv_tsit5_step(float u1, float u2, float k1a, float k1b, float h,
+                                              const float* th, float& n1, float& n2, float& k7a,
+                                              float& k7b, float& e1, float& e2) {
+  const float p1 = th[0], p2 = th[1], p3 = th[2], p4 = th[3];
+  float k2a, k2b, k3a, k3b, k4a, k4b, k5a, k5b, k6a, k6b;
+  lv_rhs(u1 + h * (TS_A21 * k1a), u2 + h * (TS_A21 * k1b), p1, p2, p3, p4, k2a, k2b);
+  lv_rhs(u1 + h * (TS_A31 * k1a + TS_A32 * k2a), u2 + h * (TS_A31 * k1b + TS_A32 * k2b), p1, p2,
+         p3, p4, k3a, k3b);
+  lv_rhs(u1 + h * ((TS_A41 * k1a + TS_A42 * k2a) + TS_A43 * k3a),
+         u2 + h * ((TS_A41 * k1b + TS_A42 * k2b) + TS_A43 * k3b), p1, p2, p3, p4, k4a, k4b);
+  lv_rhs(u1 + h * (((TS_A51 * k1a + TS_A52 * k2a) + TS_A53 * k3a) + TS_A54 * k4a),
+         u2 + h * (((TS_A51 * k1b + TS_A52 * k2b) + TS_A53 * k3b) + TS_A54 * k4b), p1, p2, p3, p4,
+         k5a, k5b);
+  lv_rhs(u1 + h * ((((TS_A61 * k1a + TS_A62 * k2a) + TS_A63 * k3a) + TS_A64 * k4a) + TS_A65 * k5a),
+         u2 + h * ((((TS_A61 * k1b + TS_A62 * k2b) + TS_A63 * k3b) + TS_A64 * k4b) + TS_A65 * k5b),
+         p1, p2, p3, p4, k6a, k6b);
+  n1 = u1 + h * (((((TS_B1 * k1a + TS_B2 * k2a) + TS_B3 * k3a) + TS_B4 * k4a) + TS_B5 * k5a) +
+                 TS_B6 * k6a);
+  n2 = u2 + h * (((((TS_B1 * k1b + TS_B2 * k2b) + TS_B3 * k3b) + TS_B4 * k4b) + TS_B5 * k5b) +
+                 TS_B6 * k6b);
+  lv_rhs(n1, n2, p1, p2, p3, p4, k7a, k7b);
+  if (ERR) {
+    e1 = h * ((((((TS_E1 * k1a + TS_E2 * k2a) + TS_E3 * k3a) + TS_E4 * k4a) + TS_E5 * k5a) +
+               TS_E6 * k6a) + TS_E7 * k7a);
+    e2 = h * ((((((TS_E1 * k1b + TS_E2 * k2b) + TS_E3 * k3b) + TS_E4 * k4b) + TS_E5 * k5b) +
+               TS_E6 * k6b) + TS_E7 * k7b);
+  }
+}
+
+// x^(1/5) for x > 0 from basic IEEE operations only (an exponent-halving seed
+// and four Newton steps y <- (4y + x/y^4) / 5), so that every platform
+// computes the same float; 0 for x <= 0.
+__device__ __forceinline__ float root5(float x) {
+  if (!(x > 0.0f)) return 0.0f;
+  if (x == __builtin_inff()) return x;
+  const int32_t bits = (int32_t)__float_as_uint(x);
+  float y = __uint_as_float((uint32_t)((bits - 0x3f800000) / 5 + 0x3f800000));
+#pragma unroll
+  for (int it = 0; it < 4; ++it) {
+    const float y2 = y * y;
+    y = (4.0f * y + x / (y2 * y2)) / 5.0f;
+  }
+  return y;
+}
+
+// Step limit of one adaptive evaluation when par[6] is not given.
+#define LV_MAX_STEPS 10000
+
+// Lotka-Volterra SSE with Tsit5: fixed steps (dt = par[2], par[3] steps per
+// observation, the reference's parameter_estim_ode! with Val(false)) or
+// adaptive steps (par[2] = observation interval, par[3] = initial dt,
+// par[4] = abstol, par[5] = reltol; Val(true)).  Adaptive control: scaled
+// error s_i = e_i / (abstol + max(|u_i|, |u_new_i|) reltol), EEst =
+// sqrt((s_1^2 + s_2^2) / 2); accept when EEst <= 1; factor q = EEst^(1/5) /
+// 0.9 clamped to [0.1, 5] (5 when EEst is not finite), next dt = h / q; steps
+// end exactly on the observation times, and a step shortened to meet one does
+// not shrink the following proposal.  More steps than par[6] (10000 when
+// par[6] <= 0), or a step too small to advance t, make the loss +Inf: a
+// parameter vector whose trajectory turns stiff (the whole-vector clamp of
+// the ODE path admits negative rates) would otherwise hold its wavefront for
+// the explicit method's stability-limited steps.  Loss as lv_sse.
+template <bool ADAPTIVE>
+__device__ __forceinline__ float lv_sse_tsit5(const float* th, const ProbArgs& P) {
+  float u1 = P.par[0], u2 = P.par[1];
+  float k1a, k1b;
+  lv_rhs(u1, u2, th[0], th[1], th[2], th[3], k1a, k1b);
+  float loss = 0.0f;
+  const float* obs = P.obs;
+  float t = 0.0f, dt = ADAPTIVE ? P.par[3] : P.par[2];
+  int32_t steps = 0;
+  for (int k = 0; k < P.n_obs; ++k) {
+    if (k > 0) {
+      if (!ADAPTIVE) {
+        for (int s = 0; s < P.steps_per_obs; ++s) {
+          float n1, n2, k7a, k7b, e1, e2;
+          lv_tsit5_step<false>(u1, u2, k1a, k1b, dt, th, n1, n2, k7a, k7b, e1, e2);
+          u1 = n1;
+          u2 = n2;
+          k1a = k7a;
+          k1b = k7b;
+        }
+      } else {
+        const float t_next = (float)k * P.par[2];
+        const float atol = P.par[4], rtol = P.par[5];
+        const int32_t max_steps = P.par[6] > 0.0f ? (int32_t)P.par[6] : LV_MAX_STEPS;
+        while (t < t_next) {
+          const bool hit = !(t + dt < t_next);
+          const float h = hit ? t_next - t : dt;
+          if (++steps > max_steps || !(t + h > t)) return __builtin_inff();
+          float n1, n2, k7a, k7b, e1, e2;
+          lv_tsit5_step<true>(u1, u2, k1a, k1b, h, th, n1, n2, k7a, k7b, e1, e2);
+          const float a1 = __builtin_fabsf(u1), b1 = __builtin_fabsf(n1);
+          const float a2 = __builtin_fabsf(u2), b2 = __builtin_fabsf(n2);
+          const float s1 = e1 / (atol + (a1 > b1 ? a1 : b1) * rtol);
+          const float s2 = e2 / (atol + (a2 > b2 ? a2 : b2) * rtol);
+          const float eest = sqrtf((s1 * s1 + s2 * s2) * 0.5f);
+          float q = 5.0f;
+          if (eest <= 3.0e38f) {
+            q = root5(eest) / 0.9f;
+            q = q < 0.1f ? 0.1f : (q > 5.0f ? 5.0f : q);
+          }
+          const float hnew = h / q;
+          if (eest <= 1.0f) {
+            t = hit ? t_next : t + h;
+            u1 = n1;
+            u2 = n2;
+            k1a = k7a;
+            k1b = k7b;
+            dt = hit ? (hnew > dt ? hnew : dt) : hnew;
+          } else {
+            dt = hnew;
+          }
+        }
+      }
+    }
+    const float e1 = obs[2 * k] - u1;
+    const float e2 = obs[2 * k + 1] - u2;
+    loss = loss + (e1 * e1 + e2 * e2);
+  }
+  return loss;
+}
+
 // Objective f(x - shift) over a register array (compile-time capacity MAXD,
 // runtime dimension d <= MAXD).  OBJ is a compile-time PPS_OBJ_* constant.
 template <int MAXD, int OBJ>
@@ -248,6 +415,10 @@ __device__ __forceinline__ float objective(const float (&xin)[MAXD], const int d
     return a * a + P.par[1] * (b * b);
   } else if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA) {
     return lv_sse(z, P);
+  } else if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA_TSIT5) {
+    return lv_sse_tsit5<false>(z, P);
+  } else if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5) {
+    return lv_sse_tsit5<true>(z, P);
   } else {
     return __uint_as_float(0x7FC00000u);
   }
@@ -335,7 +506,7 @@ __device__ __forceinline__ void velocity_position(
   }
   // The lexicographic clamp is the ODE path's (src/ode_pso.jl:18-19): it is
   // compiled only into the Lotka-Volterra kernels.
-  if (OBJ != PPS_OBJ_LOTKA_VOLTERRA || P.clamp == PPS_CLAMP_COMPONENT) {
+  if (!is_lotka_volterra(OBJ) || P.clamp == PPS_CLAMP_COMPONENT) {
 #pragma unroll
     for (int j = 0; j < MAXD; ++j) {
       if (j < d) x[j] = jl_min(jl_max(x[j], P.lb[j]), P.ub[j]);

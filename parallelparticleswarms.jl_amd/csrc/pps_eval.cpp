@@ -2,6 +2,7 @@
 // re-evaluate FP32 minima found on the device (the reference evaluates the
 // final objective on the host in the problem's element type,
 // src/solve.jl:28-33).
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -37,6 +38,88 @@ double lv_sse_f64(const double* th, const pps_problem* P) {
       }
     }
     const double e1 = (double)P->obs[2 * k] - u1, e2 = (double)P->obs[2 * k + 1] - u2;
+    loss += e1 * e1 + e2 * e2;
+  }
+  return loss;
+}
+
+// Tsitouras 5(4) in FP64, fixed or adaptive (same controller as the device's
+// lv_sse_tsit5: RMS scaled error, factor EEst^(1/5)/0.9 in [0.1, 5], steps
+// ending on the observation times).
+double lv_sse_tsit5_f64(const double* th, const pps_problem* P, bool adaptive) {
+  static const double A[7][6] = {
+      {0, 0, 0, 0, 0, 0},
+      {0.161, 0, 0, 0, 0, 0},
+      {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
+      {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
+      {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
+      {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401,
+       -0.028269050394068383, 0},
+      {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081,
+       2.324710524099774}};
+  static const double E[7] = {-0.00178001105222577714, -0.0008164344596567469,
+                              0.007880878010261995,    -0.1447110071732629,
+                              0.5823571654525552,      -0.45808210592918697,
+                              0.015151515151515152};
+  auto rhs = [&](double a, double b, double& f1, double& f2) {
+    f1 = th[0] * a - th[1] * a * b;
+    f2 = -th[2] * b + th[3] * a * b;
+  };
+  double u[2] = {P->params[0], P->params[1]}, k[7][2];
+  rhs(u[0], u[1], k[0][0], k[0][1]);
+  auto step = [&](double h, double* un, double* err) {
+    for (int i = 1; i < 7; ++i) {
+      double y[2];
+      for (int c = 0; c < 2; ++c) {
+        double acc = 0.0;
+        for (int j = 0; j < i; ++j) acc += A[i][j] * k[j][c];
+        y[c] = u[c] + h * acc;
+      }
+      if (i == 6) un[0] = y[0], un[1] = y[1];
+      rhs(y[0], y[1], k[i][0], k[i][1]);
+    }
+    for (int c = 0; c < 2; ++c) {
+      double acc = 0.0;
+      for (int j = 0; j < 7; ++j) acc += E[j] * k[j][c];
+      err[c] = h * acc;
+    }
+  };
+  double loss = 0.0, t = 0.0, dt = adaptive ? P->params[3] : P->params[2];
+  long steps = 0;
+  for (int kk = 0; kk < P->n_obs; ++kk) {
+    if (kk > 0) {
+      const double t_next = kk * (double)P->params[2];
+      int fixed_left = adaptive ? 0 : (int)P->params[3];
+      while (adaptive ? t < t_next : fixed_left-- > 0) {
+        const bool hit = adaptive && !(t + dt < t_next);
+        const double h = hit ? t_next - t : dt;
+        const long max_steps = P->n_params > 6 && P->params[6] > 0 ? (long)P->params[6] : 10000;
+        if (adaptive && (++steps > max_steps || !(t + h > t))) return INFINITY;
+        double un[2], err[2];
+        step(h, un, err);
+        if (!adaptive) {
+          u[0] = un[0], u[1] = un[1];
+          k[0][0] = k[6][0], k[0][1] = k[6][1];
+          t += h;
+          continue;
+        }
+        const double s1 = err[0] / (P->params[4] + std::max(std::fabs(u[0]), std::fabs(un[0])) * P->params[5]);
+        const double s2 = err[1] / (P->params[4] + std::max(std::fabs(u[1]), std::fabs(un[1])) * P->params[5]);
+        const double eest = std::sqrt((s1 * s1 + s2 * s2) / 2.0);
+        double q = 5.0;
+        if (std::isfinite(eest)) q = std::min(5.0, std::max(0.1, std::pow(eest, 0.2) / 0.9));
+        const double hnew = h / q;
+        if (eest <= 1.0) {
+          t = hit ? t_next : t + h;
+          u[0] = un[0], u[1] = un[1];
+          k[0][0] = k[6][0], k[0][1] = k[6][1];
+          dt = hit ? std::max(dt, hnew) : hnew;
+        } else {
+          dt = hnew;
+        }
+      }
+    }
+    const double e1 = (double)P->obs[2 * kk] - u[0], e2 = (double)P->obs[2 * kk + 1] - u[1];
     loss += e1 * e1 + e2 * e2;
   }
   return loss;
@@ -94,6 +177,10 @@ double eval_one(const pps_problem* P, const double* xin) {
     }
     case PPS_OBJ_LOTKA_VOLTERRA:
       return lv_sse_f64(z.data(), P);
+    case PPS_OBJ_LOTKA_VOLTERRA_TSIT5:
+      return lv_sse_tsit5_f64(z.data(), P, false);
+    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5:
+      return lv_sse_tsit5_f64(z.data(), P, true);
   }
   return NAN;
 }
@@ -102,8 +189,12 @@ double eval_one(const pps_problem* P, const double* xin) {
 
 extern "C" int pps_eval_f64(const pps_problem* prob, int64_t n, const double* x, double* f) {
   if (!prob || (n > 0 && (!x || !f)) || n < 0 || prob->d < 1 || prob->d > 64) return PPS_E_INVALID;
-  if (prob->objective == PPS_OBJ_LOTKA_VOLTERRA &&
-      (prob->d != 4 || !prob->obs || prob->n_obs < 1 || !prob->params || prob->n_params < 4))
+  const bool lv = prob->objective == PPS_OBJ_LOTKA_VOLTERRA ||
+                  prob->objective == PPS_OBJ_LOTKA_VOLTERRA_TSIT5 ||
+                  prob->objective == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5;
+  const int need = prob->objective == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5 ? 6 : 4;
+  if (lv && (prob->d != 4 || !prob->obs || prob->n_obs < 1 || !prob->params ||
+             prob->n_params < need))
     return PPS_E_INVALID;
   for (int64_t i = 0; i < n; ++i) f[i] = eval_one(prob, x + i * prob->d);
   return PPS_OK;

@@ -15,7 +15,14 @@ const KernelSet* kernels_two_dim(int obj, int d) {
     default: return nullptr;
   }
 }
-const KernelSet* kernels_lotka_volterra(int d) {
-  return d == 4 ? Launch<4, true, PPS_OBJ_LOTKA_VOLTERRA>::set() : nullptr;
+const KernelSet* kernels_lotka_volterra(int obj, int d) {
+  if (d != 4) return nullptr;
+  switch (obj) {
+    case PPS_OBJ_LOTKA_VOLTERRA: return Launch<4, true, PPS_OBJ_LOTKA_VOLTERRA>::set();
+    case PPS_OBJ_LOTKA_VOLTERRA_TSIT5: return Launch<4, true, PPS_OBJ_LOTKA_VOLTERRA_TSIT5>::set();
+    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5:
+      return Launch<4, true, PPS_OBJ_LOTKA_VOLTERRA_ATSIT5>::set();
+    default: return nullptr;
+  }
 }
 }  // namespace pps

@@ -82,20 +82,37 @@ int fill_prob_args(const pps_problem* prob, ProbArgs* P) {
     case PPS_OBJ_ROSENBROCK:
     case PPS_OBJ_ROSEN2: need_par = 2; break;
     case PPS_OBJ_QUADRATIC: need_par = 1; break;
-    case PPS_OBJ_LOTKA_VOLTERRA: need_par = 4; break;
+    case PPS_OBJ_LOTKA_VOLTERRA:
+    case PPS_OBJ_LOTKA_VOLTERRA_TSIT5: need_par = 4; break;
+    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5: need_par = 6; break;
     default: break;
   }
   if (prob->n_params < need_par)
     return fail(PPS_E_INVALID, "objective %d needs %d parameters, got %d", prob->objective,
                 need_par, prob->n_params);
-  if (prob->objective == PPS_OBJ_LOTKA_VOLTERRA) {
+  const bool lv = prob->objective == PPS_OBJ_LOTKA_VOLTERRA ||
+                  prob->objective == PPS_OBJ_LOTKA_VOLTERRA_TSIT5 ||
+                  prob->objective == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5;
+  if (lv) {
     if (d != 4) return fail(PPS_E_INVALID, "Lotka-Volterra estimation has 4 parameters, d=%d", d);
     if (!prob->obs || prob->n_obs < 1) return fail(PPS_E_INVALID, "Lotka-Volterra needs observations");
-    const float spo = prob->params[3];
-    if (!(spo >= 1.0f && spo <= 1.0e6f) || spo != std::floor(spo))
-      return fail(PPS_E_INVALID, "steps per observation must be a positive integer");
     P->n_obs = prob->n_obs;
-    P->steps_per_obs = (int32_t)spo;
+    if (prob->objective == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5) {
+      const float* q = prob->params;
+      if (!(q[2] > 0.0f && std::isfinite(q[2])))
+        return fail(PPS_E_INVALID, "observation interval must be positive");
+      if (!(q[3] > 0.0f && std::isfinite(q[3])))
+        return fail(PPS_E_INVALID, "initial step must be positive");
+      if (!(q[4] >= 0.0f && q[5] >= 0.0f && q[4] + q[5] > 0.0f && std::isfinite(q[4] + q[5])))
+        return fail(PPS_E_INVALID, "abstol and reltol must be >= 0 with a positive sum");
+      if (prob->n_params > 6 && !(q[6] >= 1.0f && q[6] <= 1.0e7f && q[6] == std::floor(q[6])))
+        return fail(PPS_E_INVALID, "the step limit must be an integer in 1..1e7");
+    } else {
+      const float spo = prob->params[3];
+      if (!(spo >= 1.0f && spo <= 1.0e6f) || spo != std::floor(spo))
+        return fail(PPS_E_INVALID, "steps per observation must be a positive integer");
+      P->steps_per_obs = (int32_t)spo;
+    }
   }
   if (prob->n_cons < 0 || prob->n_cons > PPS_MAXCONS)
     return fail(PPS_E_INVALID, "n_cons %d outside 0..%d", prob->n_cons, PPS_MAXCONS);
@@ -408,7 +425,9 @@ static int swarm_create(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_p
     return fail(PPS_E_INVALID, "unknown gbest_from %d", prm->gbest_from);
   if (prm->clamp != PPS_CLAMP_COMPONENT && prm->clamp != PPS_CLAMP_LEXICOGRAPHIC)
     return fail(PPS_E_INVALID, "unknown clamp %d", prm->clamp);
-  if (prm->clamp == PPS_CLAMP_LEXICOGRAPHIC && prob && prob->objective != PPS_OBJ_LOTKA_VOLTERRA)
+  if (prm->clamp == PPS_CLAMP_LEXICOGRAPHIC && prob && prob->objective != PPS_OBJ_LOTKA_VOLTERRA &&
+      prob->objective != PPS_OBJ_LOTKA_VOLTERRA_TSIT5 &&
+      prob->objective != PPS_OBJ_LOTKA_VOLTERRA_ATSIT5)
     return fail(PPS_E_INVALID,
                 "lexicographic clamping is the ODE estimation path's; it needs "
                 "PPS_OBJ_LOTKA_VOLTERRA (objective %d given)", prob->objective);

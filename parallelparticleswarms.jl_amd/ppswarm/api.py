@@ -153,21 +153,46 @@ def Rosen2():
 
 @dataclass(frozen=True)
 class LotkaVolterraSSE(Objective):
-    """Sum of squared errors of an FP32 RK4 Lotka-Volterra trajectory against
+    """Sum of squared errors of an FP32 Lotka-Volterra trajectory against
     observations (the loss of src/ode_pso.jl:47-52).  The decision variable is
-    (α, β, γ, δ); ``obs`` is (n_obs, 2), sampled every ``steps_per_obs`` RK4
-    steps of ``dt`` starting from ``u0`` at sample 0."""
+    (α, β, γ, δ); ``obs`` is (n_obs, 2), sample k taken at t = k * dt *
+    steps_per_obs starting from ``u0`` at sample 0.  Solved in the kernel with
+    fixed-step RK4 (``ode_alg="rk4"``), fixed-step Tsit5 (``"tsit5"``, the
+    reference's GPUTsit5 with Val(false)) or adaptive Tsit5 (``"tsit5",
+    adaptive=True``, Val(true): ``dt`` is the initial step, ``abstol`` /
+    ``reltol`` the tolerances, ``maxiters`` the step limit of one evaluation,
+    beyond which the loss is +Inf)."""
 
     obs: Any = None
     u0: tuple = (1.0, 1.0)
     dt: float = 0.01
     steps_per_obs: int = 10
+    adaptive: bool = False
+    abstol: float = 1e-6
+    reltol: float = 1e-3
+    maxiters: int = 10000
+
+    def params(self) -> np.ndarray:
+        if self.adaptive:
+            obs_dt = np.float32(np.float64(self.dt) * self.steps_per_obs)
+            return np.array([self.u0[0], self.u0[1], obs_dt, self.dt, self.abstol, self.reltol,
+                             self.maxiters], np.float32)
+        return np.array([self.u0[0], self.u0[1], self.dt, self.steps_per_obs], np.float32)
 
 
-def lotka_volterra_sse(obs, u0=(1.0, 1.0), dt=0.01, steps_per_obs=10) -> LotkaVolterraSSE:
+def lotka_volterra_sse(obs, u0=(1.0, 1.0), dt=0.01, steps_per_obs=10, *, ode_alg="rk4",
+                       adaptive=False, abstol=1e-6, reltol=1e-3,
+                       maxiters=10000) -> LotkaVolterraSSE:
     obs = np.ascontiguousarray(obs, np.float32).reshape(-1, 2)
-    return LotkaVolterraSSE(L.OBJ_LOTKA_VOLTERRA, "lotka_volterra", 4, 4, obs, tuple(u0),
-                            float(dt), int(steps_per_obs))
+    if ode_alg not in ("rk4", "tsit5"):
+        raise ValueError(f"ode_alg is 'rk4' or 'tsit5', got {ode_alg!r}")
+    if adaptive and ode_alg != "tsit5":
+        raise ValueError("adaptive stepping uses Tsit5 (ode_alg='tsit5')")
+    code = (L.OBJ_LOTKA_VOLTERRA if ode_alg == "rk4" else
+            L.OBJ_LOTKA_VOLTERRA_ATSIT5 if adaptive else L.OBJ_LOTKA_VOLTERRA_TSIT5)
+    return LotkaVolterraSSE(code, "lotka_volterra", 4, 6 if adaptive else 4, obs, tuple(u0),
+                            float(dt), int(steps_per_obs), bool(adaptive), float(abstol),
+                            float(reltol), int(maxiters))
 
 
 # ---------------------------------------------------------------------------
@@ -260,7 +285,7 @@ class OptimizationProblem:
         if f.fixed_dim is not None and self.d != f.fixed_dim:
             raise ValueError(f"{f.name} is {f.fixed_dim}-dimensional, u0 has {self.d}")
         if isinstance(f, LotkaVolterraSSE):
-            self.p = np.array([f.u0[0], f.u0[1], f.dt, f.steps_per_obs], np.float32)
+            self.p = f.params()
         elif p is None:
             self.p = np.zeros(0, np.float32)
         else:

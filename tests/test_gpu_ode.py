@@ -1,4 +1,5 @@
-"""Lotka-Volterra parameter estimation (fused PSO step + in-kernel FP32 RK4).
+"""Lotka-Volterra parameter estimation (fused PSO step + in-kernel FP32 RK4, and
+Tsit5 with fixed or adaptive steps).
 
 At the benchmark shape (2^14 particles, 4 parameters in [0, 5], 101
 observations, RK4 dt = 0.01): bit-exact parity with the oracle over the first
@@ -41,3 +42,44 @@ def test_lv_estimation_recovers_parameters():
     assert sol.objective < 0.6
     np.testing.assert_allclose(sol.u, theta, rtol=0.05)
     assert abs(sol.objective - sol.objective_f32) < 1e-3
+
+
+def _tsit5_pair(n, seed, adaptive, mode=L.MODE_SYNC, launch=L.LAUNCH_GRAPH):
+    f = P.lotka_volterra_sse(OBS, u0=(1.0, 1.0), dt=0.01, steps_per_obs=10, ode_alg="tsit5",
+                             adaptive=adaptive, abstol=1e-6, reltol=1e-3)
+    lb, ub = np.zeros(4, np.float32), np.full(4, 5.0, np.float32)
+    prob = P.OptimizationProblem(f, np.zeros(4), lb=lb, ub=ub)
+    g = P.Swarm(P.context(), prob, P.swarm_params(n, mode, seed=seed, clamp=L.CLAMP_LEXICOGRAPHIC,
+                                                  launch=launch))
+    obj = O.LOTKA_VOLTERRA_ATSIT5 if adaptive else O.LOTKA_VOLTERRA_TSIT5
+    o = O.Swarm(O.Problem(obj, 4, lb, ub, f.params(), clamp=O.CLAMP_LEXICOGRAPHIC, obs=OBS), n,
+                seed)
+    return g, o
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+@pytest.mark.parametrize("mode,n", [(L.MODE_SYNC, 4096), (L.MODE_SYNC, 1000), (L.MODE_ASYNC, 2048)])
+def test_lv_tsit5_parity(adaptive, mode, n):
+    # the reference's default GPUTsit5, fixed (Val(false)) and adaptive (Val(true))
+    note(f"LV Tsit5 adaptive={adaptive} mode={mode} n={n}")
+    g, o = _tsit5_pair(n, 31 + n, adaptive, mode,
+                       L.LAUNCH_GRAPH if mode == L.MODE_SYNC and n > 1024 else L.LAUNCH_AUTO)
+    assert_same_state(g, o, "after init")
+    assert_same_gbest(g, o, "after init")
+    g.run(3)
+    if mode == L.MODE_SYNC:
+        o.run_sync(3)
+    else:
+        o.run_async(3)
+    assert_same_state(g, o, "after 3 iterations")
+    assert_same_gbest(g, o)
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_lv_tsit5_estimation_recovers_parameters(adaptive):
+    note(f"LV Tsit5 estimation adaptive={adaptive}")
+    sol, sw = P.parameter_estim_ode(OBS, np.zeros(4), np.full(4, 5.0), 1 << 14, 300, seed=6,
+                                    ode_alg="tsit5", adaptive=adaptive)
+    theta = np.array(DOC["theta"])
+    assert sol.objective < 0.6
+    np.testing.assert_allclose(sol.u, theta, rtol=0.05)
