@@ -317,6 +317,45 @@ function solve(prob::OptimizationProblem,
     return solve!(init(prob, alg; kwargs...); maxiters, kwargs...)
 end
 
+# parameter_estim_ode! (src/ode_pso.jl:54-170) for the Lotka-Volterra model,
+# solved inside the fused PSO kernel.  `data` is 2 x n_obs (sample k at
+# t = k * dt * saveat_steps from u0); ode_alg :rk4 or :tsit5 (the reference's
+# default GPUTsit5), adaptive = true for its Val(true) path (dt = initial step).
+# Returns (gbest position, FP32 loss).
+function parameter_estim_lv(data::AbstractMatrix, lb, ub; backend = MI355XBackend(),
+        num_particles = 2^14, maxiters = 300, u0 = (1.0f0, 1.0f0), dt = 0.01f0,
+        saveat_steps = 10, ode_alg = :tsit5, adaptive = false, abstol = 1.0f-6,
+        reltol = 1.0f-3, max_steps = 10000, w = 0.7298f0, wdamp = 1.0f0,
+        seed::Integer = rand(UInt64))
+    code = ode_alg === :rk4 ? Int32(16) : adaptive ? Int32(18) : Int32(17)
+    (ode_alg === :rk4 && adaptive) && throw(ArgumentError("adaptive stepping uses :tsit5"))
+    par = adaptive ?
+          Float32[u0[1], u0[2], Float32(dt) * saveat_steps, dt, abstol, reltol, max_steps] :
+          Float32[u0[1], u0[2], dt, saveat_steps]
+    obs = Float32.(vec(data))                      # [u1(t0), u2(t0), u1(t1), ...]
+    l, u = Float32.(collect(lb)), Float32.(collect(ub))
+    z = zeros(Float32, 4)
+    c = CProblem(code, Int32(4), pointer(l), pointer(u), pointer(par), Int32(length(par)),
+        Ptr{Float32}(C_NULL), pointer(obs), Int32(size(data, 2)), Ptr{Float32}(C_NULL),
+        Int32(0), pointer(z))
+    prm = CSwarmParams(num_particles, PPS_MODE_SYNC, PPS_GBEST_PBEST, Int32(1), Int32(0),
+        UInt64(seed), 1.4962f0, 1.4962f0, Int32(2), Int32(0), PPS_SAMPLER_UNIFORM)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve l u par obs z begin
+        check(ccall((:pps_swarm_init, libpps), Cint,
+            (Ptr{Cvoid}, Ref{CProblem}, Ref{CSwarmParams}, Ptr{Ptr{Cvoid}}),
+            context(backend), c, prm, h))
+    end
+    try
+        check(ccall((:pps_swarm_run, libpps), Cint, (Ptr{Cvoid}, Int32, Float32, Float32),
+            h[], Int32(maxiters), Float32(w), Float32(wdamp)))
+        pos, cost, _ = read_gbest(h[], 4)
+        return pos, cost
+    finally
+        ccall((:pps_swarm_free, libpps), Cint, (Ptr{Cvoid},), h[])
+    end
+end
+
 # HybridPSO{MI355XBackend, SimpleLBFGS} (src/hybrid.jl:100-158): PSO, then
 # projected L-BFGS from every particle's best position, keep the better.
 function solve(prob::OptimizationProblem, alg::HybridPSO{<:MI355XBackend, <:SimpleLBFGS},
@@ -344,6 +383,7 @@ function solve(prob::OptimizationProblem, alg::HybridPSO{<:MI355XBackend, <:Simp
         stats = OptimizationStats(; time = (time() - t0) + sol_pso.stats.time))
 end
 
-export MI355XBackend, PPSObjective, PPSQuadraticConstraints, PPSLatinHypercube
+export MI355XBackend, PPSObjective, PPSQuadraticConstraints, PPSLatinHypercube,
+    parameter_estim_lv
 
 end # module
