@@ -459,7 +459,7 @@ __global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArg
 
 // ---------------------------------------------------------------------------
 // Launchers.
-constexpr int BLK = 256;   // workgroup size of the streaming kernels
+constexpr int BLK = PPS_BLK;  // workgroup size of the streaming kernels
 constexpr int QBLK = 1024; // queue-lock block (the reference's workgroup size)
 constexpr int WGB = 1024;  // single-workgroup sync loop
 

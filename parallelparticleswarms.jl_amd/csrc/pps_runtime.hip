@@ -301,8 +301,8 @@ static int swarm_prepare_launch(pps_swarm* s) {
     s->S.G = 1;
     s->S.block = 1024;
   } else {
-    s->S.block = 256;
-    const int64_t tiles = (n + 255) / 256;
+    s->S.block = PPS_BLK;
+    const int64_t tiles = (n + PPS_BLK - 1) / PPS_BLK;
     int per_cu = PPS_SYNC_GRID_PER_CU;
     if (mode == PPS_MODE_ASYNC) {
       per_cu = 8;
