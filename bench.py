@@ -202,6 +202,22 @@ def ode_c3(P):
         sw.free()
         out[key] = {"device_ms": ms, "particle_iters_per_s": (1 << 14) * 300 / (ms / 1e3),
                     "theta": [float(v) for v in sol.u], "loss_f64": sol.objective}
+    # the reference's own Lotka-Volterra benchmark shape (10000 particles, 100
+    # iterations, rates in [0, 10], GPUTsit5 dt = 0.1 with saveat every 0.01
+    # on [0, 3], noise-free data)
+    doc = _json.load(open(os.path.join(ROOT, "tests", "golden", "lv_obs_benchmark.json")))
+    obs = np.array(doc["obs"], np.float32)
+    P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 10.0), 10000, 5, seed=1, ode_alg="tsit5",
+                          dt=0.1, saveat=0.01)  # warm-up
+    sol, sw = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 10.0), 10000, 100, seed=1,
+                                    ode_alg="tsit5", dt=0.1, saveat=0.01)
+    ms = sw.elapsed_ms()
+    sw.free()
+    out["reference_benchmark_shape"] = {
+        "device_ms": ms, "particle_iters_per_s": 10000 * 100 / (ms / 1e3),
+        "theta": [float(v) for v in sol.u], "loss_f64": sol.objective,
+        "max_rel_err_theta": float(np.max(np.abs(np.asarray(sol.u) / np.asarray(doc["theta"]) - 1))),
+        "workload": "10000 particles x 100 iterations, Tsit5 dt 0.1, 301 samples every 0.01"}
     return out
 
 

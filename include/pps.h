@@ -69,7 +69,13 @@ enum {
    * in the kernel by one of three methods. */
   PPS_OBJ_LOTKA_VOLTERRA = 16,       /* fixed-step RK4                         */
   PPS_OBJ_LOTKA_VOLTERRA_TSIT5 = 17, /* fixed-step Tsit5 (GPUTsit5, Val(false)) */
-  PPS_OBJ_LOTKA_VOLTERRA_ATSIT5 = 18 /* adaptive Tsit5 (GPUTsit5, Val(true))    */
+  PPS_OBJ_LOTKA_VOLTERRA_ATSIT5 = 18, /* adaptive Tsit5 (GPUTsit5, Val(true))   */
+  /* Tsit5 with saveat semantics: steps are not tied to the observation times;
+   * each observation inside a step is read from Tsit5's continuous extension
+   * (how GPUTsit5 fills `saveat`, e.g. dt = 0.1 with saveat every 0.01 in
+   * benchmarks/Lotka_Volterra/lotka_volterra.jl:124-127). */
+  PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT = 19,  /* fixed dt                     */
+  PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT = 20  /* adaptive                     */
 };
 
 /* Swarm update semantics. */
@@ -124,7 +130,9 @@ typedef struct pps_problem {
    * sampled every (int)params[3] steps.  Adaptive Tsit5: t_k = k * params[2],
    * initial step params[3], abstol params[4], reltol params[5], optional step
    * limit params[6] (default 10000; beyond it the loss is +Inf; controller
-   * documented at lv_sse_tsit5 in the device sources). */
+   * documented at lv_sse_tsit5 in the device sources).  The _SAVEAT forms:
+   * t_k = k * params[2], step (initial step when adaptive) params[3], and
+   * for the adaptive one params[4..6] as above; the last step ends on t_{n_obs-1}. */
   const float* obs;
   int32_t n_obs;
   /* Inequality constraints g_k(x) <= 0, k = 0..n_cons-1 (n_cons <= PPS_MAXCONS),

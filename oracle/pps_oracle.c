@@ -39,7 +39,8 @@ enum {
   ORA_ROSENBROCK = 1, ORA_RASTRIGIN = 2, ORA_SPHERE = 3, ORA_BEALE = 4,
   ORA_BOOTH = 5, ORA_HIMMELBLAU = 6, ORA_QUADRATIC = 7,
   ORA_BOUNDARY_QUADRATIC = 8, ORA_ROSEN2 = 9, ORA_LOTKA_VOLTERRA = 16,
-  ORA_LOTKA_VOLTERRA_TSIT5 = 17, ORA_LOTKA_VOLTERRA_ATSIT5 = 18
+  ORA_LOTKA_VOLTERRA_TSIT5 = 17, ORA_LOTKA_VOLTERRA_ATSIT5 = 18,
+  ORA_LOTKA_VOLTERRA_TSIT5_SAVEAT = 19, ORA_LOTKA_VOLTERRA_ATSIT5_SAVEAT = 20
 };
 enum { ORA_GBEST_PBEST = 0, ORA_GBEST_CURRENT = 1 };
 enum { ORA_CLAMP_COMPONENT = 0, ORA_CLAMP_LEXICOGRAPHIC = 1 };
@@ -165,10 +166,30 @@ static const float TS_E[7] = {-0.00178001105222577714f, -0.0008164344596567469f,
                               0.5823571654525552f,      -0.45808210592918697f,
                               0.015151515151515152f};
 
+/* Continuous extension (Tsitouras 2011, Sec. 4, order 4): b_i(theta) =
+ * r_i0 theta + r_i1 theta^2 + r_i2 theta^3 + r_i3 theta^4, b_i(1) = b_i. */
+static const float TS_R[7][4] = {
+    {1.0f, -2.763706197274826f, 2.9132554618219126f, -1.0530884977290216f},
+    {0.0f, 0.13169999999999998f, -0.2234f, 0.1017f},
+    {0.0f, 3.9302962368947516f, -5.941033872131505f, 2.490627285651253f},
+    {0.0f, -12.411077166933676f, 30.33818863028232f, -16.548102889244902f},
+    {0.0f, 37.50931341651104f, -88.1789048947664f, 47.37952196281928f},
+    {0.0f, -27.896526289197286f, 65.09189467479366f, -34.87065786149661f},
+    {0.0f, 1.5f, -4.0f, 2.5f}};
+
 /* The tableau as used (for the order-condition checks). */
 void ora_tsit5_tableau(float* a /* 7 x 6 */, float* e /* 7 */) {
   memcpy(a, TS_A, sizeof TS_A);
   memcpy(e, TS_E, sizeof TS_E);
+}
+
+/* The continuous extension's weights b_i(theta) as evaluated:
+ * b_0 = theta (r_00 + theta (r_01 + theta (r_02 + theta r_03))),
+ * b_i = theta (theta (r_i1 + theta (r_i2 + theta r_i3))) for i >= 1. */
+void ora_tsit5_interp_weights(float theta, float* b /* 7 */) {
+  b[0] = theta * (TS_R[0][0] + theta * (TS_R[0][1] + theta * (TS_R[0][2] + theta * TS_R[0][3])));
+  for (int i = 1; i < 7; ++i)
+    b[i] = theta * (theta * (TS_R[i][1] + theta * (TS_R[i][2] + theta * TS_R[i][3])));
 }
 
 /* One Tsit5 step from u with k[0] = f(u): y_i = u + h * (a_i0 k_0 + ... ),
@@ -275,6 +296,71 @@ float ora_lv_sse_tsit5(const float* th, const ora_problem* P, int adaptive) {
   return loss;
 }
 
+/* Lotka-Volterra SSE with Tsit5 and saveat semantics (GPUTsit5 filling
+ * `saveat` between steps, e.g. benchmarks/Lotka_Volterra/lotka_volterra.jl:
+ * 124-127 with dt = 0.1 and saves every 0.01): observation k at t_k = k *
+ * par[2] (FP32 product); steps of par[3] (initial step when adaptive) from
+ * t = 0, each shortened to end on t_end = t_{n_obs-1} when t + dt does not
+ * fall short of it; after an accepted step [t, t + h] every unsaved t_k <=
+ * t + h gets u + h * (b_0(theta) k_0 + ... + b_6(theta) k_6), sum left to
+ * right, theta = (t_k - t) / h.  Adaptive: the controller of
+ * ora_lv_sse_tsit5 with next step h / q after every attempt; step limit par[6]
+ * (10000 when <= 0); +Inf on too many steps or a step that does not advance
+ * t.  Fixed steps have no limit (the host bounds their number). */
+float ora_lv_sse_tsit5_saveat(const float* th, const ora_problem* P, int adaptive) {
+  float u[2] = {P->par[0], P->par[1]}, k[7][2];
+  lv_rhs(u[0], u[1], th, &k[0][0], &k[0][1]);
+  const float tau = P->par[2];
+  const float t_end = (float)(P->n_obs - 1) * tau;
+  float e1 = P->obs[0] - u[0], e2 = P->obs[1] - u[1];
+  float loss = e1 * e1 + e2 * e2, t = 0.0f, dt = P->par[3];
+  int32_t steps = 0;
+  const int32_t max_steps =
+      adaptive ? (P->par[6] > 0.0f ? (int32_t)P->par[6] : 10000) : 0x7fffffff;
+  int kk = 1;
+  while (kk < P->n_obs) {
+    const int last = !(t + dt < t_end);
+    const float h = last ? t_end - t : dt;
+    if (++steps > max_steps || !(t + h > t)) return INFINITY;
+    float un[2], err[2];
+    lv_tsit5_step(u, k, h, th, un, err);
+    if (adaptive) {
+      float s[2];
+      for (int c = 0; c < 2; ++c) {
+        float a = fabsf(u[c]), b = fabsf(un[c]);
+        s[c] = err[c] / (P->par[4] + (a > b ? a : b) * P->par[5]);
+      }
+      float eest = sqrtf((s[0] * s[0] + s[1] * s[1]) * 0.5f);
+      float q = 5.0f;
+      if (eest <= 3.0e38f) {
+        q = ora_root5(eest) / 0.9f;
+        q = q < 0.1f ? 0.1f : (q > 5.0f ? 5.0f : q);
+      }
+      dt = h / q;
+      if (!(eest <= 1.0f)) continue;
+    }
+    const float t_new = last ? t_end : t + h;
+    for (; kk < P->n_obs; ++kk) {
+      const float tk = (float)kk * tau;
+      if (tk > t_new) break;
+      float b[7], v[2];
+      ora_tsit5_interp_weights((tk - t) / h, b);
+      for (int c = 0; c < 2; ++c) {
+        float acc = b[0] * k[0][c];
+        for (int j = 1; j < 7; ++j) acc = acc + b[j] * k[j][c];
+        v[c] = u[c] + h * acc;
+      }
+      e1 = P->obs[2 * kk] - v[0];
+      e2 = P->obs[2 * kk + 1] - v[1];
+      loss = loss + (e1 * e1 + e2 * e2);
+    }
+    t = t_new;
+    u[0] = un[0], u[1] = un[1];
+    k[0][0] = k[6][0], k[0][1] = k[6][1];
+  }
+  return loss;
+}
+
 float ora_lv_sse(const float* th, const ora_problem* P) {
   float u1 = P->par[0], u2 = P->par[1];
   float dt = P->par[2];
@@ -359,6 +445,10 @@ float ora_objective(const ora_problem* P, const float* xin) {
       return ora_lv_sse_tsit5(z, P, 0);
     case ORA_LOTKA_VOLTERRA_ATSIT5:
       return ora_lv_sse_tsit5(z, P, 1);
+    case ORA_LOTKA_VOLTERRA_TSIT5_SAVEAT:
+      return ora_lv_sse_tsit5_saveat(z, P, 0);
+    case ORA_LOTKA_VOLTERRA_ATSIT5_SAVEAT:
+      return ora_lv_sse_tsit5_saveat(z, P, 1);
   }
   return NAN;
 }

@@ -27,6 +27,7 @@ ROSENBROCK, RASTRIGIN, SPHERE, BEALE, BOOTH, HIMMELBLAU, QUADRATIC, BOUNDARY_QUA
     1, 2, 3, 4, 5, 6, 7, 8, 9)
 LOTKA_VOLTERRA = 16
 LOTKA_VOLTERRA_TSIT5, LOTKA_VOLTERRA_ATSIT5 = 17, 18
+LOTKA_VOLTERRA_TSIT5_SAVEAT, LOTKA_VOLTERRA_ATSIT5_SAVEAT = 19, 20
 GBEST_PBEST, GBEST_CURRENT = 0, 1
 SAMPLER_UNIFORM, SAMPLER_UNBOUNDED, SAMPLER_LATIN_HYPERCUBE = 0, 1, 2
 CLAMP_COMPONENT, CLAMP_LEXICOGRAPHIC = 0, 1
@@ -82,6 +83,7 @@ def lib():
         L.ora_root5.argtypes = [C.c_float]
         L.ora_root5.restype = C.c_float
         L.ora_tsit5_tableau.argtypes = [fp, fp]
+        L.ora_tsit5_interp_weights.argtypes = [C.c_float, fp]
         L.ora_lv_sse.restype = C.c_float
         L.ora_eval.argtypes = [C.POINTER(_Prob), C.c_int64, fp, fp]
         L.ora_argmin.argtypes = [fp, C.c_int64, C.c_int]
@@ -185,6 +187,13 @@ def tsit5_tableau():
     e = np.zeros(7, np.float32)
     lib().ora_tsit5_tableau(_fp(a), _fp(e))
     return a, e
+
+
+def tsit5_interp_weights(theta: float) -> np.ndarray:
+    """b_i(theta) of Tsit5's continuous extension as the oracle evaluates them."""
+    b = np.zeros(7, np.float32)
+    lib().ora_tsit5_interp_weights(float(np.float32(theta)), _fp(b))
+    return b
 
 
 def lhs_permute(i: int, n: int, j: int, epoch: int, seed: int) -> int:

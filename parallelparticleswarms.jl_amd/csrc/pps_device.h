@@ -12,10 +12,6 @@
 
 #include "pps_shared.h"
 
-constexpr bool is_lotka_volterra(int obj) {
-  return obj == PPS_OBJ_LOTKA_VOLTERRA || obj == PPS_OBJ_LOTKA_VOLTERRA_TSIT5 ||
-         obj == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5;
-}
 
 namespace pps {
 
@@ -222,38 +218,97 @@ __device__ __forceinline__ float lv_sse(const float* th, const ProbArgs& P) {
 #define TS_E6 -0.45808210592918697f
 #define TS_E7 0.015151515151515152f
 
-// One Tsit5 step of the Lotka-Volterra system from (u1, u2) with k1 = f(u):
-// stage sums left to right, y_i = u + h * sum_j a_ij k_j; returns the new
-// state, k7 = f(new state) (the next step's k1) and, when ERR, the error
-// estimate h * sum_j btilde_j k_j.
+// One Tsit5 step of the Lotka-Volterra system from (u1, u2) with k1 = f(u) in
+// ka[0], kb[0]: stage sums left to right, y_i = u + h * sum_j a_ij k_j; fills
+// ka/kb[1..6] with k2..k7 (k7 = f(new state), the next step's k1) and returns
+// the new state and, when ERR, the error estimate h * sum_j btilde_j k_j.
+template <bool ERR>
+__device__ __forceinline__ void lv_tsit5_stages(float u1, float u2, float h, const float* th,
+                                                float (&ka)[7], float (&kb)[7], float& n1,
+                                                float& n2, float& e1, float& e2) {
+  const float p1 = th[0], p2 = th[1], p3 = th[2], p4 = th[3];
+  lv_rhs(u1 + h * (TS_A21 * ka[0]), u2 + h * (TS_A21 * kb[0]), p1, p2, p3, p4, ka[1], kb[1]);
+  lv_rhs(u1 + h * (TS_A31 * ka[0] + TS_A32 * ka[1]), u2 + h * (TS_A31 * kb[0] + TS_A32 * kb[1]),
+         p1, p2, p3, p4, ka[2], kb[2]);
+  lv_rhs(u1 + h * ((TS_A41 * ka[0] + TS_A42 * ka[1]) + TS_A43 * ka[2]),
+         u2 + h * ((TS_A41 * kb[0] + TS_A42 * kb[1]) + TS_A43 * kb[2]), p1, p2, p3, p4, ka[3],
+         kb[3]);
+  lv_rhs(u1 + h * (((TS_A51 * ka[0] + TS_A52 * ka[1]) + TS_A53 * ka[2]) + TS_A54 * ka[3]),
+         u2 + h * (((TS_A51 * kb[0] + TS_A52 * kb[1]) + TS_A53 * kb[2]) + TS_A54 * kb[3]), p1, p2,
+         p3, p4, ka[4], kb[4]);
+  lv_rhs(u1 + h * ((((TS_A61 * ka[0] + TS_A62 * ka[1]) + TS_A63 * ka[2]) + TS_A64 * ka[3]) +
+                   TS_A65 * ka[4]),
+         u2 + h * ((((TS_A61 * kb[0] + TS_A62 * kb[1]) + TS_A63 * kb[2]) + TS_A64 * kb[3]) +
+                   TS_A65 * kb[4]),
+         p1, p2, p3, p4, ka[5], kb[5]);
+  n1 = u1 + h * (((((TS_B1 * ka[0] + TS_B2 * ka[1]) + TS_B3 * ka[2]) + TS_B4 * ka[3]) +
+                  TS_B5 * ka[4]) + TS_B6 * ka[5]);
+  n2 = u2 + h * (((((TS_B1 * kb[0] + TS_B2 * kb[1]) + TS_B3 * kb[2]) + TS_B4 * kb[3]) +
+                  TS_B5 * kb[4]) + TS_B6 * kb[5]);
+  lv_rhs(n1, n2, p1, p2, p3, p4, ka[6], kb[6]);
+  if (ERR) {
+    e1 = h * ((((((TS_E1 * ka[0] + TS_E2 * ka[1]) + TS_E3 * ka[2]) + TS_E4 * ka[3]) +
+                TS_E5 * ka[4]) + TS_E6 * ka[5]) + TS_E7 * ka[6]);
+    e2 = h * ((((((TS_E1 * kb[0] + TS_E2 * kb[1]) + TS_E3 * kb[2]) + TS_E4 * kb[3]) +
+                TS_E5 * kb[4]) + TS_E6 * kb[5]) + TS_E7 * kb[6]);
+  }
+}
+
+// The same step with k1 passed and k7 returned as scalars.
 template <bool ERR>
 __device__ __forceinline__ void lv_tsit5_step(float u1, float u2, float k1a, float k1b, float h,
                                               const float* th, float& n1, float& n2, float& k7a,
                                               float& k7b, float& e1, float& e2) {
-  const float p1 = th[0], p2 = th[1], p3 = th[2], p4 = th[3];
-  float k2a, k2b, k3a, k3b, k4a, k4b, k5a, k5b, k6a, k6b;
-  lv_rhs(u1 + h * (TS_A21 * k1a), u2 + h * (TS_A21 * k1b), p1, p2, p3, p4, k2a, k2b);
-  lv_rhs(u1 + h * (TS_A31 * k1a + TS_A32 * k2a), u2 + h * (TS_A31 * k1b + TS_A32 * k2b), p1, p2,
-         p3, p4, k3a, k3b);
-  lv_rhs(u1 + h * ((TS_A41 * k1a + TS_A42 * k2a) + TS_A43 * k3a),
-         u2 + h * ((TS_A41 * k1b + TS_A42 * k2b) + TS_A43 * k3b), p1, p2, p3, p4, k4a, k4b);
-  lv_rhs(u1 + h * (((TS_A51 * k1a + TS_A52 * k2a) + TS_A53 * k3a) + TS_A54 * k4a),
-         u2 + h * (((TS_A51 * k1b + TS_A52 * k2b) + TS_A53 * k3b) + TS_A54 * k4b), p1, p2, p3, p4,
-         k5a, k5b);
-  lv_rhs(u1 + h * ((((TS_A61 * k1a + TS_A62 * k2a) + TS_A63 * k3a) + TS_A64 * k4a) + TS_A65 * k5a),
-         u2 + h * ((((TS_A61 * k1b + TS_A62 * k2b) + TS_A63 * k3b) + TS_A64 * k4b) + TS_A65 * k5b),
-         p1, p2, p3, p4, k6a, k6b);
-  n1 = u1 + h * (((((TS_B1 * k1a + TS_B2 * k2a) + TS_B3 * k3a) + TS_B4 * k4a) + TS_B5 * k5a) +
-                 TS_B6 * k6a);
-  n2 = u2 + h * (((((TS_B1 * k1b + TS_B2 * k2b) + TS_B3 * k3b) + TS_B4 * k4b) + TS_B5 * k5b) +
-                 TS_B6 * k6b);
-  lv_rhs(n1, n2, p1, p2, p3, p4, k7a, k7b);
-  if (ERR) {
-    e1 = h * ((((((TS_E1 * k1a + TS_E2 * k2a) + TS_E3 * k3a) + TS_E4 * k4a) + TS_E5 * k5a) +
-               TS_E6 * k6a) + TS_E7 * k7a);
-    e2 = h * ((((((TS_E1 * k1b + TS_E2 * k2b) + TS_E3 * k3b) + TS_E4 * k4b) + TS_E5 * k5b) +
-               TS_E6 * k6b) + TS_E7 * k7b);
-  }
+  float ka[7], kb[7];
+  ka[0] = k1a;
+  kb[0] = k1b;
+  lv_tsit5_stages<ERR>(u1, u2, h, th, ka, kb, n1, n2, e1, e2);
+  k7a = ka[6];
+  k7b = kb[6];
+}
+
+// Tsit5's continuous extension (Tsitouras 2011, Sec. 4; 4th order):
+// u(t + theta h) = u + h * sum_i b_i(theta) k_i with
+// b_1(theta) = theta (r_11 + theta (r_12 + theta (r_13 + theta r_14))) and
+// b_i(theta) = theta (theta (r_i2 + theta (r_i3 + theta r_i4))) for i >= 2;
+// b_i(1) = b_i.  Sum over i left to right.
+#define TS_R11 1.0f
+#define TS_R12 -2.763706197274826f
+#define TS_R13 2.9132554618219126f
+#define TS_R14 -1.0530884977290216f
+#define TS_R22 0.13169999999999998f
+#define TS_R23 -0.2234f
+#define TS_R24 0.1017f
+#define TS_R32 3.9302962368947516f
+#define TS_R33 -5.941033872131505f
+#define TS_R34 2.490627285651253f
+#define TS_R42 -12.411077166933676f
+#define TS_R43 30.33818863028232f
+#define TS_R44 -16.548102889244902f
+#define TS_R52 37.50931341651104f
+#define TS_R53 -88.1789048947664f
+#define TS_R54 47.37952196281928f
+#define TS_R62 -27.896526289197286f
+#define TS_R63 65.09189467479366f
+#define TS_R64 -34.87065786149661f
+#define TS_R72 1.5f
+#define TS_R73 -4.0f
+#define TS_R74 2.5f
+
+__device__ __forceinline__ void tsit5_interp(float u1, float u2, float h, float tt,
+                                             const float (&ka)[7], const float (&kb)[7],
+                                             float& v1, float& v2) {
+  const float b1 = tt * (TS_R11 + tt * (TS_R12 + tt * (TS_R13 + tt * TS_R14)));
+  const float b2 = tt * (tt * (TS_R22 + tt * (TS_R23 + tt * TS_R24)));
+  const float b3 = tt * (tt * (TS_R32 + tt * (TS_R33 + tt * TS_R34)));
+  const float b4 = tt * (tt * (TS_R42 + tt * (TS_R43 + tt * TS_R44)));
+  const float b5 = tt * (tt * (TS_R52 + tt * (TS_R53 + tt * TS_R54)));
+  const float b6 = tt * (tt * (TS_R62 + tt * (TS_R63 + tt * TS_R64)));
+  const float b7 = tt * (tt * (TS_R72 + tt * (TS_R73 + tt * TS_R74)));
+  v1 = u1 + h * ((((((b1 * ka[0] + b2 * ka[1]) + b3 * ka[2]) + b4 * ka[3]) + b5 * ka[4]) +
+                  b6 * ka[5]) + b7 * ka[6]);
+  v2 = u2 + h * ((((((b1 * kb[0] + b2 * kb[1]) + b3 * kb[2]) + b4 * kb[3]) + b5 * kb[4]) +
+                  b6 * kb[5]) + b7 * kb[6]);
 }
 
 // x^(1/5) for x > 0 from basic IEEE operations only (an exponent-halving seed
@@ -349,6 +404,67 @@ __device__ __forceinline__ float lv_sse_tsit5(const float* th, const ProbArgs& P
   return loss;
 }
 
+// Lotka-Volterra SSE with Tsit5 and saveat semantics: observation k at
+// t_k = k * par[2] (FP32 product), steps of par[3] (the initial step when
+// ADAPTIVE) from t = 0, the last step shortened to end on t_end = t_{n_obs-1};
+// after each accepted step [t, t + h] every unsaved t_k <= t + h is read from
+// the continuous extension at theta = (t_k - t) / h.  Adaptive control, step
+// limit par[6] and the +Inf results as lv_sse_tsit5; fixed steps carry no
+// limit (the host bounds their number).  Loss as lv_sse.
+template <bool ADAPTIVE>
+__device__ __forceinline__ float lv_sse_tsit5_saveat(const float* th, const ProbArgs& P) {
+  float u1 = P.par[0], u2 = P.par[1];
+  float ka[7], kb[7];
+  lv_rhs(u1, u2, th[0], th[1], th[2], th[3], ka[0], kb[0]);
+  const float* obs = P.obs;
+  const float tau = P.par[2];
+  const float t_end = (float)(P.n_obs - 1) * tau;
+  const float o1 = obs[0] - u1, o2 = obs[1] - u2;
+  float loss = o1 * o1 + o2 * o2;
+  int k = 1;
+  float t = 0.0f, dt = P.par[3];
+  int32_t steps = 0;
+  const int32_t max_steps =
+      ADAPTIVE ? (P.par[6] > 0.0f ? (int32_t)P.par[6] : LV_MAX_STEPS) : 0x7fffffff;
+  while (k < P.n_obs) {
+    const bool last = !(t + dt < t_end);
+    const float h = last ? t_end - t : dt;
+    if (++steps > max_steps || !(t + h > t)) return __builtin_inff();
+    float n1, n2, e1, e2;
+    lv_tsit5_stages<ADAPTIVE>(u1, u2, h, th, ka, kb, n1, n2, e1, e2);
+    if (ADAPTIVE) {
+      const float a1 = __builtin_fabsf(u1), b1 = __builtin_fabsf(n1);
+      const float a2 = __builtin_fabsf(u2), b2 = __builtin_fabsf(n2);
+      const float s1 = e1 / (P.par[4] + (a1 > b1 ? a1 : b1) * P.par[5]);
+      const float s2 = e2 / (P.par[4] + (a2 > b2 ? a2 : b2) * P.par[5]);
+      const float eest = sqrtf((s1 * s1 + s2 * s2) * 0.5f);
+      float q = 5.0f;
+      if (eest <= 3.0e38f) {
+        q = root5(eest) / 0.9f;
+        q = q < 0.1f ? 0.1f : (q > 5.0f ? 5.0f : q);
+      }
+      dt = h / q;
+      if (!(eest <= 1.0f)) continue;
+    }
+    const float t_new = last ? t_end : t + h;
+    while (k < P.n_obs) {
+      const float tk = (float)k * tau;
+      if (tk > t_new) break;
+      float v1, v2;
+      tsit5_interp(u1, u2, h, (tk - t) / h, ka, kb, v1, v2);
+      const float d1 = obs[2 * k] - v1, d2 = obs[2 * k + 1] - v2;
+      loss = loss + (d1 * d1 + d2 * d2);
+      ++k;
+    }
+    t = t_new;
+    u1 = n1;
+    u2 = n2;
+    ka[0] = ka[6];
+    kb[0] = kb[6];
+  }
+  return loss;
+}
+
 // Objective f(x - shift) over a register array (compile-time capacity MAXD,
 // runtime dimension d <= MAXD).  OBJ is a compile-time PPS_OBJ_* constant.
 template <int MAXD, int OBJ>
@@ -419,6 +535,10 @@ __device__ __forceinline__ float objective(const float (&xin)[MAXD], const int d
     return lv_sse_tsit5<false>(z, P);
   } else if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5) {
     return lv_sse_tsit5<true>(z, P);
+  } else if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT) {
+    return lv_sse_tsit5_saveat<false>(z, P);
+  } else if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT) {
+    return lv_sse_tsit5_saveat<true>(z, P);
   } else {
     return __uint_as_float(0x7FC00000u);
   }

@@ -7,7 +7,7 @@
 #include <cstdint>
 #include <vector>
 
-#include "pps.h"
+#include "pps_shared.h"
 
 namespace {
 
@@ -43,24 +43,34 @@ double lv_sse_f64(const double* th, const pps_problem* P) {
   return loss;
 }
 
+// Tsitouras 5(4) tableau (Comput. Math. Appl. 62 (2011) 770-775): stage
+// weights (row 6 = solution weights), error weights b - bhat, and the
+// continuous extension's coefficients b_i(theta) = sum_j R[i][j] theta^(j+1).
+const double TSA[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {0.161, 0, 0, 0, 0, 0},
+    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401,
+     -0.028269050394068383, 0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081,
+     2.324710524099774}};
+const double TSE[7] = {-0.00178001105222577714, -0.0008164344596567469, 0.007880878010261995,
+                       -0.1447110071732629,     0.5823571654525552,      -0.45808210592918697,
+                       0.015151515151515152};
+const double TSR[7][4] = {{1.0, -2.763706197274826, 2.9132554618219126, -1.0530884977290216},
+                          {0.0, 0.13169999999999998, -0.2234, 0.1017},
+                          {0.0, 3.9302962368947516, -5.941033872131505, 2.490627285651253},
+                          {0.0, -12.411077166933676, 30.33818863028232, -16.548102889244902},
+                          {0.0, 37.50931341651104, -88.1789048947664, 47.37952196281928},
+                          {0.0, -27.896526289197286, 65.09189467479366, -34.87065786149661},
+                          {0.0, 1.5, -4.0, 2.5}};
+
 // Tsitouras 5(4) in FP64, fixed or adaptive (same controller as the device's
 // lv_sse_tsit5: RMS scaled error, factor EEst^(1/5)/0.9 in [0.1, 5], steps
 // ending on the observation times).
 double lv_sse_tsit5_f64(const double* th, const pps_problem* P, bool adaptive) {
-  static const double A[7][6] = {
-      {0, 0, 0, 0, 0, 0},
-      {0.161, 0, 0, 0, 0, 0},
-      {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
-      {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
-      {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
-      {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401,
-       -0.028269050394068383, 0},
-      {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081,
-       2.324710524099774}};
-  static const double E[7] = {-0.00178001105222577714, -0.0008164344596567469,
-                              0.007880878010261995,    -0.1447110071732629,
-                              0.5823571654525552,      -0.45808210592918697,
-                              0.015151515151515152};
   auto rhs = [&](double a, double b, double& f1, double& f2) {
     f1 = th[0] * a - th[1] * a * b;
     f2 = -th[2] * b + th[3] * a * b;
@@ -72,7 +82,7 @@ double lv_sse_tsit5_f64(const double* th, const pps_problem* P, bool adaptive) {
       double y[2];
       for (int c = 0; c < 2; ++c) {
         double acc = 0.0;
-        for (int j = 0; j < i; ++j) acc += A[i][j] * k[j][c];
+        for (int j = 0; j < i; ++j) acc += TSA[i][j] * k[j][c];
         y[c] = u[c] + h * acc;
       }
       if (i == 6) un[0] = y[0], un[1] = y[1];
@@ -80,7 +90,7 @@ double lv_sse_tsit5_f64(const double* th, const pps_problem* P, bool adaptive) {
     }
     for (int c = 0; c < 2; ++c) {
       double acc = 0.0;
-      for (int j = 0; j < 7; ++j) acc += E[j] * k[j][c];
+      for (int j = 0; j < 7; ++j) acc += TSE[j] * k[j][c];
       err[c] = h * acc;
     }
   };
@@ -121,6 +131,71 @@ double lv_sse_tsit5_f64(const double* th, const pps_problem* P, bool adaptive) {
     }
     const double e1 = (double)P->obs[2 * kk] - u[0], e2 = (double)P->obs[2 * kk + 1] - u[1];
     loss += e1 * e1 + e2 * e2;
+  }
+  return loss;
+}
+
+// Tsit5 in FP64 with saveat semantics (as the device's lv_sse_tsit5_saveat):
+// free steps from t = 0, the last ending on t_end = (n_obs - 1) * params[2],
+// observations inside a step read from the continuous extension.
+double lv_sse_tsit5_saveat_f64(const double* th, const pps_problem* P, bool adaptive) {
+  auto rhs = [&](double a, double b, double& f1, double& f2) {
+    f1 = th[0] * a - th[1] * a * b;
+    f2 = -th[2] * b + th[3] * a * b;
+  };
+  double u[2] = {P->params[0], P->params[1]}, k[7][2];
+  rhs(u[0], u[1], k[0][0], k[0][1]);
+  const double tau = P->params[2], t_end = (P->n_obs - 1) * tau;
+  double e1 = (double)P->obs[0] - u[0], e2 = (double)P->obs[1] - u[1];
+  double loss = e1 * e1 + e2 * e2, t = 0.0, dt = P->params[3];
+  const long max_steps = !adaptive ? (1L << 40)
+                         : P->n_params > 6 && P->params[6] > 0 ? (long)P->params[6] : 10000;
+  long steps = 0;
+  int kk = 1;
+  while (kk < P->n_obs) {
+    const bool last = !(t + dt < t_end);
+    const double h = last ? t_end - t : dt;
+    if (++steps > max_steps || !(t + h > t)) return INFINITY;
+    double un[2], err[2];
+    for (int i = 1; i < 7; ++i) {
+      double y[2];
+      for (int c = 0; c < 2; ++c) {
+        double acc = 0.0;
+        for (int j = 0; j < i; ++j) acc += TSA[i][j] * k[j][c];
+        y[c] = u[c] + h * acc;
+      }
+      if (i == 6) un[0] = y[0], un[1] = y[1];
+      rhs(y[0], y[1], k[i][0], k[i][1]);
+    }
+    if (adaptive) {
+      for (int c = 0; c < 2; ++c) {
+        double acc = 0.0;
+        for (int j = 0; j < 7; ++j) acc += TSE[j] * k[j][c];
+        err[c] = h * acc / (P->params[4] + std::max(std::fabs(u[c]), std::fabs(un[c])) * P->params[5]);
+      }
+      const double eest = std::sqrt((err[0] * err[0] + err[1] * err[1]) / 2.0);
+      double q = 5.0;
+      if (std::isfinite(eest)) q = std::min(5.0, std::max(0.1, std::pow(eest, 0.2) / 0.9));
+      dt = h / q;
+      if (!(eest <= 1.0)) continue;
+    }
+    const double t_new = last ? t_end : t + h;
+    for (; kk < P->n_obs && kk * tau <= t_new; ++kk) {
+      const double th_ = (kk * tau - t) / h;
+      double v[2] = {u[0], u[1]};
+      for (int i = 0; i < 7; ++i) {
+        double b = 0.0;
+        for (int j = 3; j >= 0; --j) b = (b + TSR[i][j]) * th_;
+        v[0] += h * b * k[i][0];
+        v[1] += h * b * k[i][1];
+      }
+      e1 = (double)P->obs[2 * kk] - v[0];
+      e2 = (double)P->obs[2 * kk + 1] - v[1];
+      loss += e1 * e1 + e2 * e2;
+    }
+    t = t_new;
+    u[0] = un[0], u[1] = un[1];
+    k[0][0] = k[6][0], k[0][1] = k[6][1];
   }
   return loss;
 }
@@ -181,6 +256,10 @@ double eval_one(const pps_problem* P, const double* xin) {
       return lv_sse_tsit5_f64(z.data(), P, false);
     case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5:
       return lv_sse_tsit5_f64(z.data(), P, true);
+    case PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT:
+      return lv_sse_tsit5_saveat_f64(z.data(), P, false);
+    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT:
+      return lv_sse_tsit5_saveat_f64(z.data(), P, true);
   }
   return NAN;
 }
@@ -189,10 +268,11 @@ double eval_one(const pps_problem* P, const double* xin) {
 
 extern "C" int pps_eval_f64(const pps_problem* prob, int64_t n, const double* x, double* f) {
   if (!prob || (n > 0 && (!x || !f)) || n < 0 || prob->d < 1 || prob->d > 64) return PPS_E_INVALID;
-  const bool lv = prob->objective == PPS_OBJ_LOTKA_VOLTERRA ||
-                  prob->objective == PPS_OBJ_LOTKA_VOLTERRA_TSIT5 ||
-                  prob->objective == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5;
-  const int need = prob->objective == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5 ? 6 : 4;
+  const bool lv = is_lotka_volterra(prob->objective);
+  const int need = prob->objective == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5 ||
+                           prob->objective == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT
+                       ? 6
+                       : 4;
   if (lv && (prob->d != 4 || !prob->obs || prob->n_obs < 1 || !prob->params ||
              prob->n_params < need))
     return PPS_E_INVALID;

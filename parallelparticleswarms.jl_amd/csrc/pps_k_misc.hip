@@ -3,6 +3,8 @@
 #include "pps_kernels_impl.h"
 
 namespace pps {
+const KernelSet* kernels_lotka_volterra_saveat(int obj);
+
 const KernelSet* kernels_two_dim(int obj, int d) {
   switch (obj) {
     case PPS_OBJ_BEALE: return d == 2 ? Launch<2, true, PPS_OBJ_BEALE>::set() : nullptr;
@@ -22,7 +24,7 @@ const KernelSet* kernels_lotka_volterra(int obj, int d) {
     case PPS_OBJ_LOTKA_VOLTERRA_TSIT5: return Launch<4, true, PPS_OBJ_LOTKA_VOLTERRA_TSIT5>::set();
     case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5:
       return Launch<4, true, PPS_OBJ_LOTKA_VOLTERRA_ATSIT5>::set();
-    default: return nullptr;
+    default: return kernels_lotka_volterra_saveat(obj);
   }
 }
 }  // namespace pps

@@ -100,6 +100,8 @@ const KernelSet* find_kernels(int obj, int d, bool constrained) {
     case PPS_OBJ_LOTKA_VOLTERRA:
     case PPS_OBJ_LOTKA_VOLTERRA_TSIT5:
     case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5:
+    case PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT:
+    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT:
       return constrained ? nullptr : kernels_lotka_volterra(obj, d);
     default: return constrained ? nullptr : kernels_two_dim(obj, d);
   }

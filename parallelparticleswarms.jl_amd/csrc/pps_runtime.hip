@@ -84,30 +84,39 @@ int fill_prob_args(const pps_problem* prob, ProbArgs* P) {
     case PPS_OBJ_QUADRATIC: need_par = 1; break;
     case PPS_OBJ_LOTKA_VOLTERRA:
     case PPS_OBJ_LOTKA_VOLTERRA_TSIT5: need_par = 4; break;
-    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5: need_par = 6; break;
+    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5:
+    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT: need_par = 6; break;
+    case PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT: need_par = 4; break;
     default: break;
   }
   if (prob->n_params < need_par)
     return fail(PPS_E_INVALID, "objective %d needs %d parameters, got %d", prob->objective,
                 need_par, prob->n_params);
-  const bool lv = prob->objective == PPS_OBJ_LOTKA_VOLTERRA ||
-                  prob->objective == PPS_OBJ_LOTKA_VOLTERRA_TSIT5 ||
-                  prob->objective == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5;
-  if (lv) {
+  if (is_lotka_volterra(prob->objective)) {
     if (d != 4) return fail(PPS_E_INVALID, "Lotka-Volterra estimation has 4 parameters, d=%d", d);
     if (!prob->obs || prob->n_obs < 1) return fail(PPS_E_INVALID, "Lotka-Volterra needs observations");
     P->n_obs = prob->n_obs;
-    if (prob->objective == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5) {
+    const int obj = prob->objective;
+    const bool adaptive =
+        obj == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5 || obj == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT;
+    const bool saveat =
+        obj == PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT || obj == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT;
+    if (adaptive || saveat) {
       const float* q = prob->params;
-      if (!(q[2] > 0.0f && std::isfinite(q[2])))
-        return fail(PPS_E_INVALID, "observation interval must be positive");
+      if (!(q[2] > 0.0f && std::isfinite((float)(prob->n_obs - 1) * q[2])))
+        return fail(PPS_E_INVALID, "observation interval must be positive (and n_obs times it finite)");
       if (!(q[3] > 0.0f && std::isfinite(q[3])))
-        return fail(PPS_E_INVALID, "initial step must be positive");
+        return fail(PPS_E_INVALID, adaptive ? "initial step must be positive" : "step must be positive");
+      if (!adaptive && (double)(prob->n_obs - 1) * q[2] / q[3] > 1.0e7)
+        return fail(PPS_E_INVALID, "more than 1e7 fixed steps per evaluation");
+    }
+    if (adaptive) {
+      const float* q = prob->params;
       if (!(q[4] >= 0.0f && q[5] >= 0.0f && q[4] + q[5] > 0.0f && std::isfinite(q[4] + q[5])))
         return fail(PPS_E_INVALID, "abstol and reltol must be >= 0 with a positive sum");
       if (prob->n_params > 6 && !(q[6] >= 1.0f && q[6] <= 1.0e7f && q[6] == std::floor(q[6])))
         return fail(PPS_E_INVALID, "the step limit must be an integer in 1..1e7");
-    } else {
+    } else if (!saveat) {
       const float spo = prob->params[3];
       if (!(spo >= 1.0f && spo <= 1.0e6f) || spo != std::floor(spo))
         return fail(PPS_E_INVALID, "steps per observation must be a positive integer");
@@ -425,9 +434,7 @@ static int swarm_create(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_p
     return fail(PPS_E_INVALID, "unknown gbest_from %d", prm->gbest_from);
   if (prm->clamp != PPS_CLAMP_COMPONENT && prm->clamp != PPS_CLAMP_LEXICOGRAPHIC)
     return fail(PPS_E_INVALID, "unknown clamp %d", prm->clamp);
-  if (prm->clamp == PPS_CLAMP_LEXICOGRAPHIC && prob && prob->objective != PPS_OBJ_LOTKA_VOLTERRA &&
-      prob->objective != PPS_OBJ_LOTKA_VOLTERRA_TSIT5 &&
-      prob->objective != PPS_OBJ_LOTKA_VOLTERRA_ATSIT5)
+  if (prm->clamp == PPS_CLAMP_LEXICOGRAPHIC && prob && !is_lotka_volterra(prob->objective))
     return fail(PPS_E_INVALID,
                 "lexicographic clamping is the ODE estimation path's; it needs "
                 "PPS_OBJ_LOTKA_VOLTERRA (objective %d given)", prob->objective);

@@ -2,6 +2,13 @@
 #pragma once
 #include "pps.h"
 
+// The Lotka-Volterra estimation objectives (four rates, observations).
+constexpr bool is_lotka_volterra(int obj) {
+  return obj == PPS_OBJ_LOTKA_VOLTERRA || obj == PPS_OBJ_LOTKA_VOLTERRA_TSIT5 ||
+         obj == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5 || obj == PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT ||
+         obj == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT;
+}
+
 #define PPS_MAXD 64    // largest supported dimension
 #define PPS_MAXPAR 8   // objective parameters carried in kernel arguments
 

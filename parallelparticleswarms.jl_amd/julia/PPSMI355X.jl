@@ -319,19 +319,30 @@ end
 
 # parameter_estim_ode! (src/ode_pso.jl:54-170) for the Lotka-Volterra model,
 # solved inside the fused PSO kernel.  `data` is 2 x n_obs (sample k at
-# t = k * dt * saveat_steps from u0); ode_alg :rk4 or :tsit5 (the reference's
-# default GPUTsit5), adaptive = true for its Val(true) path (dt = initial step).
+# t = k * dt * saveat_steps from u0, or k * saveat when `saveat` is given);
+# ode_alg :rk4 or :tsit5 (the reference's default GPUTsit5), adaptive = true for
+# its Val(true) path (dt = initial step).  `saveat` (an interval, :tsit5 only)
+# decouples the steps from the samples, which are read from Tsit5's
+# interpolant as GPUTsit5 fills saveat (the reference's benchmark: dt = 0.1,
+# saveat = 0.01, benchmarks/Lotka_Volterra/lotka_volterra.jl:124-127).
 # Returns (gbest position, FP32 loss).
 function parameter_estim_lv(data::AbstractMatrix, lb, ub; backend = MI355XBackend(),
         num_particles = 2^14, maxiters = 300, u0 = (1.0f0, 1.0f0), dt = 0.01f0,
-        saveat_steps = 10, ode_alg = :tsit5, adaptive = false, abstol = 1.0f-6,
-        reltol = 1.0f-3, max_steps = 10000, w = 0.7298f0, wdamp = 1.0f0,
+        saveat_steps = 10, saveat = nothing, ode_alg = :tsit5, adaptive = false,
+        abstol = 1.0f-6, reltol = 1.0f-3, max_steps = 10000, w = 0.7298f0, wdamp = 1.0f0,
         seed::Integer = rand(UInt64))
-    code = ode_alg === :rk4 ? Int32(16) : adaptive ? Int32(18) : Int32(17)
     (ode_alg === :rk4 && adaptive) && throw(ArgumentError("adaptive stepping uses :tsit5"))
-    par = adaptive ?
-          Float32[u0[1], u0[2], Float32(dt) * saveat_steps, dt, abstol, reltol, max_steps] :
-          Float32[u0[1], u0[2], dt, saveat_steps]
+    (ode_alg === :rk4 && saveat !== nothing) &&
+        throw(ArgumentError("saveat between steps uses :tsit5's interpolant"))
+    if saveat !== nothing
+        code = adaptive ? Int32(20) : Int32(19)
+        par = Float32[u0[1], u0[2], saveat, dt, abstol, reltol, max_steps]
+    else
+        code = ode_alg === :rk4 ? Int32(16) : adaptive ? Int32(18) : Int32(17)
+        par = adaptive ?
+              Float32[u0[1], u0[2], Float32(dt) * saveat_steps, dt, abstol, reltol, max_steps] :
+              Float32[u0[1], u0[2], dt, saveat_steps]
+    end
     obs = Float32.(vec(data))                      # [u1(t0), u2(t0), u1(t1), ...]
     l, u = Float32.(collect(lb)), Float32.(collect(ub))
     z = zeros(Float32, 4)

@@ -21,6 +21,7 @@ OK, E_INVALID, E_HIP, E_NOMEM, E_NOTFOUND, E_PENDING, E_CANCELLED, E_TIMEOUT, E_
 OBJ_ROSENBROCK, OBJ_RASTRIGIN, OBJ_SPHERE, OBJ_BEALE, OBJ_BOOTH, OBJ_HIMMELBLAU = 1, 2, 3, 4, 5, 6
 OBJ_QUADRATIC, OBJ_BOUNDARY_QUADRATIC, OBJ_ROSEN2, OBJ_LOTKA_VOLTERRA = 7, 8, 9, 16
 OBJ_LOTKA_VOLTERRA_TSIT5, OBJ_LOTKA_VOLTERRA_ATSIT5 = 17, 18
+OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT, OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT = 19, 20
 
 MODE_SYNC, MODE_ASYNC, MODE_QLOCK = 0, 1, 2
 GBEST_PBEST, GBEST_CURRENT = 0, 1

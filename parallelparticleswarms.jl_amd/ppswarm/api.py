@@ -161,7 +161,11 @@ class LotkaVolterraSSE(Objective):
     reference's GPUTsit5 with Val(false)) or adaptive Tsit5 (``"tsit5",
     adaptive=True``, Val(true): ``dt`` is the initial step, ``abstol`` /
     ``reltol`` the tolerances, ``maxiters`` the step limit of one evaluation,
-    beyond which the loss is +Inf)."""
+    beyond which the loss is +Inf).  With ``saveat`` (Tsit5 only) sample k is
+    at t = k * saveat and steps of ``dt`` run independently of the samples,
+    which are read from Tsit5's continuous extension, as GPUTsit5 fills
+    ``saveat`` (benchmarks/Lotka_Volterra/lotka_volterra.jl:124-127: dt = 0.1,
+    samples every 0.01); ``steps_per_obs`` is then unused."""
 
     obs: Any = None
     u0: tuple = (1.0, 1.0)
@@ -171,8 +175,12 @@ class LotkaVolterraSSE(Objective):
     abstol: float = 1e-6
     reltol: float = 1e-3
     maxiters: int = 10000
+    saveat: Optional[float] = None
 
     def params(self) -> np.ndarray:
+        if self.saveat is not None:
+            return np.array([self.u0[0], self.u0[1], self.saveat, self.dt, self.abstol,
+                             self.reltol, self.maxiters], np.float32)
         if self.adaptive:
             obs_dt = np.float32(np.float64(self.dt) * self.steps_per_obs)
             return np.array([self.u0[0], self.u0[1], obs_dt, self.dt, self.abstol, self.reltol,
@@ -181,18 +189,25 @@ class LotkaVolterraSSE(Objective):
 
 
 def lotka_volterra_sse(obs, u0=(1.0, 1.0), dt=0.01, steps_per_obs=10, *, ode_alg="rk4",
-                       adaptive=False, abstol=1e-6, reltol=1e-3,
-                       maxiters=10000) -> LotkaVolterraSSE:
+                       adaptive=False, abstol=1e-6, reltol=1e-3, maxiters=10000,
+                       saveat=None) -> LotkaVolterraSSE:
     obs = np.ascontiguousarray(obs, np.float32).reshape(-1, 2)
     if ode_alg not in ("rk4", "tsit5"):
         raise ValueError(f"ode_alg is 'rk4' or 'tsit5', got {ode_alg!r}")
     if adaptive and ode_alg != "tsit5":
         raise ValueError("adaptive stepping uses Tsit5 (ode_alg='tsit5')")
-    code = (L.OBJ_LOTKA_VOLTERRA if ode_alg == "rk4" else
-            L.OBJ_LOTKA_VOLTERRA_ATSIT5 if adaptive else L.OBJ_LOTKA_VOLTERRA_TSIT5)
+    if saveat is not None and ode_alg != "tsit5":
+        raise ValueError("saveat between steps uses Tsit5's interpolant (ode_alg='tsit5')")
+    if saveat is not None:
+        code = (L.OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT if adaptive
+                else L.OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT)
+        saveat = float(saveat)
+    else:
+        code = (L.OBJ_LOTKA_VOLTERRA if ode_alg == "rk4" else
+                L.OBJ_LOTKA_VOLTERRA_ATSIT5 if adaptive else L.OBJ_LOTKA_VOLTERRA_TSIT5)
     return LotkaVolterraSSE(code, "lotka_volterra", 4, 6 if adaptive else 4, obs, tuple(u0),
                             float(dt), int(steps_per_obs), bool(adaptive), float(abstol),
-                            float(reltol), int(maxiters))
+                            float(reltol), int(maxiters), saveat)
 
 
 # ---------------------------------------------------------------------------

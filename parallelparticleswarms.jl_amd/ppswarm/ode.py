@@ -23,13 +23,15 @@ from . import api
 def parameter_estim_ode(obs, lb, ub, num_particles=1 << 14, maxiters=300, *, u0=(1.0, 1.0),
                         dt=0.01, steps_per_obs=10, w=0.7298, wdamp=1.0, seed=None,
                         clamp="lexicographic", launch=L.LAUNCH_AUTO, ode_alg="rk4",
-                        adaptive=False, abstol=1e-6, reltol=1e-3):
+                        adaptive=False, abstol=1e-6, reltol=1e-3, saveat=None):
     """Estimate (alpha, beta, gamma, delta) of Lotka-Volterra from `obs`
-    (n_obs x 2, sample k at t = k * dt * steps_per_obs).  ode_alg "rk4" or
-    "tsit5"; adaptive=True (Tsit5 only) is the reference's Val(true) path with
-    `dt` as the initial step.  Returns (solution, swarm)."""
+    (n_obs x 2, sample k at t = k * dt * steps_per_obs, or k * saveat when
+    `saveat` is given).  ode_alg "rk4" or "tsit5"; adaptive=True (Tsit5 only)
+    is the reference's Val(true) path with `dt` as the initial step; saveat
+    (Tsit5 only) decouples the steps from the samples, which are read from
+    Tsit5's interpolant.  Returns (solution, swarm)."""
     f = api.lotka_volterra_sse(obs, u0=u0, dt=dt, steps_per_obs=steps_per_obs, ode_alg=ode_alg,
-                               adaptive=adaptive, abstol=abstol, reltol=reltol)
+                               adaptive=adaptive, abstol=abstol, reltol=reltol, saveat=saveat)
     prob = api.OptimizationProblem(f, np.zeros(4, np.float32), lb=np.asarray(lb, np.float32),
                                    ub=np.asarray(ub, np.float32))
     seed = api._next_seed() if seed is None else int(seed)

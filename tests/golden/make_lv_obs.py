@@ -4,6 +4,12 @@ du1 = a u1 - b u1 u2, du2 = -c u2 + d u1 u2 with (a, b, c, d) = (1.5, 1, 3, 1),
 u0 = (1, 1), t in [0, 10] sampled at 101 points: classical RK4 in FP64 with
 dt = 0.01 (10 steps between samples), plus Gaussian noise N(0, 0.05^2) drawn
 from numpy's default_rng(20240601).  Run: python tests/golden/make_lv_obs.py
+
+Also lv_obs_benchmark.json, the data of the reference's Lotka-Volterra benchmark
+(benchmarks/Lotka_Volterra/lotka_volterra.jl:21-37: the same system on
+t in [0, 3] sampled every 0.01, 301 points, noise-free from a tight-tolerance
+solve): here FP64 RK4 with 20 steps of 0.0005 between samples (global error
+~1e-12).
 """
 import json
 import os
@@ -41,7 +47,15 @@ def main():
     obs = traj + noise
     doc = {"theta": THETA, "u0": U0, "dt": DT, "steps_per_obs": STEPS_PER_OBS, "sigma": SIGMA,
            "seed": SEED, "trajectory": traj.tolist(), "obs": obs.tolist()}
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lv_obs.json")
+    here = os.path.dirname(os.path.abspath(__file__))
+    path = os.path.join(here, "lv_obs.json")
+    with open(path, "w") as fh:
+        json.dump(doc, fh)
+    print(f"wrote {path}")
+    bench = rk4_trajectory(dt=0.0005, spo=20, n_obs=301)
+    doc = {"theta": THETA, "u0": U0, "saveat": 0.01, "t_end": 3.0, "integrator":
+           "FP64 RK4, dt = 0.0005", "obs": bench.tolist()}
+    path = os.path.join(here, "lv_obs_benchmark.json")
     with open(path, "w") as fh:
         json.dump(doc, fh)
     print(f"wrote {path}")

@@ -83,3 +83,51 @@ def test_lv_tsit5_estimation_recovers_parameters(adaptive):
     theta = np.array(DOC["theta"])
     assert sol.objective < 0.6
     np.testing.assert_allclose(sol.u, theta, rtol=0.05)
+
+
+def _saveat_pair(n, seed, adaptive, mode, launch):
+    # steps of 0.03 (fixed) or from 0.01 (adaptive) against samples every 0.1:
+    # most samples are read from Tsit5's continuous extension
+    f = P.lotka_volterra_sse(OBS, u0=(1.0, 1.0), dt=0.01 if adaptive else 0.03, ode_alg="tsit5",
+                             adaptive=adaptive, abstol=1e-6, reltol=1e-3, saveat=0.1)
+    lb, ub = np.zeros(4, np.float32), np.full(4, 5.0, np.float32)
+    prob = P.OptimizationProblem(f, np.zeros(4), lb=lb, ub=ub)
+    g = P.Swarm(P.context(), prob, P.swarm_params(n, mode, seed=seed, clamp=L.CLAMP_LEXICOGRAPHIC,
+                                                  launch=launch))
+    obj = O.LOTKA_VOLTERRA_ATSIT5_SAVEAT if adaptive else O.LOTKA_VOLTERRA_TSIT5_SAVEAT
+    o = O.Swarm(O.Problem(obj, 4, lb, ub, f.params(), clamp=O.CLAMP_LEXICOGRAPHIC, obs=OBS), n,
+                seed)
+    return g, o
+
+
+@pytest.mark.parametrize("adaptive", [False, True])
+@pytest.mark.parametrize("mode,n", [(L.MODE_SYNC, 4096), (L.MODE_SYNC, 1000), (L.MODE_ASYNC, 2048)])
+def test_lv_tsit5_saveat_parity(adaptive, mode, n):
+    # GPUTsit5 with saveat between steps (src/ode_pso.jl:54-170 passing
+    # saveat/dt through to the solver)
+    note(f"LV Tsit5 saveat adaptive={adaptive} mode={mode} n={n}")
+    g, o = _saveat_pair(n, 57 + n, adaptive, mode,
+                        L.LAUNCH_GRAPH if mode == L.MODE_SYNC and n > 1024 else L.LAUNCH_AUTO)
+    assert_same_state(g, o, "after init")
+    assert_same_gbest(g, o, "after init")
+    g.run(3)
+    if mode == L.MODE_SYNC:
+        o.run_sync(3)
+    else:
+        o.run_async(3)
+    assert_same_state(g, o, "after 3 iterations")
+    assert_same_gbest(g, o)
+
+
+def test_reference_lv_benchmark_shape():
+    # benchmarks/Lotka_Volterra/lotka_volterra.jl:21-37,91-127: 10000
+    # particles, 100 iterations, rates in [0, 10], GPUTsit5 with dt = 0.1 and
+    # saveat every 0.01 on [0, 3] against noise-free data
+    note("LV reference benchmark shape (saveat 0.01, dt 0.1)")
+    doc = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "lv_obs_benchmark.json")))
+    obs = np.array(doc["obs"], np.float32)
+    sol, sw = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 10.0), 10000, 100, seed=1,
+                                    ode_alg="tsit5", dt=0.1, saveat=0.01)
+    np.testing.assert_allclose(sol.u, doc["theta"], rtol=1e-3)
+    assert sol.objective < 1e-5

@@ -11,6 +11,12 @@ not vendored; parity with it is unpinned.  What pins the restatement:
   and at tight tolerances it reaches the FP32 floor;
 * the FP64 evaluator (pps_eval_f64, product host code) agrees with the FP32
   oracle on noisy data.
+Saveat semantics (samples read from Tsit5's continuous extension between steps):
+* the extension's weights satisfy the 8 continuous order conditions of order
+  <= 4 at every theta, reduce to b at theta = 1 and vanish at theta = 0;
+* with samples on step ends it reproduces step-end sampling;
+* with steps 2-8 times the sample spacing it still converges at 5th order (FP32
+  oracle and FP64 evaluator), and the adaptive form meets its tolerances.
 """
 import ctypes as C
 
@@ -138,3 +144,106 @@ def test_fp64_evaluator_agrees_with_the_fp32_oracle(pps):
                                      C.byref(out)))
         # FP32 rounding over the trajectory: ~1e-5 relative on this loss
         assert abs(out.value - f32) < 5e-5 * f32 + 1e-7, (obj, out.value, f32)
+
+
+# ---------------------------------------------------------------------------
+# saveat semantics: observations read from Tsit5's continuous extension
+# (GPUTsit5 filling `saveat` between steps, as the reference's benchmark does
+# with dt = 0.1 and samples every 0.01,
+# benchmarks/Lotka_Volterra/lotka_volterra.jl:124-127).
+
+def test_interpolant_weights_satisfy_the_continuous_order_conditions():
+    # a continuous extension of order 4: sum_i b_i(theta) Phi_i = theta^r / gamma
+    # for the 8 trees of order <= 4 at every theta, b_i(1) = b_i, b_i(0) = 0,
+    # and (inside the step) not order 5
+    a, _ = O.tsit5_tableau()
+    A = np.zeros((7, 7))
+    A[:, :6] = a.astype(np.float64)
+    c = A.sum(axis=1)
+    Ac = A @ c
+    for th in (0.1, 0.25, 0.37, 0.5, 0.83, 1.0):
+        B = O.tsit5_interp_weights(th).astype(np.float64)
+        conds = [B.sum() - th, B @ c - th ** 2 / 2, B @ c ** 2 - th ** 3 / 3, B @ Ac - th ** 3 / 6,
+                 B @ c ** 3 - th ** 4 / 4, B @ (c * Ac) - th ** 4 / 8,
+                 B @ (A @ c ** 2) - th ** 4 / 12, B @ (A @ Ac) - th ** 4 / 24]
+        assert max(abs(x) for x in conds) < 2e-6, th
+    np.testing.assert_allclose(O.tsit5_interp_weights(1.0), A[6], atol=2e-6)
+    assert not O.tsit5_interp_weights(0.0).any()
+    B = O.tsit5_interp_weights(0.5).astype(np.float64)
+    assert abs(B @ c ** 4 - 0.5 ** 5 / 5) > 1e-4
+
+
+def _dense_reference():
+    return _reference_trajectory(obs_dt=0.05, n_obs=61, sub=200)   # t in [0, 3]
+
+
+def test_saveat_on_step_ends_matches_step_end_sampling():
+    # saveat = spo * dt puts every observation on a step end, where the
+    # extension (theta = 1) must give the step's own value: losses against
+    # any data agree to rounding
+    rng = np.random.default_rng(2)
+    for obs in (np.zeros((21, 2)), rng.uniform(0, 5, (21, 2))):
+        for spo in (1, 2, 5):
+            a = float(O.objective(_lv(O.LOTKA_VOLTERRA_TSIT5, [1, 1, 0.5 / spo, spo], obs),
+                                  THETA.astype(F)))
+            b = float(O.objective(_lv(O.LOTKA_VOLTERRA_TSIT5_SAVEAT, [1, 1, 0.5, 0.5 / spo], obs),
+                                  THETA.astype(F)))
+            assert abs(a - b) <= 2e-6 * a, (spo, a, b)
+
+
+def test_saveat_fixed_steps_converge_at_fifth_order():
+    # steps of 0.4 .. 0.1 with samples every 0.05: most samples come from the
+    # extension; a wrong weight would drop the order to 1 or 2
+    traj = _dense_reference()
+    err = [np.sqrt(float(O.objective(_lv(O.LOTKA_VOLTERRA_TSIT5_SAVEAT, [1, 1, 0.05, dt], traj),
+                                     THETA.astype(F))))
+           for dt in (0.4, 0.2, 0.1)]
+    for e0, e1 in zip(err, err[1:]):
+        assert 4.5 < np.log2(e0 / e1) < 7, err
+
+
+def test_saveat_adaptive_meets_its_tolerances():
+    traj = _dense_reference()
+    err = [np.sqrt(float(O.objective(
+        _lv(O.LOTKA_VOLTERRA_ATSIT5_SAVEAT, [1, 1, 0.05, 0.01, tol, tol], traj), THETA.astype(F))))
+        for tol in (1e-3, 1e-5, 1e-7)]
+    assert err[0] > err[1] > err[2] and err[2] < 2e-5, err
+
+
+def test_saveat_adaptive_gives_up_with_inf():
+    traj = _reference_trajectory()
+    pr = _lv(O.LOTKA_VOLTERRA_ATSIT5_SAVEAT, [1, 1, 0.5, 0.01, 1e-9, 1e-9], traj)
+    assert O.objective(pr, np.array([5, 0, 0, 5], F)) == np.inf
+
+
+def _f64(obj, params, obs):
+    from ppswarm import _lib as L
+
+    pr = L.pps_problem()
+    par = np.asarray(params, F)
+    lb, ub = np.zeros(4, F), np.full(4, 5, F)
+    obs = np.ascontiguousarray(obs, F)
+    pr.objective, pr.d = obj, 4
+    pr.lb, pr.ub = lb.ctypes.data_as(L.fp), ub.ctypes.data_as(L.fp)
+    pr.params, pr.n_params = par.ctypes.data_as(L.fp), len(par)
+    pr.obs, pr.n_obs = obs.ctypes.data_as(L.fp), len(obs)
+    x = THETA.astype(np.float64)
+    out = C.c_double()
+    L.check(L.lib().pps_eval_f64(C.byref(pr), 1, x.ctypes.data_as(C.POINTER(C.c_double)),
+                                 C.byref(out)))
+    return out.value
+
+
+def test_saveat_fp64_evaluator_converges_and_agrees_with_the_oracle(pps):
+    traj = _dense_reference()
+    err = [np.sqrt(_f64(O.LOTKA_VOLTERRA_TSIT5_SAVEAT, [1, 1, 0.05, dt], traj))
+           for dt in (0.4, 0.2, 0.1, 0.05)]
+    for e0, e1 in zip(err, err[1:]):
+        assert 4.5 < np.log2(e0 / e1) < 7, err
+    rng = np.random.default_rng(3)
+    noisy = (traj + rng.normal(0, 0.05, traj.shape)).astype(F)
+    for obj, params in ((O.LOTKA_VOLTERRA_TSIT5_SAVEAT, [1, 1, 0.05, 0.03]),
+                        (O.LOTKA_VOLTERRA_ATSIT5_SAVEAT, [1, 1, 0.05, 0.01, 1e-7, 1e-6])):
+        f32 = float(O.objective(_lv(obj, params, noisy), THETA.astype(F)))
+        f64 = _f64(obj, params, noisy)
+        assert abs(f64 - f32) < 5e-5 * f32 + 1e-7, (obj, f64, f32)
