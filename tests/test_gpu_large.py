@@ -1,7 +1,8 @@
 """Parity at the full single-GPU benchmark shape: 2^20 particles, 10-D
 Rastrigin and 10-D Rosenbrock, bounds [-5.12, 5.12] — the whole swarm compared
 bit for bit with the oracle for a few iterations, plus size-independent
-properties over a longer run."""
+properties over a longer run; and two states past the Infinity Cache, where
+the sync step switches to nontemporal streams."""
 import numpy as np
 import pytest
 
@@ -22,6 +23,18 @@ def test_sync_full_shape_bitwise(obj):
     g.run(3)
     o.run_sync(3)
     assert_same_state(g, o, "after 3 iterations")
+    assert_same_gbest(g, o)
+
+
+@pytest.mark.parametrize("obj,d,n", [(O.RASTRIGIN, 32, 1 << 20), (O.ROSENBROCK, 10, 1 << 22)])
+def test_sync_beyond_the_cache_bitwise(obj, d, n):
+    # states of 407 MB and 520 MB: the sync step streams them with
+    # nontemporal hints (chosen above 288 MiB); values must not change
+    note(f"sync beyond cache obj={obj} d={d} n={n}")
+    g, o = make_pair(obj, d, n, seed=99 + d, launch=L.LAUNCH_GRAPH)
+    g.run(2)
+    o.run_sync(2)
+    assert_same_state(g, o, "after 2 iterations")
     assert_same_gbest(g, o)
 
 
