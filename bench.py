@@ -218,6 +218,19 @@ def ode_c3(P):
         "theta": [float(v) for v in sol.u], "loss_f64": sol.objective,
         "max_rel_err_theta": float(np.max(np.abs(np.asarray(sol.u) / np.asarray(doc["theta"]) - 1))),
         "workload": "10000 particles x 100 iterations, Tsit5 dt 0.1, 301 samples every 0.01"}
+    # the reference's FitzHugh-Nagumo benchmark shape (same solver settings,
+    # rates in [0, 5])
+    doc = _json.load(open(os.path.join(ROOT, "tests", "golden", "fhn_obs_benchmark.json")))
+    obs = np.array(doc["obs"], np.float32)
+    kw = dict(ode_alg="tsit5", dt=0.1, saveat=0.01, model="fitzhugh_nagumo")
+    P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 10000, 5, seed=0, **kw)  # warm-up
+    sol, sw = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 10000, 100, seed=0, **kw)
+    ms = sw.elapsed_ms()
+    sw.free()
+    out["fitzhugh_nagumo_reference_benchmark_shape"] = {
+        "device_ms": ms, "particle_iters_per_s": 10000 * 100 / (ms / 1e3),
+        "theta": [float(v) for v in sol.u], "loss_f64": sol.objective,
+        "workload": "10000 particles x 100 iterations, Tsit5 dt 0.1, 301 samples every 0.01"}
     return out
 
 

@@ -75,7 +75,16 @@ enum {
    * (how GPUTsit5 fills `saveat`, e.g. dt = 0.1 with saveat every 0.01 in
    * benchmarks/Lotka_Volterra/lotka_volterra.jl:124-127). */
   PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT = 19,  /* fixed dt                     */
-  PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT = 20  /* adaptive                     */
+  PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT = 20, /* adaptive                     */
+  /* FitzHugh-Nagumo parameter estimation, rates (a, b, tau_inv, l):
+   * dv = v - v^3/3 - w + l, dw = tau_inv (v + a - b w) (reference
+   * benchmarks/Fitzhugh_Nagumo/ode_estimation.jl:6-12), with the same five
+   * solvers, parameters and observation layout as Lotka-Volterra. */
+  PPS_OBJ_FITZHUGH_NAGUMO = 21,
+  PPS_OBJ_FITZHUGH_NAGUMO_TSIT5 = 22,
+  PPS_OBJ_FITZHUGH_NAGUMO_ATSIT5 = 23,
+  PPS_OBJ_FITZHUGH_NAGUMO_TSIT5_SAVEAT = 24,
+  PPS_OBJ_FITZHUGH_NAGUMO_ATSIT5_SAVEAT = 25
 };
 
 /* Swarm update semantics. */
@@ -102,7 +111,7 @@ enum {
   PPS_CLAMP_COMPONENT = 0,    /* min.(max.(x, lb), ub)  src/kernels.jl:15-17 */
   PPS_CLAMP_LEXICOGRAPHIC = 1 /* min(max(x, lb), ub) on whole vectors,
                                  src/ode_pso.jl:18-19; the ODE estimation
-                                 path's, so PPS_OBJ_LOTKA_VOLTERRA only      */
+                                 path's, so the ODE objectives only          */
 };
 
 /* Launch strategy for PPS_MODE_SYNC. */
@@ -117,14 +126,15 @@ enum {
  * f, lb, ub, p; reference src/init.jl:88-115, src/utils.jl:310-318). */
 typedef struct pps_problem {
   int32_t objective;     /* PPS_OBJ_*                                        */
-  int32_t d;             /* dimension, 1..64 (LOTKA_VOLTERRA: 4)             */
+  int32_t d;             /* dimension, 1..64 (ODE objectives: 4)             */
   const float* lb;       /* d floats, lb <= ub; finite, or -Inf/+Inf in every */
   const float* ub;       /* coordinate; both NULL = unbounded (the unbounded
                             sampler around u0, src/init.jl:70-74)            */
   const float* params;   /* n_params floats (Rosenbrock: [p1, p2])           */
   int32_t n_params;
   const float* shift;    /* d floats or NULL: f is evaluated at x - shift    */
-  /* Lotka-Volterra objectives only: observations u(t_k), k = 0..n_obs-1,
+  /* ODE objectives (Lotka-Volterra, FitzHugh-Nagumo) only: observations
+   * u(t_k), k = 0..n_obs-1,
    * interleaved [u1(t0), u2(t0), u1(t1), ...]; the trajectory starts at
    * params[0..1] = u0.  RK4 and fixed-step Tsit5: steps dt = params[2],
    * sampled every (int)params[3] steps.  Adaptive Tsit5: t_k = k * params[2],

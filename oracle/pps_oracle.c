@@ -40,7 +40,8 @@ enum {
   ORA_BOOTH = 5, ORA_HIMMELBLAU = 6, ORA_QUADRATIC = 7,
   ORA_BOUNDARY_QUADRATIC = 8, ORA_ROSEN2 = 9, ORA_LOTKA_VOLTERRA = 16,
   ORA_LOTKA_VOLTERRA_TSIT5 = 17, ORA_LOTKA_VOLTERRA_ATSIT5 = 18,
-  ORA_LOTKA_VOLTERRA_TSIT5_SAVEAT = 19, ORA_LOTKA_VOLTERRA_ATSIT5_SAVEAT = 20
+  ORA_LOTKA_VOLTERRA_TSIT5_SAVEAT = 19, ORA_LOTKA_VOLTERRA_ATSIT5_SAVEAT = 20,
+  ORA_FITZHUGH_NAGUMO = 21, ORA_FITZHUGH_NAGUMO_ATSIT5_SAVEAT = 25
 };
 enum { ORA_GBEST_PBEST = 0, ORA_GBEST_CURRENT = 1 };
 enum { ORA_CLAMP_COMPONENT = 0, ORA_CLAMP_LEXICOGRAPHIC = 1 };
@@ -139,12 +140,26 @@ float ora_sin2pi(float x) {
   return ora_cos2pi(r - 0.25f);
 }
 
-/* Lotka-Volterra (benchmarks/Lotka_Volterra/lotka_volterra.jl:9-13) and the
- * fixed-step classical RK4 SSE loss against observations (src/ode_pso.jl:47-52:
- * per sample sum(x.^2) of the 2-vector error, samples summed in order). */
-static void lv_rhs(float u1, float u2, const float* p, float* f1, float* f2) {
-  *f1 = p[0] * u1 - (p[1] * u1) * u2;
-  *f2 = (-p[2]) * u2 + (p[3] * u1) * u2;
+/* ODE estimation models, objective codes 16..20 (Lotka-Volterra,
+ * benchmarks/Lotka_Volterra/lotka_volterra.jl:9-13) and 21..25
+ * (FitzHugh-Nagumo, benchmarks/Fitzhugh_Nagumo/ode_estimation.jl:6-12, rates
+ * (a, b, tau_inv, l): dv = v - v^3/3 - w + l, dw = tau_inv (v + a - b w), v^3
+ * as Julia's literal power (v*v)*v), each with five solvers in the order RK4,
+ * Tsit5, adaptive Tsit5, and the two saveat forms; and the fixed-step
+ * classical RK4 SSE loss against observations (src/ode_pso.jl:47-52: per
+ * sample sum(x.^2) of the 2-vector error, samples summed in order). */
+static int ode_model(int obj) { return obj >= ORA_FITZHUGH_NAGUMO; }
+static int ode_solver(int obj) {
+  return obj - (obj >= ORA_FITZHUGH_NAGUMO ? ORA_FITZHUGH_NAGUMO : ORA_LOTKA_VOLTERRA);
+}
+static void ode_rhs(int model, float u1, float u2, const float* p, float* f1, float* f2) {
+  if (model) {
+    *f1 = ((u1 - ((u1 * u1) * u1) / 3.0f) - u2) + p[3];
+    *f2 = p[2] * ((u1 + p[0]) - p[1] * u2);
+  } else {
+    *f1 = p[0] * u1 - (p[1] * u1) * u2;
+    *f2 = (-p[2]) * u2 + (p[3] * u1) * u2;
+  }
 }
 /* Tsitouras 5(4) (Comput. Math. Appl. 62 (2011) 770-775), the method of the
  * reference's default ode_alg = GPUTsit5() (src/ode_pso.jl:54-170): stage
@@ -195,7 +210,7 @@ void ora_tsit5_interp_weights(float theta, float* b /* 7 */) {
 /* One Tsit5 step from u with k[0] = f(u): y_i = u + h * (a_i0 k_0 + ... ),
  * sums left to right; k[i] = f(y_i); the new state is y_6; k[6] = f(y_6)
  * (FSAL); err = h * (e_0 k_0 + ... + e_6 k_6). */
-static void lv_tsit5_step(const float u[2], float k[7][2], float h, const float* th,
+static void tsit5_step(int model, const float u[2], float k[7][2], float h, const float* th,
                           float un[2], float err[2]) {
   for (int i = 1; i < 7; ++i) {
     float y[2];
@@ -204,7 +219,7 @@ static void lv_tsit5_step(const float u[2], float k[7][2], float h, const float*
       for (int j = 1; j < i; ++j) acc = acc + TS_A[i][j] * k[j][c];
       y[c] = u[c] + h * acc;
     }
-    lv_rhs(y[0], y[1], th, &k[i][0], &k[i][1]);
+    ode_rhs(model, y[0], y[1], th, &k[i][0], &k[i][1]);
     if (i == 6) un[0] = y[0], un[1] = y[1];
   }
   for (int c = 0; c < 2; ++c) {
@@ -242,9 +257,10 @@ float ora_root5(float x) {
  * observation times (t_k = k * par[2] in FP32) and a step shortened to meet
  * one keeps the larger of the old and new proposal; more steps than par[6]
  * (10000 when par[6] <= 0), or a step that does not advance t, give +Inf. */
-float ora_lv_sse_tsit5(const float* th, const ora_problem* P, int adaptive) {
+float ora_ode_sse_tsit5(const float* th, const ora_problem* P, int adaptive) {
+  const int model = ode_model(P->obj);
   float u[2] = {P->par[0], P->par[1]}, k[7][2];
-  lv_rhs(u[0], u[1], th, &k[0][0], &k[0][1]);
+  ode_rhs(model, u[0], u[1], th, &k[0][0], &k[0][1]);
   float loss = 0.0f, t = 0.0f, dt = adaptive ? P->par[3] : P->par[2];
   int32_t steps = 0;
   for (int kk = 0; kk < P->n_obs; ++kk) {
@@ -252,7 +268,7 @@ float ora_lv_sse_tsit5(const float* th, const ora_problem* P, int adaptive) {
       if (!adaptive) {
         for (int s = 0; s < P->steps_per_obs; ++s) {
           float un[2], err[2];
-          lv_tsit5_step(u, k, dt, th, un, err);
+          tsit5_step(model, u, k, dt, th, un, err);
           u[0] = un[0], u[1] = un[1];
           k[0][0] = k[6][0], k[0][1] = k[6][1];
         }
@@ -265,7 +281,7 @@ float ora_lv_sse_tsit5(const float* th, const ora_problem* P, int adaptive) {
           float h = hit ? t_next - t : dt;
           if (++steps > max_steps || !(t + h > t)) return INFINITY;
           float un[2], err[2];
-          lv_tsit5_step(u, k, h, th, un, err);
+          tsit5_step(model, u, k, h, th, un, err);
           float s[2];
           for (int c = 0; c < 2; ++c) {
             float a = fabsf(u[c]), b = fabsf(un[c]);
@@ -304,12 +320,13 @@ float ora_lv_sse_tsit5(const float* th, const ora_problem* P, int adaptive) {
  * fall short of it; after an accepted step [t, t + h] every unsaved t_k <=
  * t + h gets u + h * (b_0(theta) k_0 + ... + b_6(theta) k_6), sum left to
  * right, theta = (t_k - t) / h.  Adaptive: the controller of
- * ora_lv_sse_tsit5 with next step h / q after every attempt; step limit par[6]
+ * ora_ode_sse_tsit5 with next step h / q after every attempt; step limit par[6]
  * (10000 when <= 0); +Inf on too many steps or a step that does not advance
  * t.  Fixed steps have no limit (the host bounds their number). */
-float ora_lv_sse_tsit5_saveat(const float* th, const ora_problem* P, int adaptive) {
+float ora_ode_sse_tsit5_saveat(const float* th, const ora_problem* P, int adaptive) {
+  const int model = ode_model(P->obj);
   float u[2] = {P->par[0], P->par[1]}, k[7][2];
-  lv_rhs(u[0], u[1], th, &k[0][0], &k[0][1]);
+  ode_rhs(model, u[0], u[1], th, &k[0][0], &k[0][1]);
   const float tau = P->par[2];
   const float t_end = (float)(P->n_obs - 1) * tau;
   float e1 = P->obs[0] - u[0], e2 = P->obs[1] - u[1];
@@ -323,7 +340,7 @@ float ora_lv_sse_tsit5_saveat(const float* th, const ora_problem* P, int adaptiv
     const float h = last ? t_end - t : dt;
     if (++steps > max_steps || !(t + h > t)) return INFINITY;
     float un[2], err[2];
-    lv_tsit5_step(u, k, h, th, un, err);
+    tsit5_step(model, u, k, h, th, un, err);
     if (adaptive) {
       float s[2];
       for (int c = 0; c < 2; ++c) {
@@ -361,7 +378,8 @@ float ora_lv_sse_tsit5_saveat(const float* th, const ora_problem* P, int adaptiv
   return loss;
 }
 
-float ora_lv_sse(const float* th, const ora_problem* P) {
+float ora_ode_sse_rk4(const float* th, const ora_problem* P) {
+  const int model = ode_model(P->obj);
   float u1 = P->par[0], u2 = P->par[1];
   float dt = P->par[2];
   float h2 = dt * 0.5f;
@@ -371,10 +389,10 @@ float ora_lv_sse(const float* th, const ora_problem* P) {
     if (k > 0) {
       for (int s = 0; s < P->steps_per_obs; ++s) {
         float k1a, k1b, k2a, k2b, k3a, k3b, k4a, k4b;
-        lv_rhs(u1, u2, th, &k1a, &k1b);
-        lv_rhs(u1 + h2 * k1a, u2 + h2 * k1b, th, &k2a, &k2b);
-        lv_rhs(u1 + h2 * k2a, u2 + h2 * k2b, th, &k3a, &k3b);
-        lv_rhs(u1 + dt * k3a, u2 + dt * k3b, th, &k4a, &k4b);
+        ode_rhs(model, u1, u2, th, &k1a, &k1b);
+        ode_rhs(model, u1 + h2 * k1a, u2 + h2 * k1b, th, &k2a, &k2b);
+        ode_rhs(model, u1 + h2 * k2a, u2 + h2 * k2b, th, &k3a, &k3b);
+        ode_rhs(model, u1 + dt * k3a, u2 + dt * k3b, th, &k4a, &k4b);
         u1 = u1 + h6 * (((k1a + 2.0f * k2a) + 2.0f * k3a) + k4a);
         u2 = u2 + h6 * (((k1b + 2.0f * k2b) + 2.0f * k3b) + k4b);
       }
@@ -439,16 +457,15 @@ float ora_objective(const ora_problem* P, const float* xin) {
       float a = P->par[0] - z[0], b = z[1] - z[0] * z[0];
       return a * a + P->par[1] * (b * b);
     }
-    case ORA_LOTKA_VOLTERRA:
-      return ora_lv_sse(z, P);
-    case ORA_LOTKA_VOLTERRA_TSIT5:
-      return ora_lv_sse_tsit5(z, P, 0);
-    case ORA_LOTKA_VOLTERRA_ATSIT5:
-      return ora_lv_sse_tsit5(z, P, 1);
-    case ORA_LOTKA_VOLTERRA_TSIT5_SAVEAT:
-      return ora_lv_sse_tsit5_saveat(z, P, 0);
-    case ORA_LOTKA_VOLTERRA_ATSIT5_SAVEAT:
-      return ora_lv_sse_tsit5_saveat(z, P, 1);
+    default:
+      if (P->obj < ORA_LOTKA_VOLTERRA || P->obj > ORA_FITZHUGH_NAGUMO_ATSIT5_SAVEAT) break;
+      switch (ode_solver(P->obj)) {
+        case 0: return ora_ode_sse_rk4(z, P);
+        case 1: return ora_ode_sse_tsit5(z, P, 0);
+        case 2: return ora_ode_sse_tsit5(z, P, 1);
+        case 3: return ora_ode_sse_tsit5_saveat(z, P, 0);
+        default: return ora_ode_sse_tsit5_saveat(z, P, 1);
+      }
   }
   return NAN;
 }

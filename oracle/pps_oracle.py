@@ -28,6 +28,8 @@ ROSENBROCK, RASTRIGIN, SPHERE, BEALE, BOOTH, HIMMELBLAU, QUADRATIC, BOUNDARY_QUA
 LOTKA_VOLTERRA = 16
 LOTKA_VOLTERRA_TSIT5, LOTKA_VOLTERRA_ATSIT5 = 17, 18
 LOTKA_VOLTERRA_TSIT5_SAVEAT, LOTKA_VOLTERRA_ATSIT5_SAVEAT = 19, 20
+(FITZHUGH_NAGUMO, FITZHUGH_NAGUMO_TSIT5, FITZHUGH_NAGUMO_ATSIT5, FITZHUGH_NAGUMO_TSIT5_SAVEAT,
+ FITZHUGH_NAGUMO_ATSIT5_SAVEAT) = 21, 22, 23, 24, 25
 GBEST_PBEST, GBEST_CURRENT = 0, 1
 SAMPLER_UNIFORM, SAMPLER_UNBOUNDED, SAMPLER_LATIN_HYPERCUBE = 0, 1, 2
 CLAMP_COMPONENT, CLAMP_LEXICOGRAPHIC = 0, 1
@@ -79,12 +81,12 @@ def lib():
         L.ora_penalty.restype = C.c_float
         L.ora_cost.argtypes = [C.POINTER(_Prob), fp, C.c_float]
         L.ora_cost.restype = C.c_float
-        L.ora_lv_sse.argtypes = [fp, C.POINTER(_Prob)]
+        L.ora_ode_sse_rk4.argtypes = [fp, C.POINTER(_Prob)]
         L.ora_root5.argtypes = [C.c_float]
         L.ora_root5.restype = C.c_float
         L.ora_tsit5_tableau.argtypes = [fp, fp]
         L.ora_tsit5_interp_weights.argtypes = [C.c_float, fp]
-        L.ora_lv_sse.restype = C.c_float
+        L.ora_ode_sse_rk4.restype = C.c_float
         L.ora_eval.argtypes = [C.POINTER(_Prob), C.c_int64, fp, fp]
         L.ora_argmin.argtypes = [fp, C.c_int64, C.c_int]
         L.ora_argmin.restype = C.c_int64

@@ -142,21 +142,32 @@ __device__ __forceinline__ float sin2pi(float x) {
   return cos2pi(r - 0.25f);
 }
 
-// Lotka-Volterra right-hand side, in the operation order of the reference's
-// benchmark model (benchmarks/Lotka_Volterra/lotka_volterra.jl:9-13):
+// Right-hand sides of the ODE estimation models (two states, four rates), in
+// the operation order of the reference's benchmark models:
+// Lotka-Volterra (benchmarks/Lotka_Volterra/lotka_volterra.jl:9-13)
 //   du1 = p1*u1 - p2*u1*u2 ;  du2 = -p3*u2 + p4*u1*u2
-__device__ __forceinline__ void lv_rhs(float u1, float u2, float p1, float p2,
-                                       float p3, float p4, float& f1,
-                                       float& f2) {
-  f1 = p1 * u1 - (p2 * u1) * u2;
-  f2 = (-p3) * u2 + (p4 * u1) * u2;
+// FitzHugh-Nagumo (benchmarks/Fitzhugh_Nagumo/ode_estimation.jl:6-12), rates
+// (a, b, tau_inv, l):
+//   dv = v - v^3/3 - w + l ;  dw = tau_inv * (v + a - b*w)
+// with v^3 = (v*v)*v (Julia's literal power) and the division by 3 exact IEEE.
+template <int MODEL>
+__device__ __forceinline__ void ode_rhs(float u1, float u2, float p1, float p2, float p3,
+                                        float p4, float& f1, float& f2) {
+  if constexpr (MODEL == PPS_ODE_FITZHUGH_NAGUMO) {
+    f1 = ((u1 - ((u1 * u1) * u1) / 3.0f) - u2) + p4;
+    f2 = p3 * ((u1 + p1) - p2 * u2);
+  } else {
+    f1 = p1 * u1 - (p2 * u1) * u2;
+    f2 = (-p3) * u2 + (p4 * u1) * u2;
+  }
 }
 
 // Sum of squared errors of a fixed-step RK4 trajectory (dt = par[2], sampled
 // every steps_per_obs steps, starting at u0 = par[0..1] at sample 0) against
 // the observations; the per-sample error is e1^2 + e2^2 and samples are added
 // in time order (reference src/ode_pso.jl:47-52).
-__device__ __forceinline__ float lv_sse(const float* th, const ProbArgs& P) {
+template <int MODEL>
+__device__ __forceinline__ float ode_sse_rk4(const float* th, const ProbArgs& P) {
   const float p1 = th[0], p2 = th[1], p3 = th[2], p4 = th[3];
   float u1 = P.par[0], u2 = P.par[1];
   const float dt = P.par[2];
@@ -168,10 +179,10 @@ __device__ __forceinline__ float lv_sse(const float* th, const ProbArgs& P) {
     if (k > 0) {
       for (int s = 0; s < P.steps_per_obs; ++s) {
         float k1a, k1b, k2a, k2b, k3a, k3b, k4a, k4b;
-        lv_rhs(u1, u2, p1, p2, p3, p4, k1a, k1b);
-        lv_rhs(u1 + h2 * k1a, u2 + h2 * k1b, p1, p2, p3, p4, k2a, k2b);
-        lv_rhs(u1 + h2 * k2a, u2 + h2 * k2b, p1, p2, p3, p4, k3a, k3b);
-        lv_rhs(u1 + dt * k3a, u2 + dt * k3b, p1, p2, p3, p4, k4a, k4b);
+        ode_rhs<MODEL>(u1, u2, p1, p2, p3, p4, k1a, k1b);
+        ode_rhs<MODEL>(u1 + h2 * k1a, u2 + h2 * k1b, p1, p2, p3, p4, k2a, k2b);
+        ode_rhs<MODEL>(u1 + h2 * k2a, u2 + h2 * k2b, p1, p2, p3, p4, k3a, k3b);
+        ode_rhs<MODEL>(u1 + dt * k3a, u2 + dt * k3b, p1, p2, p3, p4, k4a, k4b);
         u1 = u1 + h6 * (((k1a + 2.0f * k2a) + 2.0f * k3a) + k4a);
         u2 = u2 + h6 * (((k1b + 2.0f * k2b) + 2.0f * k3b) + k4b);
       }
@@ -218,25 +229,25 @@ __device__ __forceinline__ float lv_sse(const float* th, const ProbArgs& P) {
 #define TS_E6 -0.45808210592918697f
 #define TS_E7 0.015151515151515152f
 
-// One Tsit5 step of the Lotka-Volterra system from (u1, u2) with k1 = f(u) in
+// One Tsit5 step of the ODE model from (u1, u2) with k1 = f(u) in
 // ka[0], kb[0]: stage sums left to right, y_i = u + h * sum_j a_ij k_j; fills
 // ka/kb[1..6] with k2..k7 (k7 = f(new state), the next step's k1) and returns
 // the new state and, when ERR, the error estimate h * sum_j btilde_j k_j.
-template <bool ERR>
-__device__ __forceinline__ void lv_tsit5_stages(float u1, float u2, float h, const float* th,
-                                                float (&ka)[7], float (&kb)[7], float& n1,
-                                                float& n2, float& e1, float& e2) {
+template <int MODEL, bool ERR>
+__device__ __forceinline__ void tsit5_stages(float u1, float u2, float h, const float* th,
+                                             float (&ka)[7], float (&kb)[7], float& n1, float& n2,
+                                             float& e1, float& e2) {
   const float p1 = th[0], p2 = th[1], p3 = th[2], p4 = th[3];
-  lv_rhs(u1 + h * (TS_A21 * ka[0]), u2 + h * (TS_A21 * kb[0]), p1, p2, p3, p4, ka[1], kb[1]);
-  lv_rhs(u1 + h * (TS_A31 * ka[0] + TS_A32 * ka[1]), u2 + h * (TS_A31 * kb[0] + TS_A32 * kb[1]),
+  ode_rhs<MODEL>(u1 + h * (TS_A21 * ka[0]), u2 + h * (TS_A21 * kb[0]), p1, p2, p3, p4, ka[1], kb[1]);
+  ode_rhs<MODEL>(u1 + h * (TS_A31 * ka[0] + TS_A32 * ka[1]), u2 + h * (TS_A31 * kb[0] + TS_A32 * kb[1]),
          p1, p2, p3, p4, ka[2], kb[2]);
-  lv_rhs(u1 + h * ((TS_A41 * ka[0] + TS_A42 * ka[1]) + TS_A43 * ka[2]),
+  ode_rhs<MODEL>(u1 + h * ((TS_A41 * ka[0] + TS_A42 * ka[1]) + TS_A43 * ka[2]),
          u2 + h * ((TS_A41 * kb[0] + TS_A42 * kb[1]) + TS_A43 * kb[2]), p1, p2, p3, p4, ka[3],
          kb[3]);
-  lv_rhs(u1 + h * (((TS_A51 * ka[0] + TS_A52 * ka[1]) + TS_A53 * ka[2]) + TS_A54 * ka[3]),
+  ode_rhs<MODEL>(u1 + h * (((TS_A51 * ka[0] + TS_A52 * ka[1]) + TS_A53 * ka[2]) + TS_A54 * ka[3]),
          u2 + h * (((TS_A51 * kb[0] + TS_A52 * kb[1]) + TS_A53 * kb[2]) + TS_A54 * kb[3]), p1, p2,
          p3, p4, ka[4], kb[4]);
-  lv_rhs(u1 + h * ((((TS_A61 * ka[0] + TS_A62 * ka[1]) + TS_A63 * ka[2]) + TS_A64 * ka[3]) +
+  ode_rhs<MODEL>(u1 + h * ((((TS_A61 * ka[0] + TS_A62 * ka[1]) + TS_A63 * ka[2]) + TS_A64 * ka[3]) +
                    TS_A65 * ka[4]),
          u2 + h * ((((TS_A61 * kb[0] + TS_A62 * kb[1]) + TS_A63 * kb[2]) + TS_A64 * kb[3]) +
                    TS_A65 * kb[4]),
@@ -245,7 +256,7 @@ __device__ __forceinline__ void lv_tsit5_stages(float u1, float u2, float h, con
                   TS_B5 * ka[4]) + TS_B6 * ka[5]);
   n2 = u2 + h * (((((TS_B1 * kb[0] + TS_B2 * kb[1]) + TS_B3 * kb[2]) + TS_B4 * kb[3]) +
                   TS_B5 * kb[4]) + TS_B6 * kb[5]);
-  lv_rhs(n1, n2, p1, p2, p3, p4, ka[6], kb[6]);
+  ode_rhs<MODEL>(n1, n2, p1, p2, p3, p4, ka[6], kb[6]);
   if (ERR) {
     e1 = h * ((((((TS_E1 * ka[0] + TS_E2 * ka[1]) + TS_E3 * ka[2]) + TS_E4 * ka[3]) +
                 TS_E5 * ka[4]) + TS_E6 * ka[5]) + TS_E7 * ka[6]);
@@ -255,14 +266,14 @@ __device__ __forceinline__ void lv_tsit5_stages(float u1, float u2, float h, con
 }
 
 // The same step with k1 passed and k7 returned as scalars.
-template <bool ERR>
-__device__ __forceinline__ void lv_tsit5_step(float u1, float u2, float k1a, float k1b, float h,
-                                              const float* th, float& n1, float& n2, float& k7a,
-                                              float& k7b, float& e1, float& e2) {
+template <int MODEL, bool ERR>
+__device__ __forceinline__ void tsit5_step(float u1, float u2, float k1a, float k1b, float h,
+                                           const float* th, float& n1, float& n2, float& k7a,
+                                           float& k7b, float& e1, float& e2) {
   float ka[7], kb[7];
   ka[0] = k1a;
   kb[0] = k1b;
-  lv_tsit5_stages<ERR>(u1, u2, h, th, ka, kb, n1, n2, e1, e2);
+  tsit5_stages<MODEL, ERR>(u1, u2, h, th, ka, kb, n1, n2, e1, e2);
   k7a = ka[6];
   k7b = kb[6];
 }
@@ -328,9 +339,9 @@ __device__ __forceinline__ float root5(float x) {
 }
 
 // Step limit of one adaptive evaluation when par[6] is not given.
-#define LV_MAX_STEPS 10000
+#define ODE_MAX_STEPS 10000
 
-// Lotka-Volterra SSE with Tsit5: fixed steps (dt = par[2], par[3] steps per
+// ODE-model SSE with Tsit5: fixed steps (dt = par[2], par[3] steps per
 // observation, the reference's parameter_estim_ode! with Val(false)) or
 // adaptive steps (par[2] = observation interval, par[3] = initial dt,
 // par[4] = abstol, par[5] = reltol; Val(true)).  Adaptive control: scaled
@@ -342,12 +353,12 @@ __device__ __forceinline__ float root5(float x) {
 // par[6] <= 0), or a step too small to advance t, make the loss +Inf: a
 // parameter vector whose trajectory turns stiff (the whole-vector clamp of
 // the ODE path admits negative rates) would otherwise hold its wavefront for
-// the explicit method's stability-limited steps.  Loss as lv_sse.
-template <bool ADAPTIVE>
-__device__ __forceinline__ float lv_sse_tsit5(const float* th, const ProbArgs& P) {
+// the explicit method's stability-limited steps.  Loss as ode_sse_rk4.
+template <int MODEL, bool ADAPTIVE>
+__device__ __forceinline__ float ode_sse_tsit5(const float* th, const ProbArgs& P) {
   float u1 = P.par[0], u2 = P.par[1];
   float k1a, k1b;
-  lv_rhs(u1, u2, th[0], th[1], th[2], th[3], k1a, k1b);
+  ode_rhs<MODEL>(u1, u2, th[0], th[1], th[2], th[3], k1a, k1b);
   float loss = 0.0f;
   const float* obs = P.obs;
   float t = 0.0f, dt = ADAPTIVE ? P.par[3] : P.par[2];
@@ -357,7 +368,7 @@ __device__ __forceinline__ float lv_sse_tsit5(const float* th, const ProbArgs& P
       if (!ADAPTIVE) {
         for (int s = 0; s < P.steps_per_obs; ++s) {
           float n1, n2, k7a, k7b, e1, e2;
-          lv_tsit5_step<false>(u1, u2, k1a, k1b, dt, th, n1, n2, k7a, k7b, e1, e2);
+          tsit5_step<MODEL, false>(u1, u2, k1a, k1b, dt, th, n1, n2, k7a, k7b, e1, e2);
           u1 = n1;
           u2 = n2;
           k1a = k7a;
@@ -366,13 +377,13 @@ __device__ __forceinline__ float lv_sse_tsit5(const float* th, const ProbArgs& P
       } else {
         const float t_next = (float)k * P.par[2];
         const float atol = P.par[4], rtol = P.par[5];
-        const int32_t max_steps = P.par[6] > 0.0f ? (int32_t)P.par[6] : LV_MAX_STEPS;
+        const int32_t max_steps = P.par[6] > 0.0f ? (int32_t)P.par[6] : ODE_MAX_STEPS;
         while (t < t_next) {
           const bool hit = !(t + dt < t_next);
           const float h = hit ? t_next - t : dt;
           if (++steps > max_steps || !(t + h > t)) return __builtin_inff();
           float n1, n2, k7a, k7b, e1, e2;
-          lv_tsit5_step<true>(u1, u2, k1a, k1b, h, th, n1, n2, k7a, k7b, e1, e2);
+          tsit5_step<MODEL, true>(u1, u2, k1a, k1b, h, th, n1, n2, k7a, k7b, e1, e2);
           const float a1 = __builtin_fabsf(u1), b1 = __builtin_fabsf(n1);
           const float a2 = __builtin_fabsf(u2), b2 = __builtin_fabsf(n2);
           const float s1 = e1 / (atol + (a1 > b1 ? a1 : b1) * rtol);
@@ -404,18 +415,18 @@ __device__ __forceinline__ float lv_sse_tsit5(const float* th, const ProbArgs& P
   return loss;
 }
 
-// Lotka-Volterra SSE with Tsit5 and saveat semantics: observation k at
+// ODE-model SSE with Tsit5 and saveat semantics: observation k at
 // t_k = k * par[2] (FP32 product), steps of par[3] (the initial step when
 // ADAPTIVE) from t = 0, the last step shortened to end on t_end = t_{n_obs-1};
 // after each accepted step [t, t + h] every unsaved t_k <= t + h is read from
 // the continuous extension at theta = (t_k - t) / h.  Adaptive control, step
-// limit par[6] and the +Inf results as lv_sse_tsit5; fixed steps carry no
-// limit (the host bounds their number).  Loss as lv_sse.
-template <bool ADAPTIVE>
-__device__ __forceinline__ float lv_sse_tsit5_saveat(const float* th, const ProbArgs& P) {
+// limit par[6] and the +Inf results as ode_sse_tsit5; fixed steps carry no
+// limit (the host bounds their number).  Loss as ode_sse_rk4.
+template <int MODEL, bool ADAPTIVE>
+__device__ __forceinline__ float ode_sse_tsit5_saveat(const float* th, const ProbArgs& P) {
   float u1 = P.par[0], u2 = P.par[1];
   float ka[7], kb[7];
-  lv_rhs(u1, u2, th[0], th[1], th[2], th[3], ka[0], kb[0]);
+  ode_rhs<MODEL>(u1, u2, th[0], th[1], th[2], th[3], ka[0], kb[0]);
   const float* obs = P.obs;
   const float tau = P.par[2];
   const float t_end = (float)(P.n_obs - 1) * tau;
@@ -425,13 +436,13 @@ __device__ __forceinline__ float lv_sse_tsit5_saveat(const float* th, const Prob
   float t = 0.0f, dt = P.par[3];
   int32_t steps = 0;
   const int32_t max_steps =
-      ADAPTIVE ? (P.par[6] > 0.0f ? (int32_t)P.par[6] : LV_MAX_STEPS) : 0x7fffffff;
+      ADAPTIVE ? (P.par[6] > 0.0f ? (int32_t)P.par[6] : ODE_MAX_STEPS) : 0x7fffffff;
   while (k < P.n_obs) {
     const bool last = !(t + dt < t_end);
     const float h = last ? t_end - t : dt;
     if (++steps > max_steps || !(t + h > t)) return __builtin_inff();
     float n1, n2, e1, e2;
-    lv_tsit5_stages<ADAPTIVE>(u1, u2, h, th, ka, kb, n1, n2, e1, e2);
+    tsit5_stages<MODEL, ADAPTIVE>(u1, u2, h, th, ka, kb, n1, n2, e1, e2);
     if (ADAPTIVE) {
       const float a1 = __builtin_fabsf(u1), b1 = __builtin_fabsf(n1);
       const float a2 = __builtin_fabsf(u2), b2 = __builtin_fabsf(n2);
@@ -529,16 +540,13 @@ __device__ __forceinline__ float objective(const float (&xin)[MAXD], const int d
     const float a = P.par[0] - x1;
     const float b = x2 - x1 * x1;
     return a * a + P.par[1] * (b * b);
-  } else if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA) {
-    return lv_sse(z, P);
-  } else if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA_TSIT5) {
-    return lv_sse_tsit5<false>(z, P);
-  } else if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5) {
-    return lv_sse_tsit5<true>(z, P);
-  } else if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT) {
-    return lv_sse_tsit5_saveat<false>(z, P);
-  } else if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT) {
-    return lv_sse_tsit5_saveat<true>(z, P);
+  } else if constexpr (is_ode_objective(OBJ)) {
+    constexpr int M = ode_model(OBJ), S = ode_solver(OBJ);
+    if constexpr (S == PPS_ODE_RK4) return ode_sse_rk4<M>(z, P);
+    else if constexpr (S == PPS_ODE_TSIT5) return ode_sse_tsit5<M, false>(z, P);
+    else if constexpr (S == PPS_ODE_ATSIT5) return ode_sse_tsit5<M, true>(z, P);
+    else if constexpr (S == PPS_ODE_TSIT5_SAVEAT) return ode_sse_tsit5_saveat<M, false>(z, P);
+    else return ode_sse_tsit5_saveat<M, true>(z, P);
   } else {
     return __uint_as_float(0x7FC00000u);
   }
@@ -626,7 +634,7 @@ __device__ __forceinline__ void velocity_position(
   }
   // The lexicographic clamp is the ODE path's (src/ode_pso.jl:18-19): it is
   // compiled only into the Lotka-Volterra kernels.
-  if (!is_lotka_volterra(OBJ) || P.clamp == PPS_CLAMP_COMPONENT) {
+  if (!is_ode_objective(OBJ) || P.clamp == PPS_CLAMP_COMPONENT) {
 #pragma unroll
     for (int j = 0; j < MAXD; ++j) {
       if (j < d) x[j] = jl_min(jl_max(x[j], P.lb[j]), P.ub[j]);

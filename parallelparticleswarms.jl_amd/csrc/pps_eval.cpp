@@ -16,14 +16,24 @@ double cos2pi_f64(double x) {
   return std::cos(2.0 * M_PI * r);
 }
 
-double lv_sse_f64(const double* th, const pps_problem* P) {
+// Right-hand sides of the ODE estimation models (Lotka-Volterra,
+// FitzHugh-Nagumo with rates (a, b, tau_inv, l)).
+void ode_rhs_f64(int model, const double* th, double u1, double u2, double& f1, double& f2) {
+  if (model == PPS_ODE_FITZHUGH_NAGUMO) {
+    f1 = u1 - u1 * u1 * u1 / 3.0 - u2 + th[3];
+    f2 = th[2] * (u1 + th[0] - th[1] * u2);
+  } else {
+    f1 = th[0] * u1 - th[1] * u1 * u2;
+    f2 = -th[2] * u2 + th[3] * u1 * u2;
+  }
+}
+
+double ode_sse_rk4_f64(const double* th, const pps_problem* P) {
   double u1 = P->params[0], u2 = P->params[1];
   const double dt = P->params[2];
   const int spo = (int)P->params[3];
-  auto rhs = [&](double a, double b, double& f1, double& f2) {
-    f1 = th[0] * a - th[1] * a * b;
-    f2 = -th[2] * b + th[3] * a * b;
-  };
+  const int model = ode_model(P->objective);
+  auto rhs = [&](double a, double b, double& f1, double& f2) { ode_rhs_f64(model, th, a, b, f1, f2); };
   double loss = 0.0;
   for (int k = 0; k < P->n_obs; ++k) {
     if (k > 0) {
@@ -70,11 +80,9 @@ const double TSR[7][4] = {{1.0, -2.763706197274826, 2.9132554618219126, -1.05308
 // Tsitouras 5(4) in FP64, fixed or adaptive (same controller as the device's
 // lv_sse_tsit5: RMS scaled error, factor EEst^(1/5)/0.9 in [0.1, 5], steps
 // ending on the observation times).
-double lv_sse_tsit5_f64(const double* th, const pps_problem* P, bool adaptive) {
-  auto rhs = [&](double a, double b, double& f1, double& f2) {
-    f1 = th[0] * a - th[1] * a * b;
-    f2 = -th[2] * b + th[3] * a * b;
-  };
+double ode_sse_tsit5_f64(const double* th, const pps_problem* P, bool adaptive) {
+  const int model = ode_model(P->objective);
+  auto rhs = [&](double a, double b, double& f1, double& f2) { ode_rhs_f64(model, th, a, b, f1, f2); };
   double u[2] = {P->params[0], P->params[1]}, k[7][2];
   rhs(u[0], u[1], k[0][0], k[0][1]);
   auto step = [&](double h, double* un, double* err) {
@@ -138,11 +146,9 @@ double lv_sse_tsit5_f64(const double* th, const pps_problem* P, bool adaptive) {
 // Tsit5 in FP64 with saveat semantics (as the device's lv_sse_tsit5_saveat):
 // free steps from t = 0, the last ending on t_end = (n_obs - 1) * params[2],
 // observations inside a step read from the continuous extension.
-double lv_sse_tsit5_saveat_f64(const double* th, const pps_problem* P, bool adaptive) {
-  auto rhs = [&](double a, double b, double& f1, double& f2) {
-    f1 = th[0] * a - th[1] * a * b;
-    f2 = -th[2] * b + th[3] * a * b;
-  };
+double ode_sse_tsit5_saveat_f64(const double* th, const pps_problem* P, bool adaptive) {
+  const int model = ode_model(P->objective);
+  auto rhs = [&](double a, double b, double& f1, double& f2) { ode_rhs_f64(model, th, a, b, f1, f2); };
   double u[2] = {P->params[0], P->params[1]}, k[7][2];
   rhs(u[0], u[1], k[0][0], k[0][1]);
   const double tau = P->params[2], t_end = (P->n_obs - 1) * tau;
@@ -250,16 +256,15 @@ double eval_one(const pps_problem* P, const double* xin) {
       const double a = p[0] - z[0], b = z[1] - z[0] * z[0];
       return a * a + p[1] * b * b;
     }
-    case PPS_OBJ_LOTKA_VOLTERRA:
-      return lv_sse_f64(z.data(), P);
-    case PPS_OBJ_LOTKA_VOLTERRA_TSIT5:
-      return lv_sse_tsit5_f64(z.data(), P, false);
-    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5:
-      return lv_sse_tsit5_f64(z.data(), P, true);
-    case PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT:
-      return lv_sse_tsit5_saveat_f64(z.data(), P, false);
-    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT:
-      return lv_sse_tsit5_saveat_f64(z.data(), P, true);
+    default:
+      if (!is_ode_objective(P->objective)) break;
+      switch (ode_solver(P->objective)) {
+        case PPS_ODE_RK4: return ode_sse_rk4_f64(z.data(), P);
+        case PPS_ODE_TSIT5: return ode_sse_tsit5_f64(z.data(), P, false);
+        case PPS_ODE_ATSIT5: return ode_sse_tsit5_f64(z.data(), P, true);
+        case PPS_ODE_TSIT5_SAVEAT: return ode_sse_tsit5_saveat_f64(z.data(), P, false);
+        default: return ode_sse_tsit5_saveat_f64(z.data(), P, true);
+      }
   }
   return NAN;
 }
@@ -268,12 +273,9 @@ double eval_one(const pps_problem* P, const double* xin) {
 
 extern "C" int pps_eval_f64(const pps_problem* prob, int64_t n, const double* x, double* f) {
   if (!prob || (n > 0 && (!x || !f)) || n < 0 || prob->d < 1 || prob->d > 64) return PPS_E_INVALID;
-  const bool lv = is_lotka_volterra(prob->objective);
-  const int need = prob->objective == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5 ||
-                           prob->objective == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT
-                       ? 6
-                       : 4;
-  if (lv && (prob->d != 4 || !prob->obs || prob->n_obs < 1 || !prob->params ||
+  const bool ode = is_ode_objective(prob->objective);
+  const int need = ode && ode_adaptive(prob->objective) ? 6 : 4;
+  if (ode && (prob->d != 4 || !prob->obs || prob->n_obs < 1 || !prob->params ||
              prob->n_params < need))
     return PPS_E_INVALID;
   for (int64_t i = 0; i < n; ++i) f[i] = eval_one(prob, x + i * prob->d);

@@ -89,7 +89,7 @@ const KernelSet* kernels_rosenbrock(int d, bool constrained);
 const KernelSet* kernels_rastrigin(int d, bool constrained);
 const KernelSet* kernels_sphere(int d, bool constrained);
 const KernelSet* kernels_two_dim(int obj, int d);
-const KernelSet* kernels_lotka_volterra(int obj, int d);
+const KernelSet* kernels_ode(int obj, int d);
 
 const KernelSet* find_kernels(int obj, int d, bool constrained) {
   if (d < 1 || d > PPS_MAXD) return nullptr;
@@ -97,13 +97,9 @@ const KernelSet* find_kernels(int obj, int d, bool constrained) {
     case PPS_OBJ_ROSENBROCK: return kernels_rosenbrock(d, constrained);
     case PPS_OBJ_RASTRIGIN: return kernels_rastrigin(d, constrained);
     case PPS_OBJ_SPHERE: return kernels_sphere(d, constrained);
-    case PPS_OBJ_LOTKA_VOLTERRA:
-    case PPS_OBJ_LOTKA_VOLTERRA_TSIT5:
-    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5:
-    case PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT:
-    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT:
-      return constrained ? nullptr : kernels_lotka_volterra(obj, d);
-    default: return constrained ? nullptr : kernels_two_dim(obj, d);
+    default:
+      if (constrained) return nullptr;
+      return is_ode_objective(obj) ? kernels_ode(obj, d) : kernels_two_dim(obj, d);
   }
 }
 

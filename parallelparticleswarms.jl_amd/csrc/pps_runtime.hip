@@ -82,25 +82,19 @@ int fill_prob_args(const pps_problem* prob, ProbArgs* P) {
     case PPS_OBJ_ROSENBROCK:
     case PPS_OBJ_ROSEN2: need_par = 2; break;
     case PPS_OBJ_QUADRATIC: need_par = 1; break;
-    case PPS_OBJ_LOTKA_VOLTERRA:
-    case PPS_OBJ_LOTKA_VOLTERRA_TSIT5: need_par = 4; break;
-    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5:
-    case PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT: need_par = 6; break;
-    case PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT: need_par = 4; break;
     default: break;
   }
+  if (is_ode_objective(prob->objective)) need_par = ode_adaptive(prob->objective) ? 6 : 4;
   if (prob->n_params < need_par)
     return fail(PPS_E_INVALID, "objective %d needs %d parameters, got %d", prob->objective,
                 need_par, prob->n_params);
-  if (is_lotka_volterra(prob->objective)) {
-    if (d != 4) return fail(PPS_E_INVALID, "Lotka-Volterra estimation has 4 parameters, d=%d", d);
-    if (!prob->obs || prob->n_obs < 1) return fail(PPS_E_INVALID, "Lotka-Volterra needs observations");
+  if (is_ode_objective(prob->objective)) {
+    if (d != 4) return fail(PPS_E_INVALID, "ODE estimation has 4 rates, d=%d", d);
+    if (!prob->obs || prob->n_obs < 1) return fail(PPS_E_INVALID, "ODE estimation needs observations");
     P->n_obs = prob->n_obs;
     const int obj = prob->objective;
-    const bool adaptive =
-        obj == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5 || obj == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT;
-    const bool saveat =
-        obj == PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT || obj == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT;
+    const bool adaptive = ode_adaptive(obj);
+    const bool saveat = ode_saveat(obj);
     if (adaptive || saveat) {
       const float* q = prob->params;
       if (!(q[2] > 0.0f && std::isfinite((float)(prob->n_obs - 1) * q[2])))
@@ -434,10 +428,10 @@ static int swarm_create(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_p
     return fail(PPS_E_INVALID, "unknown gbest_from %d", prm->gbest_from);
   if (prm->clamp != PPS_CLAMP_COMPONENT && prm->clamp != PPS_CLAMP_LEXICOGRAPHIC)
     return fail(PPS_E_INVALID, "unknown clamp %d", prm->clamp);
-  if (prm->clamp == PPS_CLAMP_LEXICOGRAPHIC && prob && !is_lotka_volterra(prob->objective))
+  if (prm->clamp == PPS_CLAMP_LEXICOGRAPHIC && prob && !is_ode_objective(prob->objective))
     return fail(PPS_E_INVALID,
                 "lexicographic clamping is the ODE estimation path's; it needs "
-                "PPS_OBJ_LOTKA_VOLTERRA (objective %d given)", prob->objective);
+                "an ODE objective (objective %d given)", prob->objective);
   if (prm->grid_per_cu < 0 || prm->grid_per_cu > 64)
     return fail(PPS_E_INVALID, "grid_per_cu %d outside 0..64", prm->grid_per_cu);
   pps_swarm* s = new pps_swarm();

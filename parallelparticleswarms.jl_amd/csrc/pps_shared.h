@@ -2,11 +2,30 @@
 #pragma once
 #include "pps.h"
 
-// The Lotka-Volterra estimation objectives (four rates, observations).
-constexpr bool is_lotka_volterra(int obj) {
-  return obj == PPS_OBJ_LOTKA_VOLTERRA || obj == PPS_OBJ_LOTKA_VOLTERRA_TSIT5 ||
-         obj == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5 || obj == PPS_OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT ||
-         obj == PPS_OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT;
+// ODE estimation objectives (two-state model, four rates, observations): a
+// model (Lotka-Volterra 16..20, FitzHugh-Nagumo 21..25) times a solver, in
+// the same order for every model.
+#define PPS_ODE_LOTKA_VOLTERRA 0
+#define PPS_ODE_FITZHUGH_NAGUMO 1
+#define PPS_ODE_RK4 0
+#define PPS_ODE_TSIT5 1
+#define PPS_ODE_ATSIT5 2
+#define PPS_ODE_TSIT5_SAVEAT 3
+#define PPS_ODE_ATSIT5_SAVEAT 4
+constexpr bool is_ode_objective(int obj) {
+  return obj >= PPS_OBJ_LOTKA_VOLTERRA && obj <= PPS_OBJ_FITZHUGH_NAGUMO_ATSIT5_SAVEAT;
+}
+constexpr int ode_model(int obj) {
+  return obj >= PPS_OBJ_FITZHUGH_NAGUMO ? PPS_ODE_FITZHUGH_NAGUMO : PPS_ODE_LOTKA_VOLTERRA;
+}
+constexpr int ode_solver(int obj) {
+  return obj - (obj >= PPS_OBJ_FITZHUGH_NAGUMO ? PPS_OBJ_FITZHUGH_NAGUMO : PPS_OBJ_LOTKA_VOLTERRA);
+}
+constexpr bool ode_adaptive(int obj) {
+  return ode_solver(obj) == PPS_ODE_ATSIT5 || ode_solver(obj) == PPS_ODE_ATSIT5_SAVEAT;
+}
+constexpr bool ode_saveat(int obj) {
+  return ode_solver(obj) == PPS_ODE_TSIT5_SAVEAT || ode_solver(obj) == PPS_ODE_ATSIT5_SAVEAT;
 }
 
 #define PPS_MAXD 64    // largest supported dimension

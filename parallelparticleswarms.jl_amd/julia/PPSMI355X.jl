@@ -325,20 +325,24 @@ end
 # decouples the steps from the samples, which are read from Tsit5's
 # interpolant as GPUTsit5 fills saveat (the reference's benchmark: dt = 0.1,
 # saveat = 0.01, benchmarks/Lotka_Volterra/lotka_volterra.jl:124-127).
+# `model` :lotka_volterra (rates α, β, γ, δ) or :fitzhugh_nagumo (a, b, τ⁻¹, l;
+# benchmarks/Fitzhugh_Nagumo/ode_estimation.jl:6-12).
 # Returns (gbest position, FP32 loss).
 function parameter_estim_lv(data::AbstractMatrix, lb, ub; backend = MI355XBackend(),
         num_particles = 2^14, maxiters = 300, u0 = (1.0f0, 1.0f0), dt = 0.01f0,
         saveat_steps = 10, saveat = nothing, ode_alg = :tsit5, adaptive = false,
         abstol = 1.0f-6, reltol = 1.0f-3, max_steps = 10000, w = 0.7298f0, wdamp = 1.0f0,
-        seed::Integer = rand(UInt64))
+        model = :lotka_volterra, seed::Integer = rand(UInt64))
     (ode_alg === :rk4 && adaptive) && throw(ArgumentError("adaptive stepping uses :tsit5"))
     (ode_alg === :rk4 && saveat !== nothing) &&
         throw(ArgumentError("saveat between steps uses :tsit5's interpolant"))
+    base = model === :lotka_volterra ? Int32(16) : model === :fitzhugh_nagumo ? Int32(21) :
+           throw(ArgumentError("model is :lotka_volterra or :fitzhugh_nagumo"))
     if saveat !== nothing
-        code = adaptive ? Int32(20) : Int32(19)
+        code = base + (adaptive ? Int32(4) : Int32(3))
         par = Float32[u0[1], u0[2], saveat, dt, abstol, reltol, max_steps]
     else
-        code = ode_alg === :rk4 ? Int32(16) : adaptive ? Int32(18) : Int32(17)
+        code = base + (ode_alg === :rk4 ? Int32(0) : adaptive ? Int32(2) : Int32(1))
         par = adaptive ?
               Float32[u0[1], u0[2], Float32(dt) * saveat_steps, dt, abstol, reltol, max_steps] :
               Float32[u0[1], u0[2], dt, saveat_steps]
@@ -366,6 +370,9 @@ function parameter_estim_lv(data::AbstractMatrix, lb, ub; backend = MI355XBacken
         ccall((:pps_swarm_free, libpps), Cint, (Ptr{Cvoid},), h[])
     end
 end
+
+parameter_estim_fhn(data::AbstractMatrix, lb, ub; kwargs...) =
+    parameter_estim_lv(data, lb, ub; model = :fitzhugh_nagumo, kwargs...)
 
 # HybridPSO{MI355XBackend, SimpleLBFGS} (src/hybrid.jl:100-158): PSO, then
 # projected L-BFGS from every particle's best position, keep the better.
@@ -395,6 +402,6 @@ function solve(prob::OptimizationProblem, alg::HybridPSO{<:MI355XBackend, <:Simp
 end
 
 export MI355XBackend, PPSObjective, PPSQuadraticConstraints, PPSLatinHypercube,
-    parameter_estim_lv
+    parameter_estim_lv, parameter_estim_fhn
 
 end # module

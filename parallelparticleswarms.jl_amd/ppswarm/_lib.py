@@ -22,6 +22,10 @@ OBJ_ROSENBROCK, OBJ_RASTRIGIN, OBJ_SPHERE, OBJ_BEALE, OBJ_BOOTH, OBJ_HIMMELBLAU 
 OBJ_QUADRATIC, OBJ_BOUNDARY_QUADRATIC, OBJ_ROSEN2, OBJ_LOTKA_VOLTERRA = 7, 8, 9, 16
 OBJ_LOTKA_VOLTERRA_TSIT5, OBJ_LOTKA_VOLTERRA_ATSIT5 = 17, 18
 OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT, OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT = 19, 20
+# FitzHugh-Nagumo: the same five solvers at 21..25
+OBJ_FITZHUGH_NAGUMO = 21
+ODE_MODEL_BASE = {"lotka_volterra": OBJ_LOTKA_VOLTERRA, "fitzhugh_nagumo": OBJ_FITZHUGH_NAGUMO}
+ODE_RK4, ODE_TSIT5, ODE_ATSIT5, ODE_TSIT5_SAVEAT, ODE_ATSIT5_SAVEAT = 0, 1, 2, 3, 4
 
 MODE_SYNC, MODE_ASYNC, MODE_QLOCK = 0, 1, 2
 GBEST_PBEST, GBEST_CURRENT = 0, 1

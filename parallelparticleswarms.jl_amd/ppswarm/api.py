@@ -152,10 +152,12 @@ def Rosen2():
 
 
 @dataclass(frozen=True)
-class LotkaVolterraSSE(Objective):
-    """Sum of squared errors of an FP32 Lotka-Volterra trajectory against
-    observations (the loss of src/ode_pso.jl:47-52).  The decision variable is
-    (α, β, γ, δ); ``obs`` is (n_obs, 2), sample k taken at t = k * dt *
+class OdeSSE(Objective):
+    """Sum of squared errors of an FP32 ODE trajectory against observations
+    (the loss of src/ode_pso.jl:47-52) for a two-state model with four rates:
+    Lotka-Volterra (α, β, γ, δ; benchmarks/Lotka_Volterra/lotka_volterra.jl:9-13)
+    or FitzHugh-Nagumo (a, b, τ⁻¹, l; benchmarks/Fitzhugh_Nagumo/ode_estimation.jl:
+    6-12).  ``obs`` is (n_obs, 2), sample k taken at t = k * dt *
     steps_per_obs starting from ``u0`` at sample 0.  Solved in the kernel with
     fixed-step RK4 (``ode_alg="rk4"``), fixed-step Tsit5 (``"tsit5"``, the
     reference's GPUTsit5 with Val(false)) or adaptive Tsit5 (``"tsit5",
@@ -176,6 +178,7 @@ class LotkaVolterraSSE(Objective):
     reltol: float = 1e-3
     maxiters: int = 10000
     saveat: Optional[float] = None
+    model: str = "lotka_volterra"
 
     def params(self) -> np.ndarray:
         if self.saveat is not None:
@@ -188,9 +191,14 @@ class LotkaVolterraSSE(Objective):
         return np.array([self.u0[0], self.u0[1], self.dt, self.steps_per_obs], np.float32)
 
 
-def lotka_volterra_sse(obs, u0=(1.0, 1.0), dt=0.01, steps_per_obs=10, *, ode_alg="rk4",
-                       adaptive=False, abstol=1e-6, reltol=1e-3, maxiters=10000,
-                       saveat=None) -> LotkaVolterraSSE:
+LotkaVolterraSSE = OdeSSE
+
+
+def ode_sse(model, obs, u0=(1.0, 1.0), dt=0.01, steps_per_obs=10, *, ode_alg="rk4",
+            adaptive=False, abstol=1e-6, reltol=1e-3, maxiters=10000, saveat=None) -> OdeSSE:
+    """The SSE objective of ``model`` ("lotka_volterra" or "fitzhugh_nagumo")."""
+    if model not in L.ODE_MODEL_BASE:
+        raise ValueError(f"model is one of {sorted(L.ODE_MODEL_BASE)}, got {model!r}")
     obs = np.ascontiguousarray(obs, np.float32).reshape(-1, 2)
     if ode_alg not in ("rk4", "tsit5"):
         raise ValueError(f"ode_alg is 'rk4' or 'tsit5', got {ode_alg!r}")
@@ -199,15 +207,22 @@ def lotka_volterra_sse(obs, u0=(1.0, 1.0), dt=0.01, steps_per_obs=10, *, ode_alg
     if saveat is not None and ode_alg != "tsit5":
         raise ValueError("saveat between steps uses Tsit5's interpolant (ode_alg='tsit5')")
     if saveat is not None:
-        code = (L.OBJ_LOTKA_VOLTERRA_ATSIT5_SAVEAT if adaptive
-                else L.OBJ_LOTKA_VOLTERRA_TSIT5_SAVEAT)
+        solver = L.ODE_ATSIT5_SAVEAT if adaptive else L.ODE_TSIT5_SAVEAT
         saveat = float(saveat)
     else:
-        code = (L.OBJ_LOTKA_VOLTERRA if ode_alg == "rk4" else
-                L.OBJ_LOTKA_VOLTERRA_ATSIT5 if adaptive else L.OBJ_LOTKA_VOLTERRA_TSIT5)
-    return LotkaVolterraSSE(code, "lotka_volterra", 4, 6 if adaptive else 4, obs, tuple(u0),
-                            float(dt), int(steps_per_obs), bool(adaptive), float(abstol),
-                            float(reltol), int(maxiters), saveat)
+        solver = (L.ODE_RK4 if ode_alg == "rk4" else
+                  L.ODE_ATSIT5 if adaptive else L.ODE_TSIT5)
+    return OdeSSE(L.ODE_MODEL_BASE[model] + solver, model, 4, 6 if adaptive else 4, obs,
+                  tuple(u0), float(dt), int(steps_per_obs), bool(adaptive), float(abstol),
+                  float(reltol), int(maxiters), saveat, model)
+
+
+def lotka_volterra_sse(obs, u0=(1.0, 1.0), dt=0.01, steps_per_obs=10, **kw) -> OdeSSE:
+    return ode_sse("lotka_volterra", obs, u0, dt, steps_per_obs, **kw)
+
+
+def fitzhugh_nagumo_sse(obs, u0=(1.0, 1.0), dt=0.01, steps_per_obs=10, **kw) -> OdeSSE:
+    return ode_sse("fitzhugh_nagumo", obs, u0, dt, steps_per_obs, **kw)
 
 
 # ---------------------------------------------------------------------------
@@ -299,7 +314,7 @@ class OptimizationProblem:
         self.d = int(self.u0.size)
         if f.fixed_dim is not None and self.d != f.fixed_dim:
             raise ValueError(f"{f.name} is {f.fixed_dim}-dimensional, u0 has {self.d}")
-        if isinstance(f, LotkaVolterraSSE):
+        if isinstance(f, OdeSSE):
             self.p = f.params()
         elif p is None:
             self.p = np.zeros(0, np.float32)
@@ -339,7 +354,7 @@ class OptimizationProblem:
         pr.n_params = int(self.p.size)
         pr.shift = ptr(self.shift)
         pr.u0 = ptr(self.u0)
-        if isinstance(self.f, LotkaVolterraSSE):
+        if isinstance(self.f, OdeSSE):
             pr.obs = ptr(self.f.obs)
             pr.n_obs = int(np.asarray(self.f.obs).size // 2)
         if self.cons is not None:

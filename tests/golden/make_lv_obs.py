@@ -10,6 +10,11 @@ Also lv_obs_benchmark.json, the data of the reference's Lotka-Volterra benchmark
 t in [0, 3] sampled every 0.01, 301 points, noise-free from a tight-tolerance
 solve): here FP64 RK4 with 20 steps of 0.0005 between samples (global error
 ~1e-12).
+
+And fhn_obs_benchmark.json, the data of the reference's FitzHugh-Nagumo benchmark
+(benchmarks/Fitzhugh_Nagumo/ode_estimation.jl:6-38): dv = v - v^3/3 - w + l,
+dw = tau_inv (v + a - b w), (a, b, tau_inv, l) = (0.7, 0.8, 0.08, 0.5), u0 = (1, 1),
+t in [0, 3] sampled every 0.01 (301 points), noise-free; FP64 RK4 as above.
 """
 import json
 import os
@@ -25,9 +30,17 @@ SIGMA = 0.05
 SEED = 20240601
 
 
-def rk4_trajectory(theta=THETA, u0=U0, dt=DT, spo=STEPS_PER_OBS, n_obs=N_OBS):
+FHN_THETA = (0.7, 0.8, 0.08, 0.5)
+
+
+def fhn_rhs(theta):
+    a, b, tinv, l = theta
+    return lambda u: np.array([u[0] - u[0] ** 3 / 3 - u[1] + l, tinv * (u[0] + a - b * u[1])])
+
+
+def rk4_trajectory(theta=THETA, u0=U0, dt=DT, spo=STEPS_PER_OBS, n_obs=N_OBS, rhs=None):
     a, b, c, d = theta
-    f = lambda u: np.array([a * u[0] - b * u[0] * u[1], -c * u[1] + d * u[0] * u[1]])
+    f = rhs or (lambda u: np.array([a * u[0] - b * u[0] * u[1], -c * u[1] + d * u[0] * u[1]]))
     u = np.array(u0, np.float64)
     out = [u.copy()]
     for _ in range(n_obs - 1):
@@ -56,6 +69,13 @@ def main():
     doc = {"theta": THETA, "u0": U0, "saveat": 0.01, "t_end": 3.0, "integrator":
            "FP64 RK4, dt = 0.0005", "obs": bench.tolist()}
     path = os.path.join(here, "lv_obs_benchmark.json")
+    with open(path, "w") as fh:
+        json.dump(doc, fh)
+    print(f"wrote {path}")
+    fhn = rk4_trajectory(FHN_THETA, (1.0, 1.0), 0.0005, 20, 301, rhs=fhn_rhs(FHN_THETA))
+    doc = {"theta": FHN_THETA, "u0": (1.0, 1.0), "saveat": 0.01, "t_end": 3.0,
+           "integrator": "FP64 RK4, dt = 0.0005", "obs": fhn.tolist()}
+    path = os.path.join(here, "fhn_obs_benchmark.json")
     with open(path, "w") as fh:
         json.dump(doc, fh)
     print(f"wrote {path}")

@@ -131,3 +131,54 @@ def test_reference_lv_benchmark_shape():
                                     ode_alg="tsit5", dt=0.1, saveat=0.01)
     np.testing.assert_allclose(sol.u, doc["theta"], rtol=1e-3)
     assert sol.objective < 1e-5
+
+
+FHN_OBS = np.array(json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                               "fhn_obs_benchmark.json")))["obs"], np.float32)
+
+
+@pytest.mark.parametrize("solver,kw", [
+    (O.FITZHUGH_NAGUMO, dict(ode_alg="rk4", dt=0.01, steps_per_obs=10)),
+    (O.FITZHUGH_NAGUMO_TSIT5, dict(ode_alg="tsit5", dt=0.05, steps_per_obs=2)),
+    (O.FITZHUGH_NAGUMO_ATSIT5, dict(ode_alg="tsit5", adaptive=True, dt=0.01, steps_per_obs=10)),
+    (O.FITZHUGH_NAGUMO_TSIT5_SAVEAT, dict(ode_alg="tsit5", dt=0.1, saveat=0.1)),
+    (O.FITZHUGH_NAGUMO_ATSIT5_SAVEAT, dict(ode_alg="tsit5", adaptive=True, dt=0.01, saveat=0.1)),
+])
+@pytest.mark.parametrize("mode,n", [(L.MODE_SYNC, 3000), (L.MODE_ASYNC, 1000)])
+def test_fhn_parity(solver, kw, mode, n):
+    # FitzHugh-Nagumo (benchmarks/Fitzhugh_Nagumo/ode_estimation.jl:6-12) with
+    # every solver: samples 0.1 apart on [0, 3]
+    note(f"FHN parity obj={solver} mode={mode} n={n}")
+    obs = FHN_OBS[::10]
+    f = P.fitzhugh_nagumo_sse(obs, u0=(1.0, 1.0), **kw)
+    assert f.code == solver
+    lb, ub = np.zeros(4, np.float32), np.full(4, 5.0, np.float32)
+    prob = P.OptimizationProblem(f, np.zeros(4), lb=lb, ub=ub)
+    g = P.Swarm(P.context(), prob, P.swarm_params(n, mode, seed=71 + solver,
+                                                  clamp=L.CLAMP_LEXICOGRAPHIC,
+                                                  launch=L.LAUNCH_GRAPH if n > 1024 else L.LAUNCH_AUTO))
+    o = O.Swarm(O.Problem(solver, 4, lb, ub, f.params(), clamp=O.CLAMP_LEXICOGRAPHIC, obs=obs), n,
+                71 + solver)
+    assert_same_state(g, o, "after init")
+    g.run(3)
+    if mode == L.MODE_SYNC:
+        o.run_sync(3)
+    else:
+        o.run_async(3)
+    assert_same_state(g, o, "after 3 iterations")
+    assert_same_gbest(g, o)
+
+
+def test_reference_fhn_benchmark_shape():
+    # benchmarks/Fitzhugh_Nagumo/ode_estimation.jl:100-140: 10000 particles, 100
+    # iterations, rates in [0, 5], GPUTsit5 dt = 0.1 with saveat every 0.01 on
+    # [0, 3] against noise-free data.  Three time units determine l sharply;
+    # a, b and tau_inv act through the slow variable and are only loosely
+    # determined after 100 iterations, so the fit is judged by its loss
+    note("FHN reference benchmark shape (saveat 0.01, dt 0.1)")
+    doc = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "fhn_obs_benchmark.json")))
+    sol, sw = P.parameter_estim_ode(FHN_OBS, np.zeros(4), np.full(4, 5.0), 10000, 100, seed=0,
+                                    ode_alg="tsit5", dt=0.1, saveat=0.01, model="fitzhugh_nagumo")
+    assert sol.objective < 1e-4
+    assert abs(sol.u[3] / doc["theta"][3] - 1) < 1e-3
