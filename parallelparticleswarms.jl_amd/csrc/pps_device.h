@@ -31,9 +31,10 @@ __device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
     const uint64_t p0 = (uint64_t)PHILOX_M0 * c.x;
     const uint64_t p1 = (uint64_t)PHILOX_M1 * c.z;
     u4 n;
-    n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+    // three-input xor in one v_bitop3_b32 (truth table 0x96)
+    n.x = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c.y, k0, 0x96);
     n.y = (uint32_t)p1;
-    n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+    n.z = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c.w, k1, 0x96);
     n.w = (uint32_t)p0;
     c = n;
     k0 += PHILOX_W0;
@@ -49,18 +50,16 @@ __device__ __forceinline__ float u01(uint32_t u) {
 
 // ---------------------------------------------------------------------------
 // Julia's Base.max / Base.min for IEEE floats: NaN-propagating, and
-// max(-0.0, +0.0) = +0.0, min(-0.0, +0.0) = -0.0.  AMD v_max_f32 / v_min_f32
-// implement maxNum/minNum instead, so the selection is written out.
+// max(-0.0, +0.0) = +0.0, min(-0.0, +0.0) = -0.0.  These are IEEE 754-2019
+// maximum / minimum, which gfx950 has as single instructions
+// (v_maximum3_f32 / v_minimum3_f32); its v_max_f32 / v_min_f32 are maxNum /
+// minNum (a NaN operand loses) and would not do.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float jl_max(float x, float y) {
-  const float diff = x - y;
-  const float pick = __builtin_signbit(diff) ? y : x;
-  return (__builtin_isnan(x) | __builtin_isnan(y)) ? diff : pick;
+  return __builtin_elementwise_maximum(x, y);
 }
 __device__ __forceinline__ float jl_min(float x, float y) {
-  const float diff = x - y;
-  const float pick = __builtin_signbit(diff) ? x : y;
-  return (__builtin_isnan(x) | __builtin_isnan(y)) ? diff : pick;
+  return __builtin_elementwise_minimum(x, y);
 }
 
 // Julia's isless for Float32 (total order: -0.0 < +0.0, NaN above +Inf).
@@ -609,6 +608,9 @@ __device__ float constraint_penalty(const float (&x)[MAXD], const int d, const P
 // and use lanes (x, y, z, w) as (r1[2k], r2[2k], r1[2k+1], r2[2k+1]).
 // ---------------------------------------------------------------------------
 template <int MAXD, int OBJ>
+__device__ __forceinline__ void clamp_position(float (&x)[MAXD], const int d, const ProbArgs& P);
+
+template <int MAXD, int OBJ>
 __device__ __forceinline__ void velocity_position(
     float (&x)[MAXD], float (&v)[MAXD], const float (&pb)[MAXD],
     const float* gb, float w, float c1, float c2, uint32_t i, uint32_t t,
@@ -632,6 +634,51 @@ __device__ __forceinline__ void velocity_position(
       }
     }
   }
+  clamp_position<MAXD, OBJ>(x, d, P);
+}
+
+// The same step split in two: the draws of particle i at iteration t (they
+// depend on neither the state nor the global best, so a kernel can compute
+// them while its loads are in flight), then the update from those draws.
+// Together they compute exactly what velocity_position computes.
+template <int MAXD>
+__device__ __forceinline__ void step_draws(float (&r1)[MAXD], float (&r2)[MAXD], uint32_t i,
+                                           uint32_t t, uint32_t epoch, uint32_t k0, uint32_t k1,
+                                           const int d) {
+#pragma unroll
+  for (int k = 0; 2 * k < MAXD; ++k) {
+    if (2 * k < d) {
+      const u4 r = philox4x32_10(u4{i, t, (uint32_t)k | (PPS_TAG_STEP << 16), epoch}, k0, k1);
+      r1[2 * k] = u01(r.x);
+      r2[2 * k] = u01(r.y);
+      if (2 * k + 1 < MAXD) {
+        r1[2 * k + 1] = u01(r.z);
+        r2[2 * k + 1] = u01(r.w);
+      }
+    }
+  }
+}
+template <int MAXD, int OBJ>
+__device__ __forceinline__ void velocity_position_drawn(
+    float (&x)[MAXD], float (&v)[MAXD], const float (&pb)[MAXD], const float (&gb)[MAXD],
+    float w, float c1, float c2, const float (&r1)[MAXD], const float (&r2)[MAXD], const int d,
+    const ProbArgs& P) {
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) {
+    if (j < d) {
+      const float a = w * v[j];
+      const float b = (c1 * r1[j]) * (pb[j] - x[j]);
+      const float c = (c2 * r2[j]) * (gb[j] - x[j]);
+      v[j] = (a + b) + c;
+      x[j] = x[j] + v[j];
+    }
+  }
+  clamp_position<MAXD, OBJ>(x, d, P);
+}
+
+// Bound handling after the position update.
+template <int MAXD, int OBJ>
+__device__ __forceinline__ void clamp_position(float (&x)[MAXD], const int d, const ProbArgs& P) {
   // The lexicographic clamp is the ODE path's (src/ode_pso.jl:18-19): it is
   // compiled only into the Lotka-Volterra kernels.
   if (!is_ode_objective(OBJ) || P.clamp == PPS_CLAMP_COMPONENT) {

@@ -224,39 +224,81 @@ __device__ __forceinline__ void st_state(float* p, float v) {
   else *p = v;
 }
 
+// Start of a launch: the global best of iteration t-1 (a minimum over the G
+// block keys, then the winner's position from its block's candidate) is two
+// dependent round trips to memory.  Nothing of the first tile depends on it
+// but the update itself, so the block keys are requested first, the first
+// tile's state loads are issued behind them, and (MAXD <= 16) the tile's
+// random draws are computed while both are in flight.  At the end the block's
+// winner publishes its position from registers instead of reading back what
+// it stored.
 template <int MAXD, bool EXACT, int OBJ, int BLOCK, bool OFF32, bool NT>
 __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, ProbArgs P,
                                                                    StepArgs A, float w,
                                                                    uint32_t k) {
+  // Both cost registers: they are compiled where the workgroup size leaves
+  // room for them without spilling.
+  constexpr bool PRE = MAXD * BLOCK <= 16 * 256;   // draws computed ahead of the state
+  constexpr bool HOLD = MAXD * BLOCK <= 32 * 256;  // block winner's position in registers
   __shared__ uint64_t s_red[BLOCK / 64];
-  const uint32_t t = *S.tbase + k;
   __shared__ float s_gb[MAXD];
+  const uint32_t t = *S.tbase + k;
   const int d = EXACT ? MAXD : P.d;
-  fetch_gbest<BLOCK>(S, (t - 1u) & 1u, d, s_gb, s_red);
+  const int G = S.G;
+  const int tid = threadIdx.x;
+  const bool cur = A.gbest_from == PPS_GBEST_CURRENT;
+  const uint64_t* keys = S.bkey + (int64_t)((t - 1u) & 1u) * G;
+  uint64_t gk = tid < G ? keys[tid] : KEY_NONE;
+  // The key request goes out before the state loads: loads return in issue
+  // order, so the block reduction can start before the state has arrived.
+  __builtin_amdgcn_sched_barrier(0);
+
+  int64_t tile = blockIdx.x;
+  int64_t i = tile * BLOCK + tid;
+  float x[MAXD], v[MAXD], pb[MAXD], r1[MAXD], r2[MAXD], pc = 0.0f;
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) x[j] = v[j] = pb[j] = r1[j] = r2[j] = 0.0f;
+  auto load_tile = [&]() {
+    const RowAddr<OFF32> R(i, S.npad);
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j) {
+      if (j < d) {
+        x[j] = ld_state<NT>(R.at(S.x, j));
+        v[j] = ld_state<NT>(R.at(S.v, j));
+        pb[j] = ld_state<NT>(R.at(S.pb, j));
+      }
+    }
+    pc = ld_state<NT>(S.pc + i);
+  };
+  if (i < S.n) load_tile();
+  // Keep the loads ahead of everything below.
+  __builtin_amdgcn_sched_barrier(0);
+  if (PRE && i < S.n) step_draws<MAXD>(r1, r2, (uint32_t)i, t, S.epoch, S.key0, S.key1, d);
+  for (int j = tid + BLOCK; j < G; j += BLOCK) gk = umin64(gk, keys[j]);
+  gk = block_min_u64<BLOCK>(gk, s_red);
+  {
+    const uint32_t gidx = (uint32_t)gk;
+    const int gblk = (int)((gidx / (uint32_t)S.block) % (uint32_t)G);
+    if (tid < d) s_gb[tid] = S.cand[((int64_t)((t - 1u) & 1u) * G + gblk) * d + tid];
+  }
+  __syncthreads();
   float gb[MAXD];
 #pragma unroll
   for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
+
   uint64_t mykey = KEY_NONE;
-  for (int64_t tile = blockIdx.x; tile * BLOCK < S.n; tile += gridDim.x) {
-    const int64_t i = tile * BLOCK + threadIdx.x;
+  float hold[MAXD];
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) hold[j] = 0.0f;
+  while (tile * BLOCK < S.n) {
     if (i < S.n) {
       const RowAddr<OFF32> R(i, S.npad);
-      float x[MAXD], v[MAXD], pb[MAXD];
-#pragma unroll
-      for (int j = 0; j < MAXD; ++j) {
-        if (j < d) {
-          x[j] = ld_state<NT>(R.at(S.x, j));
-          v[j] = ld_state<NT>(R.at(S.v, j));
-          pb[j] = ld_state<NT>(R.at(S.pb, j));
-        }
+      if constexpr (PRE) {
+        velocity_position_drawn<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, r1, r2, d, P);
+      } else {
+        velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch,
+                                     S.key0, S.key1, d, P);
       }
-      float pc = ld_state<NT>(S.pc + i);
-      // Issue the whole tile's loads before any arithmetic: left alone, the
-      // scheduler interleaves them with the arithmetic pair by pair and the
-      // wave waits for memory several times per tile.
-      __builtin_amdgcn_sched_barrier(0);
-      velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch, S.key0,
-                                   S.key1, d, P);
       const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2));
 #pragma unroll
       for (int j = 0; j < MAXD; ++j) {
@@ -277,11 +319,39 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
         st_state<NT>(S.pc + i, imp ? c : pc);
       }
       if (imp) pc = c;
-      mykey = umin64(mykey, make_key(pc, (uint32_t)i));
+      const uint64_t key = make_key(pc, (uint32_t)i);
+      if constexpr (HOLD) {
+        // Particles of one thread come in increasing index order, so a
+        // strictly smaller key is the thread's new argmin.
+        const bool win = key < mykey;
+#pragma unroll
+        for (int j = 0; j < MAXD; ++j)
+          if (win) hold[j] = (cur || imp) ? x[j] : pb[j];
+      }
+      mykey = umin64(mykey, key);
+    }
+    tile += gridDim.x;
+    i = tile * BLOCK + tid;
+    if (tile * BLOCK < S.n && i < S.n) {
+      load_tile();
+      // Issue the whole tile's loads before any arithmetic: left alone, the
+      // scheduler interleaves them with the arithmetic pair by pair and the
+      // wave waits for memory several times per tile.
+      __builtin_amdgcn_sched_barrier(0);
+      if (PRE) step_draws<MAXD>(r1, r2, (uint32_t)i, t, S.epoch, S.key0, S.key1, d);
     }
   }
-  publish_block<BLOCK>(S, t & 1u, d, mykey,
-                       A.gbest_from == PPS_GBEST_CURRENT ? S.x : S.pb, s_red);
+  if constexpr (HOLD) {
+    const uint64_t bk = block_min_u64<BLOCK>(mykey, s_red);
+    if (bk != KEY_NONE && mykey == bk) {
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j)
+        if (j < d) S.cand[((int64_t)(t & 1u) * G + blockIdx.x) * d + j] = hold[j];
+    }
+    if (tid == 0) S.bkey[(int64_t)(t & 1u) * G + blockIdx.x] = bk;
+  } else {
+    publish_block<BLOCK>(S, t & 1u, d, mykey, cur ? S.x : S.pb, s_red);
+  }
 }
 
 // Whole sync loop in one workgroup of BLOCK threads, one particle per thread
