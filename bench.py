@@ -28,7 +28,11 @@ Also reported on the same line:
                 Armijo) from every particle; final minimum re-evaluated in FP64.
   ode           Lotka-Volterra parameter estimation: 2^14 particles, 300
                 iterations, in-kernel FP32 RK4 (dt = 0.01, 101 observations);
-                the same with Tsit5 at fixed and adaptive steps.
+                the same with Tsit5 at fixed and adaptive steps.  The
+                adaptive leg is timed as the library runs it (a particle's
+                loss stops once it reaches the particle's best cost) and
+                again with every loss summed in full; both must give the
+                same solution.
   cpu_baseline  the CPU oracle (reference-faithful AoS restatement, OpenMP
                 over particles + serial argmin per iteration) on a bounded
                 sample of the same workload on this host.
@@ -178,6 +182,38 @@ def hybrid_c2(P, L, ctx):
             "pso_best_f32": g.cost, "lbfgs_best_f32": bf, "final_objective_f64": best}
 
 
+def _full_loss(run):
+    """Run `run()` once more with every ODE loss summed in full
+    (PPS_ODE_FULL_LOSS=1): returns (device ms, solution position)."""
+    os.environ["PPS_ODE_FULL_LOSS"] = "1"
+    try:
+        sol, sw = run()
+        ms = sw.elapsed_ms()
+        sw.free()
+    finally:
+        del os.environ["PPS_ODE_FULL_LOSS"]
+    return ms, np.asarray(sol.u)
+
+
+def _ode_leg(run, particles, iters, extra=None, compare_full=False):
+    """One ODE estimation leg.  compare_full (adaptive solvers): also time it
+    with every loss summed in full instead of stopped at each particle's best
+    cost (the library default); the two solutions must be the same bit for
+    bit."""
+    sol, sw = run()
+    ms = sw.elapsed_ms()
+    sw.free()
+    out = {"device_ms": ms, "particle_iters_per_s": particles * iters / (ms / 1e3),
+           "theta": [float(v) for v in sol.u], "loss_f64": sol.objective}
+    if compare_full:
+        full_ms, full_u = _full_loss(run)
+        out["full_loss_device_ms"] = full_ms
+        out["same_solution_as_full_loss"] = bool(np.array_equal(np.asarray(sol.u), full_u))
+    if extra:
+        out.update(extra(sol))
+    return out
+
+
 def ode_c3(P):
     import json as _json
     doc = _json.load(open(os.path.join(ROOT, "tests", "golden", "lv_obs.json")))
@@ -196,41 +232,35 @@ def ode_c3(P):
     # the reference's default solver, GPUTsit5: fixed steps (dt = 0.01) and
     # adaptive steps (abstol 1e-6, reltol 1e-3, initial dt 0.01), same shape
     for key, adaptive in (("tsit5", False), ("tsit5_adaptive", True)):
-        sol, sw = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 1 << 14, 300, seed=3,
-                                        ode_alg="tsit5", adaptive=adaptive)
-        ms = sw.elapsed_ms()
-        sw.free()
-        out[key] = {"device_ms": ms, "particle_iters_per_s": (1 << 14) * 300 / (ms / 1e3),
-                    "theta": [float(v) for v in sol.u], "loss_f64": sol.objective}
+        out[key] = _ode_leg(
+            lambda: P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 1 << 14, 300, seed=3,
+                                          ode_alg="tsit5", adaptive=adaptive), 1 << 14, 300,
+            compare_full=adaptive)
     # the reference's own Lotka-Volterra benchmark shape (10000 particles, 100
     # iterations, rates in [0, 10], GPUTsit5 dt = 0.1 with saveat every 0.01
     # on [0, 3], noise-free data)
     doc = _json.load(open(os.path.join(ROOT, "tests", "golden", "lv_obs_benchmark.json")))
-    obs = np.array(doc["obs"], np.float32)
-    P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 10.0), 10000, 5, seed=1, ode_alg="tsit5",
+    obs_b = np.array(doc["obs"], np.float32)
+    theta_true = np.asarray(doc["theta"])
+    P.parameter_estim_ode(obs_b, np.zeros(4), np.full(4, 10.0), 10000, 5, seed=1, ode_alg="tsit5",
                           dt=0.1, saveat=0.01)  # warm-up
-    sol, sw = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 10.0), 10000, 100, seed=1,
-                                    ode_alg="tsit5", dt=0.1, saveat=0.01)
-    ms = sw.elapsed_ms()
-    sw.free()
-    out["reference_benchmark_shape"] = {
-        "device_ms": ms, "particle_iters_per_s": 10000 * 100 / (ms / 1e3),
-        "theta": [float(v) for v in sol.u], "loss_f64": sol.objective,
-        "max_rel_err_theta": float(np.max(np.abs(np.asarray(sol.u) / np.asarray(doc["theta"]) - 1))),
-        "workload": "10000 particles x 100 iterations, Tsit5 dt 0.1, 301 samples every 0.01"}
+    out["reference_benchmark_shape"] = _ode_leg(
+        lambda: P.parameter_estim_ode(obs_b, np.zeros(4), np.full(4, 10.0), 10000, 100, seed=1,
+                                      ode_alg="tsit5", dt=0.1, saveat=0.01), 10000, 100,
+        lambda sol: {"max_rel_err_theta": float(np.max(np.abs(np.asarray(sol.u) / theta_true - 1))),
+                     "workload": "10000 particles x 100 iterations, Tsit5 dt 0.1, "
+                                 "301 samples every 0.01"})
     # the reference's FitzHugh-Nagumo benchmark shape (same solver settings,
     # rates in [0, 5])
     doc = _json.load(open(os.path.join(ROOT, "tests", "golden", "fhn_obs_benchmark.json")))
-    obs = np.array(doc["obs"], np.float32)
+    obs_f = np.array(doc["obs"], np.float32)
     kw = dict(ode_alg="tsit5", dt=0.1, saveat=0.01, model="fitzhugh_nagumo")
-    P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 10000, 5, seed=0, **kw)  # warm-up
-    sol, sw = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 10000, 100, seed=0, **kw)
-    ms = sw.elapsed_ms()
-    sw.free()
-    out["fitzhugh_nagumo_reference_benchmark_shape"] = {
-        "device_ms": ms, "particle_iters_per_s": 10000 * 100 / (ms / 1e3),
-        "theta": [float(v) for v in sol.u], "loss_f64": sol.objective,
-        "workload": "10000 particles x 100 iterations, Tsit5 dt 0.1, 301 samples every 0.01"}
+    P.parameter_estim_ode(obs_f, np.zeros(4), np.full(4, 5.0), 10000, 5, seed=0, **kw)  # warm-up
+    out["fitzhugh_nagumo_reference_benchmark_shape"] = _ode_leg(
+        lambda: P.parameter_estim_ode(obs_f, np.zeros(4), np.full(4, 5.0), 10000, 100, seed=0,
+                                      **kw), 10000, 100,
+        lambda sol: {"workload": "10000 particles x 100 iterations, Tsit5 dt 0.1, "
+                                 "301 samples every 0.01"})
     return out
 
 

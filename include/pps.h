@@ -140,9 +140,17 @@ typedef struct pps_problem {
    * sampled every (int)params[3] steps.  Adaptive Tsit5: t_k = k * params[2],
    * initial step params[3], abstol params[4], reltol params[5], optional step
    * limit params[6] (default 10000; beyond it the loss is +Inf; controller
-   * documented at lv_sse_tsit5 in the device sources).  The _SAVEAT forms:
+   * documented at ode_sse_tsit5 in the device sources).  The _SAVEAT forms:
    * t_k = k * params[2], step (initial step when adaptive) params[3], and
-   * for the adaptive one params[4..6] as above; the last step ends on t_{n_obs-1}. */
+   * for the adaptive one params[4..6] as above; the last step ends on t_{n_obs-1}.
+   * With adaptive steps, a particle's loss inside a swarm step is summed only
+   * until it reaches the particle's best cost, since the sum of squares
+   * cannot fall back below it and only the comparison is used: positions,
+   * velocities, best positions, best costs and the global best are the same
+   * as with full sums, and a particle whose trajectory turns stiff stops long
+   * before the step limit.  Setting PPS_ODE_FULL_LOSS=1 in the environment
+   * before a swarm is created makes every loss a full sum (for timing
+   * comparisons). */
   const float* obs;
   int32_t n_obs;
   /* Inequality constraints g_k(x) <= 0, k = 0..n_cons-1 (n_cons <= PPS_MAXCONS),

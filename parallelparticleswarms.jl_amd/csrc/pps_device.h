@@ -353,63 +353,89 @@ __device__ __forceinline__ float root5(float x) {
 // parameter vector whose trajectory turns stiff (the whole-vector clamp of
 // the ODE path admits negative rates) would otherwise hold its wavefront for
 // the explicit method's stability-limited steps.  Loss as ode_sse_rk4.
+//
+// `bound` (adaptive steps only): the loss is a running sum of non-negative
+// terms, so once it reaches `bound` (the particle's best cost) the full sum
+// cannot fall below it and the caller's `cost < best` test is decided; the
+// partial sum is returned.  This is what keeps a particle whose trajectory
+// turns stiff from running to the step limit.  A NaN bound never stops the
+// sum.  (With fixed steps every lane of a wavefront integrates equally long,
+// and stopping early was measured to gain nothing.)
 template <int MODEL, bool ADAPTIVE>
-__device__ __forceinline__ float ode_sse_tsit5(const float* th, const ProbArgs& P) {
+__device__ __forceinline__ float ode_sse_tsit5(const float* th, const ProbArgs& P, float bound) {
   float u1 = P.par[0], u2 = P.par[1];
   float k1a, k1b;
   ode_rhs<MODEL>(u1, u2, th[0], th[1], th[2], th[3], k1a, k1b);
   float loss = 0.0f;
   const float* obs = P.obs;
-  float t = 0.0f, dt = ADAPTIVE ? P.par[3] : P.par[2];
-  int32_t steps = 0;
-  for (int k = 0; k < P.n_obs; ++k) {
-    if (k > 0) {
-      if (!ADAPTIVE) {
-        for (int s = 0; s < P.steps_per_obs; ++s) {
-          float n1, n2, k7a, k7b, e1, e2;
-          tsit5_step<MODEL, false>(u1, u2, k1a, k1b, dt, th, n1, n2, k7a, k7b, e1, e2);
-          u1 = n1;
-          u2 = n2;
-          k1a = k7a;
-          k1b = k7b;
-        }
-      } else {
-        const float t_next = (float)k * P.par[2];
-        const float atol = P.par[4], rtol = P.par[5];
-        const int32_t max_steps = P.par[6] > 0.0f ? (int32_t)P.par[6] : ODE_MAX_STEPS;
-        while (t < t_next) {
-          const bool hit = !(t + dt < t_next);
-          const float h = hit ? t_next - t : dt;
-          if (++steps > max_steps || !(t + h > t)) return __builtin_inff();
-          float n1, n2, k7a, k7b, e1, e2;
-          tsit5_step<MODEL, true>(u1, u2, k1a, k1b, h, th, n1, n2, k7a, k7b, e1, e2);
-          const float a1 = __builtin_fabsf(u1), b1 = __builtin_fabsf(n1);
-          const float a2 = __builtin_fabsf(u2), b2 = __builtin_fabsf(n2);
-          const float s1 = e1 / (atol + (a1 > b1 ? a1 : b1) * rtol);
-          const float s2 = e2 / (atol + (a2 > b2 ? a2 : b2) * rtol);
-          const float eest = sqrtf((s1 * s1 + s2 * s2) * 0.5f);
-          float q = 5.0f;
-          if (eest <= 3.0e38f) {
-            q = root5(eest) / 0.9f;
-            q = q < 0.1f ? 0.1f : (q > 5.0f ? 5.0f : q);
-          }
-          const float hnew = h / q;
-          if (eest <= 1.0f) {
-            t = hit ? t_next : t + h;
-            u1 = n1;
-            u2 = n2;
-            k1a = k7a;
-            k1b = k7b;
-            dt = hit ? (hnew > dt ? hnew : dt) : hnew;
-          } else {
-            dt = hnew;
-          }
-        }
-      }
-    }
-    const float e1 = obs[2 * k] - u1;
-    const float e2 = obs[2 * k + 1] - u2;
+  if (P.n_obs <= 0) return loss;
+  {
+    const float e1 = obs[0] - u1;
+    const float e2 = obs[1] - u2;
     loss = loss + (e1 * e1 + e2 * e2);
+  }
+  if (!ADAPTIVE) {
+    const float dt = P.par[2];
+    for (int k = 1; k < P.n_obs; ++k) {
+      for (int s = 0; s < P.steps_per_obs; ++s) {
+        float n1, n2, k7a, k7b, e1, e2;
+        tsit5_step<MODEL, false>(u1, u2, k1a, k1b, dt, th, n1, n2, k7a, k7b, e1, e2);
+        u1 = n1;
+        u2 = n2;
+        k1a = k7a;
+        k1b = k7b;
+      }
+      const float e1 = obs[2 * k] - u1;
+      const float e2 = obs[2 * k + 1] - u2;
+      loss = loss + (e1 * e1 + e2 * e2);
+    }
+    return loss;
+  }
+  // Adaptive: one flat loop over attempted steps in which each lane moves on
+  // to its next observation by itself.  (Nested loops, a step loop inside a
+  // loop over observations, would make a wavefront run, interval by
+  // interval, as many steps as its slowest lane needs in that interval.)
+  const float atol = P.par[4], rtol = P.par[5];
+  const int32_t max_steps = P.par[6] > 0.0f ? (int32_t)P.par[6] : ODE_MAX_STEPS;
+  float t = 0.0f, dt = P.par[3];
+  int32_t steps = 0;
+  int k = 1;
+  while (k < P.n_obs) {
+    const float t_next = (float)k * P.par[2];
+    if (!(t < t_next)) {
+      const float e1 = obs[2 * k] - u1;
+      const float e2 = obs[2 * k + 1] - u2;
+      loss = loss + (e1 * e1 + e2 * e2);
+      if (loss >= bound) return loss;
+      ++k;
+      continue;
+    }
+    const bool hit = !(t + dt < t_next);
+    const float h = hit ? t_next - t : dt;
+    if (++steps > max_steps || !(t + h > t)) return __builtin_inff();
+    float n1, n2, k7a, k7b, e1, e2;
+    tsit5_step<MODEL, true>(u1, u2, k1a, k1b, h, th, n1, n2, k7a, k7b, e1, e2);
+    const float a1 = __builtin_fabsf(u1), b1 = __builtin_fabsf(n1);
+    const float a2 = __builtin_fabsf(u2), b2 = __builtin_fabsf(n2);
+    const float s1 = e1 / (atol + (a1 > b1 ? a1 : b1) * rtol);
+    const float s2 = e2 / (atol + (a2 > b2 ? a2 : b2) * rtol);
+    const float eest = sqrtf((s1 * s1 + s2 * s2) * 0.5f);
+    float q = 5.0f;
+    if (eest <= 3.0e38f) {
+      q = root5(eest) / 0.9f;
+      q = q < 0.1f ? 0.1f : (q > 5.0f ? 5.0f : q);
+    }
+    const float hnew = h / q;
+    if (eest <= 1.0f) {
+      t = hit ? t_next : t + h;
+      u1 = n1;
+      u2 = n2;
+      k1a = k7a;
+      k1b = k7b;
+      dt = hit ? (hnew > dt ? hnew : dt) : hnew;
+    } else {
+      dt = hnew;
+    }
   }
   return loss;
 }
@@ -422,7 +448,8 @@ __device__ __forceinline__ float ode_sse_tsit5(const float* th, const ProbArgs& 
 // limit par[6] and the +Inf results as ode_sse_tsit5; fixed steps carry no
 // limit (the host bounds their number).  Loss as ode_sse_rk4.
 template <int MODEL, bool ADAPTIVE>
-__device__ __forceinline__ float ode_sse_tsit5_saveat(const float* th, const ProbArgs& P) {
+__device__ __forceinline__ float ode_sse_tsit5_saveat(const float* th, const ProbArgs& P,
+                                                      float bound) {
   float u1 = P.par[0], u2 = P.par[1];
   float ka[7], kb[7];
   ode_rhs<MODEL>(u1, u2, th[0], th[1], th[2], th[3], ka[0], kb[0]);
@@ -464,6 +491,7 @@ __device__ __forceinline__ float ode_sse_tsit5_saveat(const float* th, const Pro
       tsit5_interp(u1, u2, h, (tk - t) / h, ka, kb, v1, v2);
       const float d1 = obs[2 * k] - v1, d2 = obs[2 * k + 1] - v2;
       loss = loss + (d1 * d1 + d2 * d2);
+      if (ADAPTIVE && loss >= bound) return loss;
       ++k;
     }
     t = t_new;
@@ -477,9 +505,11 @@ __device__ __forceinline__ float ode_sse_tsit5_saveat(const float* th, const Pro
 
 // Objective f(x - shift) over a register array (compile-time capacity MAXD,
 // runtime dimension d <= MAXD).  OBJ is a compile-time PPS_OBJ_* constant.
+// `bound` is passed to the adaptive ODE losses (see ode_sse_tsit5); other
+// objectives ignore it.
 template <int MAXD, int OBJ>
 __device__ __forceinline__ float objective(const float (&xin)[MAXD], const int d,
-                                           const ProbArgs& P) {
+                                           const ProbArgs& P, float bound) {
   float z[MAXD];
 #pragma unroll
   // an absent shift is all +0.0f, and x - (+0.0f) is x bit for bit
@@ -542,10 +572,10 @@ __device__ __forceinline__ float objective(const float (&xin)[MAXD], const int d
   } else if constexpr (is_ode_objective(OBJ)) {
     constexpr int M = ode_model(OBJ), S = ode_solver(OBJ);
     if constexpr (S == PPS_ODE_RK4) return ode_sse_rk4<M>(z, P);
-    else if constexpr (S == PPS_ODE_TSIT5) return ode_sse_tsit5<M, false>(z, P);
-    else if constexpr (S == PPS_ODE_ATSIT5) return ode_sse_tsit5<M, true>(z, P);
-    else if constexpr (S == PPS_ODE_TSIT5_SAVEAT) return ode_sse_tsit5_saveat<M, false>(z, P);
-    else return ode_sse_tsit5_saveat<M, true>(z, P);
+    else if constexpr (S == PPS_ODE_TSIT5) return ode_sse_tsit5<M, false>(z, P, bound);
+    else if constexpr (S == PPS_ODE_ATSIT5) return ode_sse_tsit5<M, true>(z, P, bound);
+    else if constexpr (S == PPS_ODE_TSIT5_SAVEAT) return ode_sse_tsit5_saveat<M, false>(z, P, bound);
+    else return ode_sse_tsit5_saveat<M, true>(z, P, bound);
   } else {
     return __uint_as_float(0x7FC00000u);
   }

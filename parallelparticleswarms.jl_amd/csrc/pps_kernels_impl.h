@@ -28,6 +28,7 @@
 namespace pps {
 
 constexpr uint64_t KEY_NONE = ~0ull;
+#define NO_BOUND __uint_as_float(0x7FC00000u)  // NaN: an ODE loss is summed in full
 
 // Minimum over a workgroup.  Every thread receives the result.
 template <int BLOCK>
@@ -112,10 +113,13 @@ __device__ __forceinline__ void publish_block(const SwarmDev& S, uint32_t par, i
 // kernels: 1-based particle index + 1; async: 1-based iteration of the launch
 // + 1).  Constrained problems run on the runtime-dimension instantiations, so
 // the branch is compiled only there.
+// `best` is the particle's best cost when the caller only compares the result
+// with it (an adaptive ODE loss may then stop early, see ode_sse_tsit5), NaN
+// otherwise.
 template <int MAXD, bool EXACT, int OBJ>
 __device__ __forceinline__ float cost_at(const float (&x)[MAXD], const int d, const ProbArgs& P,
-                                         float iter1) {
-  float c = objective<MAXD, OBJ>(x, d, P);
+                                         float iter1, float best) {
+  float c = objective<MAXD, OBJ>(x, d, P, P.ode_stop ? best : NO_BOUND);
   if constexpr (!EXACT && (OBJ == PPS_OBJ_ROSENBROCK || OBJ == PPS_OBJ_RASTRIGIN ||
                            OBJ == PPS_OBJ_SPHERE)) {
     if (P.n_cons > 0) c = c + constraint_penalty<MAXD>(x, d, P, sqrtf(iter1));
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(BLOCK) void k_init(SwarmDev S, ProbArgs P, uint32_t
         init_position<MAXD>(x, (uint32_t)i, S.epoch, S.key0, S.key1, d, P);
 #pragma unroll
       for (int j = 0; j < MAXD; ++j) z[j] = 0.0f;
-      const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, 1.0f);
+      const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, 1.0f, NO_BOUND);
       store_vec<MAXD>(x, S.x, S.npad, i, d);
       store_vec<MAXD>(z, S.v, S.npad, i, d);
       store_vec<MAXD>(x, S.pb, S.npad, i, d);
@@ -299,7 +303,7 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
         velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch,
                                      S.key0, S.key1, d, P);
       }
-      const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2));
+      const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2), pc);
 #pragma unroll
       for (int j = 0; j < MAXD; ++j) {
         if (j < d) {
@@ -384,7 +388,7 @@ __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepA
     if (act) {
       velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
                               S.epoch, S.key0, S.key1, d, P);
-      const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2));
+      const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2), pc);
       if (c < pc) {
 #pragma unroll
         for (int j = 0; j < MAXD; ++j) pb[j] = x[j];
@@ -450,7 +454,7 @@ __global__ __launch_bounds__(BLOCK) void k_async(SwarmDev S, ProbArgs P, StepArg
         for (int j = 0; j < MAXD; ++j) g[j] = follow ? pb[j] : g0[j];
         velocity_position<MAXD, OBJ>(x, v, pb, g, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
                                 S.epoch, S.key0, S.key1, d, P);
-        const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(k + 1));
+        const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(k + 1), pc);
         if (c < pc) {
 #pragma unroll
           for (int j = 0; j < MAXD; ++j) pb[j] = x[j];
@@ -504,7 +508,7 @@ __global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArg
     pc = S.pc[i];
     velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch, S.key0,
                             S.key1, d, P);
-    const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2));
+    const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2), pc);
     store_vec<MAXD>(x, S.x, S.npad, i, d);
     store_vec<MAXD>(v, S.v, S.npad, i, d);
     if (c < pc) {

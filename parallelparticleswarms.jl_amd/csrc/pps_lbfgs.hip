@@ -91,7 +91,7 @@ __device__ __forceinline__ void lane_fg(float xl, int l, int d, const ProbArgs& 
     float xa[G > 1 ? 2 : 1];
     xa[0] = xr[0];
     if constexpr (G > 1) xa[1] = xr[1];
-    f = objective<(G > 1 ? 2 : 1), OBJ>(xa, d, P);
+    f = objective<(G > 1 ? 2 : 1), OBJ>(xa, d, P, __uint_as_float(0x7FC00000u));
     const float x1 = P.has_shift ? xr[0] - P.shift[0] : xr[0];
     const float x2 = (d > 1) ? (P.has_shift ? xr[1] - P.shift[1] : xr[1]) : 0.0f;
     float g0 = 0.0f, g1 = 0.0f;

@@ -45,6 +45,11 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 int fill_prob_args(const pps_problem* prob, ProbArgs* P) {
   if (!prob) return fail(PPS_E_INVALID, "problem is NULL");
   memset(P, 0, sizeof *P);
+  // An adaptive-step ODE loss stops summing at the particle's best cost (the
+  // swarm state is the same); PPS_ODE_FULL_LOSS=1 in the environment makes
+  // every loss full.
+  const char* full = getenv("PPS_ODE_FULL_LOSS");
+  P->ode_stop = (full && full[0] && strcmp(full, "0") != 0) ? 0 : 1;
   const int d = prob->d;
   if (d < 1 || d > PPS_MAXD) return fail(PPS_E_INVALID, "dimension %d outside 1..%d", d, PPS_MAXD);
   if (!prob->lb != !prob->ub) return fail(PPS_E_INVALID, "lb and ub are both given or both NULL");

@@ -82,6 +82,10 @@ struct ProbArgs {
   // [a_0..a_{d-1}, q_0..q_{d-1}, c], g(x) = sum a_j x_j + sum q_j x_j^2 + c.
   const float* cons;
   int32_t n_cons;
+  // Adaptive-step ODE losses stop summing once they reach the particle's best
+  // cost (1), or are always summed in full (0).  The swarm state is the same
+  // either way.
+  int32_t ode_stop;
 };
 
 // Global-best record in device memory (async and queue-lock modes keep their

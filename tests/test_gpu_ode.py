@@ -182,3 +182,35 @@ def test_reference_fhn_benchmark_shape():
                                     ode_alg="tsit5", dt=0.1, saveat=0.01, model="fitzhugh_nagumo")
     assert sol.objective < 1e-4
     assert abs(sol.u[3] / doc["theta"][3] - 1) < 1e-3
+
+
+@pytest.mark.parametrize("saveat", [False, True])
+@pytest.mark.parametrize("mode", [L.MODE_SYNC, L.MODE_ASYNC, L.MODE_QLOCK])
+def test_adaptive_early_stop_keeps_state(monkeypatch, saveat, mode):
+    # An adaptive loss stops once it reaches the particle's best cost; the
+    # swarm must be exactly the one with every loss summed in full
+    # (PPS_ODE_FULL_LOSS=1), and after enough iterations for stiff parameter
+    # vectors and non-improving particles to occur, both equal the oracle.
+    note(f"adaptive early stop saveat={saveat} mode={mode}")
+    # queue-lock: one workgroup (n <= 1024), where it is deterministic
+    n, T = (1000 if mode == L.MODE_QLOCK else 3000), 12
+    mk = _saveat_pair if saveat else _tsit5_pair
+    launch = L.LAUNCH_GRAPH if mode == L.MODE_SYNC else L.LAUNCH_AUTO
+    g, o = mk(n, 5 + mode, True, mode, launch)
+    monkeypatch.setenv("PPS_ODE_FULL_LOSS", "1")
+    gf, _ = mk(n, 5 + mode, True, mode, launch)
+    monkeypatch.delenv("PPS_ODE_FULL_LOSS")
+    g.run(T)
+    gf.run(T)
+    for a, b, name in zip(g.state()[:4], gf.state()[:4], ("x", "v", "best", "best_cost")):
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    ga, gb = g.gbest(), gf.gbest()
+    assert ga.index == gb.index and ga.cost == gb.cost
+    if mode == L.MODE_SYNC:
+        o.run_sync(T)
+    elif mode == L.MODE_ASYNC:
+        o.run_async(T)
+    else:
+        o.run_qlock(T)
+    assert_same_state(g, o, f"after {T} iterations")
+    assert_same_gbest(g, o)
