@@ -34,18 +34,76 @@ namespace pps {
 
 constexpr int LB_MAXH = 16;
 
+// Lane exchanges of the group reductions without an LDS round trip: DPP moves
+// inside a row of 16 lanes (quad permutations, row rotations) and, for the
+// offsets 4 and 16, a bit-mask swizzle inside 32 lanes.  A group never spans
+// a row boundary unless G >= 32, and a row rotation by 8 is exactly the xor-8
+// partner; a rotation by 4 equals the xor-4 partner once the values repeat
+// with period 8, which the xor-8 step leaves behind.
+constexpr int DPP_QUAD_XOR1 = 0xB1;  // quad_perm [1,0,3,2]
+constexpr int DPP_QUAD_XOR2 = 0x4E;  // quad_perm [2,3,0,1]
+constexpr int DPP_ROW_ROR1 = 0x121;  // row_ror:1
+constexpr int DPP_ROW_ROR4 = 0x124;  // row_ror:4
+constexpr int DPP_ROW_ROR8 = 0x128;  // row_ror:8
+constexpr int DPP_ROW_ROR15 = 0x12F; // row_ror:15
+constexpr int SWZ_XOR4 = (4 << 10) | 0x1F;    // ds_swizzle bit mode: lane ^ 4
+constexpr int SWZ_XOR16 = (16 << 10) | 0x1F;  // lane ^ 16
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int PAT>
+__device__ __forceinline__ float swz(float v) {
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), PAT));
+}
+
+// Value of lane (l ^ off) of the group at butterfly step `off` (steps taken in
+// the order G/2, ..., 1, so the periodicity above holds at every step).
+template <int G, int OFF>
+__device__ __forceinline__ float partner(float v) {
+  if constexpr (OFF == 32) return __shfl_xor(v, 32, 64);
+  else if constexpr (OFF == 16) return swz<SWZ_XOR16>(v);
+  else if constexpr (OFF == 8) return dpp<DPP_ROW_ROR8>(v);
+  else if constexpr (OFF == 4) return G == 8 ? swz<SWZ_XOR4>(v) : dpp<DPP_ROW_ROR4>(v);
+  else if constexpr (OFF == 2) return dpp<DPP_QUAD_XOR2>(v);
+  else return dpp<DPP_QUAD_XOR1>(v);
+}
+
+// Butterfly sum: step off adds the partner's value, v = v + v[l ^ off], for
+// off = G/2, ..., 1, so every lane of the group ends with the same float.
 template <int G>
 __device__ __forceinline__ float gsum(float v) {
-#pragma unroll
-  for (int off = G / 2; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, G);
+  if constexpr (G >= 64) v = v + partner<G, 32>(v);
+  if constexpr (G >= 32) v = v + partner<G, 16>(v);
+  if constexpr (G >= 16) v = v + partner<G, 8>(v);
+  if constexpr (G >= 8) v = v + partner<G, 4>(v);
+  if constexpr (G >= 4) v = v + partner<G, 2>(v);
+  if constexpr (G >= 2) v = v + partner<G, 1>(v);
   return v;
 }
 template <int G>
 __device__ __forceinline__ float gmaxabs(float v) {
   v = __builtin_fabsf(v);
-#pragma unroll
-  for (int off = G / 2; off >= 1; off >>= 1) v = __builtin_fmaxf(v, __shfl_xor(v, off, G));
+  if constexpr (G >= 64) v = __builtin_fmaxf(v, partner<G, 32>(v));
+  if constexpr (G >= 32) v = __builtin_fmaxf(v, partner<G, 16>(v));
+  if constexpr (G >= 16) v = __builtin_fmaxf(v, partner<G, 8>(v));
+  if constexpr (G >= 8) v = __builtin_fmaxf(v, partner<G, 4>(v));
+  if constexpr (G >= 4) v = __builtin_fmaxf(v, partner<G, 2>(v));
+  if constexpr (G >= 2) v = __builtin_fmaxf(v, partner<G, 1>(v));
   return v;
+}
+
+// z of lane (l + 1) mod G and (l - 1) mod G.
+template <int G>
+__device__ __forceinline__ float next_lane(float z, int l) {
+  if constexpr (G == 16) return dpp<DPP_ROW_ROR15>(z);
+  else return __shfl(z, (l + 1) & (G - 1), G);
+}
+template <int G>
+__device__ __forceinline__ float prev_lane(float z, int l) {
+  if constexpr (G == 16) return dpp<DPP_ROW_ROR1>(z);
+  else return __shfl(z, (l - 1) & (G - 1), G);
 }
 
 __device__ __forceinline__ float projc(float x, float lo, float hi) {
@@ -60,8 +118,8 @@ __device__ __forceinline__ void lane_fg(float xl, int l, int d, const ProbArgs& 
   const float z = act ? (P.has_shift ? xl - P.shift[l] : xl) : 0.0f;
   const float p1 = P.par[0], p2 = P.par[1];
   if constexpr (OBJ == PPS_OBJ_ROSENBROCK) {
-    const float zn = __shfl(z, (l + 1) & (G - 1), G);  // z[l+1] (used when l+1 < d)
-    const float zp = __shfl(z, (l - 1) & (G - 1), G);  // z[l-1] (used when l > 0)
+    const float zn = next_lane<G>(z, l);  // z[l+1] (used when l+1 < d)
+    const float zp = prev_lane<G>(z, l);  // z[l-1] (used when l > 0)
     float term = 0.0f, gl = 0.0f;
     if (act && l + 1 < d) {
       const float dd = zn - z * z;

@@ -24,6 +24,7 @@ def _run_both(g, o, maxiters, history=10, abstol=0.0):
     (O.ROSENBROCK, 10, 2000, -5.0, 5.0),    # G = 16 with 6 padding lanes
     (O.SPHERE, 5, 1000, -1.0, 2.0),         # G = 8
     (O.RASTRIGIN, 64, 256, -5.12, 5.12),    # G = 64, one particle per wave
+    (O.ROSENBROCK, 20, 512, -5.0, 5.0),     # G = 32 with 12 padding lanes
     (O.ROSENBROCK, 3, 999, -1.0, 10.0),     # G = 4
     (O.BEALE, 2, 500, -4.5, 4.5),
     (O.HIMMELBLAU, 2, 500, -5.0, 5.0),
