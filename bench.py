@@ -35,7 +35,9 @@ Also reported on the same line:
                 same solution.
   cpu_baseline  the CPU oracle (reference-faithful AoS restatement, OpenMP
                 over particles + serial argmin per iteration) on a bounded
-                sample of the same workload on this host.
+                sample of the same workload on this host; `c0` adds the
+                reference's CPU-backend case (3-D Rosenbrock, 1000 particles,
+                500 iterations) per solve, all threads and one thread.
 """
 import argparse
 import json
@@ -103,11 +105,27 @@ def cpu_baseline(sample_iters=200):
         s.run_sync(sample_iters)
         total_s += time.perf_counter() - t0
         total_pi += N * sample_iters
+    # the reference's own CPU-backend case: 3-D Rosenbrock, lb = -1, ub = 10, 1000
+    # particles, 500 iterations, the global best taken from the argmin's current
+    # position as the CPU backend does (src/lowerlevel_solve.jl:59-60)
+    c0 = {}
+    prob = O.Problem(O.ROSENBROCK, 3, np.full(3, -1.0, np.float32), np.full(3, 10.0, np.float32),
+                     np.asarray((1.0, 100.0), np.float32))
+    for key, nt in (("solve_ms", threads), ("solve_ms_1_thread", 1)):
+        times = []
+        for rep in range(6):
+            s = O.Swarm(prob, 1000, seed=rep, nthreads=nt)
+            t0 = time.perf_counter()
+            s.run_sync(500, gbest_from=O.GBEST_CURRENT)
+            times.append(time.perf_counter() - t0)
+        c0[key] = float(np.median(times[1:]) * 1e3)
     return {"value": total_pi / total_s, "unit": "particle-iters/s", "cores": threads,
             "kind": "port",
             "sample": f"sync PSO, 10-D Rastrigin + 10-D Rosenbrock, 2^20 particles each, "
                       f"{sample_iters} timed iterations each (after 1 warm-up), "
-                      f"{total_s:.1f} s of CPU time"}
+                      f"{total_s:.1f} s of CPU time",
+            "c0": dict(c0, workload="3-D Rosenbrock, 1000 particles, 500 iterations, "
+                                    "median of 5 solves (after 1)")}
 
 
 def c4_mix(P, L, ctx, count=4096, plain_sample=256, served_qps=None):
