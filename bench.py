@@ -203,13 +203,17 @@ def hybrid_c2(P, L, ctx):
 def _full_loss(run):
     """Run `run()` once more with every ODE loss summed in full
     (PPS_ODE_FULL_LOSS=1): returns (device ms, solution position)."""
+    prev = os.environ.get("PPS_ODE_FULL_LOSS")
     os.environ["PPS_ODE_FULL_LOSS"] = "1"
     try:
         sol, sw = run()
         ms = sw.elapsed_ms()
         sw.free()
     finally:
-        del os.environ["PPS_ODE_FULL_LOSS"]
+        if prev is None:
+            del os.environ["PPS_ODE_FULL_LOSS"]
+        else:
+            os.environ["PPS_ODE_FULL_LOSS"] = prev
     return ms, np.asarray(sol.u)
 
 
