@@ -100,14 +100,33 @@ __device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) {
   return a < b ? a : b;
 }
 
-// Minimum over the 64 lanes of a wavefront (every lane receives it).
+// 64-bit DPP move: the value of the lane the control selects, or ~0 (the
+// identity of min) in rows outside ROW_MASK.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t k) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)k, CTRL, ROW_MASK,
+                                                            0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)(uint32_t)(k >> 32), CTRL,
+                                                            ROW_MASK, 0xF, false);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Minimum over the 64 lanes of a wavefront, returned to every lane (wave-
+// uniform).  All 64 lanes must be active.  DPP moves only, no LDS: quad
+// permutations and row rotations leave every lane of a 16-lane row with the
+// row's minimum, the row broadcasts of lanes 15 and 31 fold the four rows into
+// row 3, and lane 63 is read out.  The minimum does not depend on the order
+// of the comparisons.
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t k) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const uint64_t o = __shfl_xor(k, off, 64);
-    k = umin64(k, o);
-  }
-  return k;
+  k = umin64(k, dpp_u64<0xB1, 0xF>(k));   // quad_perm [1,0,3,2]
+  k = umin64(k, dpp_u64<0x4E, 0xF>(k));   // quad_perm [2,3,0,1]
+  k = umin64(k, dpp_u64<0x124, 0xF>(k));  // row_ror:4
+  k = umin64(k, dpp_u64<0x128, 0xF>(k));  // row_ror:8
+  k = umin64(k, dpp_u64<0x142, 0xA>(k));  // row_bcast:15 into rows 1 and 3
+  k = umin64(k, dpp_u64<0x143, 0xC>(k));  // row_bcast:31 into rows 2 and 3
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(k >> 32), 63);
+  return ((uint64_t)hi << 32) | lo;
 }
 
 // ---------------------------------------------------------------------------

@@ -197,3 +197,41 @@ def test_autotune_records_a_choice():
     assert g in (1, 2, 3, 4, 6, 8) and us > 0
     assert LL.lib().pps_tuned_grid_per_cu(LL.OBJ_SPHERE, d, n) == g
     assert LL.lib().pps_tuned_grid_per_cu(LL.OBJ_SPHERE, d, 4 * n) == 0
+
+
+@pytest.mark.parametrize("n,launch", [(5000, L.LAUNCH_GRAPH), (1000, L.LAUNCH_WORKGROUP)])
+def test_argmin_at_every_lane(n, launch):
+    # the global-best argmin (wavefront DPP reduction, then across waves and
+    # workgroups) must find a single minimum wherever it sits: every lane of
+    # the first wavefront, lanes of later wavefronts and workgroups, the last
+    # particle; ties go to the lowest index (also +0.0 against -0.0), and
+    # NaN costs never win
+    note(f"argmin at every lane n={n} launch={launch}")
+    d = 2
+    g, _ = make_pair(O.SPHERE, d, n, seed=3, launch=launch)
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, (d, n)).astype(np.float32)
+    v = np.zeros((d, n), np.float32)
+    b = rng.uniform(-1, 1, (d, n)).astype(np.float32)
+    positions = list(range(64)) + [64 * k + (7 * k) % 64 for k in range(1, n // 64)] + [n - 1]
+    for p in positions:
+        bc = np.ones(n, np.float32)
+        bc[p] = 0.5
+        g.set_state(x, v, b, bc, 3)
+        gb = g.gbest()
+        assert gb.index == p and gb.cost == 0.5, (p, gb.index, gb.cost)
+        np.testing.assert_array_equal(gb.position, b[:, p])
+    for a, c, nan_below in [(5, 40, 3), (17, 63, 0), (63, 64, 0), (100, n - 1, 64), (0, n - 1, 0),
+                            (1, 2, 2)]:
+        bc = np.full(n, 2.0, np.float32)
+        bc[c] = 0.25
+        bc[a] = 0.25
+        bc[:nan_below] = np.nan
+        g.set_state(x, v, b, bc, 3)
+        gb = g.gbest()
+        assert gb.index == int(np.nanargmin(bc)), (a, c, nan_below, gb.index)
+    bc = np.full(n, 1.0, np.float32)
+    bc[n - 1] = 0.0
+    bc[n // 2] = -0.0
+    g.set_state(x, v, b, bc, 3)
+    assert g.gbest().index == n // 2
