@@ -7,8 +7,9 @@
 // particles.  The history pairs (s_i, y_i), i < m <= 16, live in registers as
 // one coordinate per lane: 2m VGPRs per lane instead of 2md per thread.  Dot
 // products and objective sums are xor butterflies inside the group
-// (v += shfl_xor(v, s) for s = G/2 .. 1), so every lane of a group holds the
-// identical value and all branch decisions are group-uniform.
+// (v = v + v[l ^ s] for s = G/2 .. 1, the partner's value brought by DPP or a
+// swizzle, see partner()), so every lane of a group holds the identical value
+// and all branch decisions are group-uniform.
 //
 // Algorithm (Nocedal & Wright Alg. 7.4 two-loop recursion, projected onto the
 // box, Armijo backtracking; the reference's SimpleOptimization._lbfgs is not
