@@ -13,6 +13,9 @@ Also reported on the same line:
                 algorithmic bytes per launch / average launch time (hipEvents
                 on the swarm's stream over the timed region).
   async         ParallelPSOKernel(global_update=false) at the same shape.
+  qlock         ParallelPSOKernel(global_update=true), the reference's default:
+                one launch per iteration against the live global best, at the
+                same shape.
   c0            3-D Rosenbrock, 1000 particles, 500 iterations (one
                 resident workgroup): time per solve and per iteration.
   batched       the 4096-problem served mix solved in packed launches
@@ -321,7 +324,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--extras", default="async,c0,c4,hybrid,ode",
+    ap.add_argument("--extras", default="async,qlock,c0,c4,hybrid,ode",
                     help="comma list of the legs to run besides the headline")
     ap.add_argument("--sweep", action="store_true",
                     help="also time the sync step at 2^10..2^24 particles (adds ~1 min)")
@@ -398,6 +401,24 @@ def main():
         for s in a:
             s.free()
         log("[bench] async done")
+    if "qlock" in legs:
+        # queue-lock (ParallelPSOKernel(global_update=true)) at the same shape
+        q = [make_swarm(P, "rastrigin", N, D, 505, L.MODE_QLOCK, L.LAUNCH_AUTO),
+             make_swarm(P, "rosenbrock", N, D, 606, L.MODE_QLOCK, L.LAUNCH_AUTO)]
+        for s in q:
+            s.run(50)
+        t0 = time.perf_counter()
+        qms = 0.0
+        for s in q:
+            s.run(T)
+            qms += s.elapsed_ms()
+        qw = time.perf_counter() - t0
+        out["qlock"] = {"value": 2 * N * T / qw, "unit": "particle-iters/s", "device_ms": qms,
+                        "us_per_launch": qms * 1e3 / (2 * T),
+                        "workload": "same shape, one launch per iteration"}
+        for s in q:
+            s.free()
+        log("[bench] qlock done")
     if "c0" in legs:
         # C0: 3-D Rosenbrock, lb=-1, ub=10, 1000 particles, 500 iterations
         prob = P.OptimizationProblem(P.Rosenbrock(), np.zeros(3, np.float32), [1.0, 100.0],
