@@ -271,11 +271,16 @@ void pps_server::loop() {
         slot_req[s] = r;
         ++running;
       }
+      // A slice's work list holds at most max_items; a problem that does not
+      // fit whole (all parts of a large problem's iteration) waits for the
+      // next slice with its state untouched.
+      const size_t cap = (size_t)max_items;
       for (size_t s = 0; s < slot_req.size(); ++s) {
         Req* r = slot_req[s];
         if (!r) continue;
         if (!r->large) {
           if (!slot_fresh[s] && __atomic_load_n(&h_out[s].done, __ATOMIC_ACQUIRE)) continue;
+          if (items.size() + 1 > cap) continue;
           BatchItem it{};
           it.slot = (int32_t)s;
           it.kind = BATCH_SMALL;
@@ -286,7 +291,7 @@ void pps_server::loop() {
         }
         any_large = true;
         if (r->t_next > (uint32_t)r->maxiters) {
-          if (r->t_next == (uint32_t)r->maxiters + 1) {
+          if (r->t_next == (uint32_t)r->maxiters + 1 && items.size() + 1 <= cap) {
             BatchItem it{};
             it.slot = (int32_t)s;
             it.kind = BATCH_FINAL;
@@ -297,6 +302,7 @@ void pps_server::loop() {
           }
           continue;
         }
+        if (items.size() + (size_t)r->parts > cap) continue;
         for (int q = 0; q < r->parts; ++q) {
           BatchItem it{};
           it.slot = (int32_t)s;
@@ -326,7 +332,6 @@ void pps_server::loop() {
     // Two slices in flight: work list i of slice n is list n & 1, rewritten
     // only after slice n - 2 (its previous user) has completed.
     hipError_t e = hipSuccess;
-    if (items.size() > (size_t)max_items) items.resize((size_t)max_items);  // rest next slice
     const int buf = (int)(slices_enqueued & 1u);
     if (slices_enqueued >= 2 && slices_completed + 2 <= slices_enqueued) {
       e = hipEventSynchronize(slice_done[buf]);
