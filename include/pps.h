@@ -338,9 +338,12 @@ int pps_server_create(pps_ctx* ctx, const pps_server_config* cfg,
 int pps_server_submit(pps_server* sv, const pps_problem* prob,
                       const pps_swarm_params* params, int32_t maxiters,
                       float w, float wdamp, uint64_t* ticket);
-/* PPS_OK with the result, PPS_E_PENDING while queued or running. */
+/* PPS_OK with the result, PPS_E_PENDING while queued or running.  A request's
+ * final status (its result, PPS_E_CANCELLED or PPS_E_HIP) is returned once, by
+ * poll or wait; the ticket is then released and unknown (PPS_E_NOTFOUND). */
 int pps_server_poll(pps_server* sv, uint64_t ticket, pps_result* out);
-/* Block until done (timeout_s <= 0: no timeout; PPS_E_TIMEOUT otherwise). */
+/* Block until done (timeout_s <= 0: no timeout; PPS_E_TIMEOUT otherwise); the
+ * final status is returned once, as for poll. */
 int pps_server_wait(pps_server* sv, uint64_t ticket, double timeout_s,
                     pps_result* out);
 /* Cancel a request that is still queued (PPS_E_STATE once admitted). */

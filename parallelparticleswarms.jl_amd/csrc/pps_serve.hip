@@ -515,15 +515,26 @@ int pps_server_submit(pps_server* sv, const pps_problem* prob, const pps_swarm_p
   return PPS_OK;
 }
 
-static int result_of(pps_server* sv, Req* r, pps_result* out) {
+// Final status of a request (caller holds mu).  A finished request (done,
+// cancelled or failed) is returned once and then forgotten: the server
+// thread no longer refers to it, so a long-running server holds only the
+// requests that are unfinished or not yet collected.
+static int collect_locked(pps_server* sv, std::unordered_map<uint64_t, Req*>::iterator it,
+                          pps_result* out) {
+  Req* r = it->second;
+  int rc;
   switch (r->state) {
     case R_DONE:
       if (out) *out = r->res;
-      return PPS_OK;
-    case R_CANCELLED: return fail(PPS_E_CANCELLED, "request cancelled");
-    case R_FAILED: return fail(PPS_E_HIP, "request failed: %s", sv->fatal.c_str());
+      rc = PPS_OK;
+      break;
+    case R_CANCELLED: rc = fail(PPS_E_CANCELLED, "request cancelled"); break;
+    case R_FAILED: rc = fail(PPS_E_HIP, "request failed: %s", sv->fatal.c_str()); break;
     default: return PPS_E_PENDING;
   }
+  sv->reqs.erase(it);
+  delete r;
+  return rc;
 }
 
 int pps_server_poll(pps_server* sv, uint64_t ticket, pps_result* out) {
@@ -531,23 +542,29 @@ int pps_server_poll(pps_server* sv, uint64_t ticket, pps_result* out) {
   std::lock_guard<std::mutex> lk(sv->mu);
   auto it = sv->reqs.find(ticket);
   if (it == sv->reqs.end()) return fail(PPS_E_NOTFOUND, "unknown ticket %llu", (unsigned long long)ticket);
-  return result_of(sv, it->second, out);
+  return collect_locked(sv, it, out);
 }
 
 int pps_server_wait(pps_server* sv, uint64_t ticket, double timeout_s, pps_result* out) {
   if (!sv) return fail(PPS_E_INVALID, "server is NULL");
   std::unique_lock<std::mutex> lk(sv->mu);
-  auto it = sv->reqs.find(ticket);
-  if (it == sv->reqs.end()) return fail(PPS_E_NOTFOUND, "unknown ticket %llu", (unsigned long long)ticket);
-  Req* r = it->second;
   const auto deadline = std::chrono::steady_clock::now() +
                         std::chrono::duration<double>(timeout_s > 0 ? timeout_s : 1e9);
-  while (r->state == R_PENDING || r->state == R_RUNNING) {
-    if (sv->cv_done.wait_until(lk, deadline) == std::cv_status::timeout &&
-        (r->state == R_PENDING || r->state == R_RUNNING))
-      return fail(PPS_E_TIMEOUT, "wait timed out");
+  while (true) {
+    // looked up again after every wake-up: another thread may have collected it
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end())
+      return fail(PPS_E_NOTFOUND, "unknown ticket %llu", (unsigned long long)ticket);
+    const int rc = collect_locked(sv, it, out);
+    if (rc != PPS_E_PENDING) return rc;
+    if (sv->cv_done.wait_until(lk, deadline) == std::cv_status::timeout) {
+      it = sv->reqs.find(ticket);
+      if (it == sv->reqs.end())
+        return fail(PPS_E_NOTFOUND, "unknown ticket %llu", (unsigned long long)ticket);
+      const int rc2 = collect_locked(sv, it, out);
+      return rc2 == PPS_E_PENDING ? fail(PPS_E_TIMEOUT, "wait timed out") : rc2;
+    }
   }
-  return result_of(sv, r, out);
 }
 
 int pps_server_cancel(pps_server* sv, uint64_t ticket) {

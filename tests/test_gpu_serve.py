@@ -124,3 +124,8 @@ def test_submit_refusals_and_cancel():
         with pytest.raises(P.PPSError):
             sv.wait(t_q)
         assert sv.wait(t_hog, timeout=120).cost_f32 >= 0
+        # a final status is returned once; the ticket is then released
+        with pytest.raises(P.PPSError):
+            sv.poll(t_hog)
+        with pytest.raises(P.PPSError):
+            sv.wait(t_q)
