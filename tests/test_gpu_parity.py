@@ -46,7 +46,11 @@ def test_sync_c0_workgroup_trajectory(gbest_from):
 
 @pytest.mark.parametrize("launch", [L.LAUNCH_EAGER, L.LAUNCH_GRAPH])
 @pytest.mark.parametrize("obj,d,n", [(O.RASTRIGIN, 10, 5000), (O.ROSENBROCK, 10, 3001),
-                                     (O.ROSENBROCK, 5, 70000)])
+                                     (O.ROSENBROCK, 5, 70000),
+                                     # draws ahead of the state (d <= 16), winner held in
+                                     # registers (d <= 32), and the 64-wide runtime path
+                                     (O.SPHERE, 16, 2000), (O.RASTRIGIN, 32, 1500),
+                                     (O.ROSENBROCK, 45, 1100)])
 def test_sync_multiblock_trajectory(launch, obj, d, n):
     note(f"sync multiblock launch={launch} obj={obj} d={d} n={n}")
     g, o = make_pair(obj, d, n, seed=7 + n, launch=launch)
