@@ -551,12 +551,12 @@ __global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArg
 
 // ---------------------------------------------------------------------------
 // Launchers.
-constexpr int BLK = PPS_BLK;  // workgroup size of the streaming kernels
 constexpr int QBLK = 1024; // queue-lock block (the reference's workgroup size)
 constexpr int WGB = 1024;  // single-workgroup sync loop
 
 template <int MAXD, bool EXACT, int OBJ>
 struct Launch {
+  static constexpr int BLK = block_of(OBJ);  // workgroup of the multi-workgroup kernels
   static hipError_t init(const SwarmDev& S, const ProbArgs& P, uint32_t par, hipStream_t st) {
     if (S.block == WGB && S.G == 1)
       hipLaunchKernelGGL((k_init<MAXD, EXACT, OBJ, WGB>), dim3(1), dim3(WGB), 0, st, S, P, par);

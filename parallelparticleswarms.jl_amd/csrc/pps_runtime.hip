@@ -309,8 +309,9 @@ static int swarm_prepare_launch(pps_swarm* s) {
     s->S.G = 1;
     s->S.block = 1024;
   } else {
-    s->S.block = PPS_BLK;
-    const int64_t tiles = (n + PPS_BLK - 1) / PPS_BLK;
+    const int blk = block_of(s->P.obj);
+    s->S.block = blk;
+    const int64_t tiles = (n + blk - 1) / blk;
     int per_cu = PPS_SYNC_GRID_PER_CU;
     if (mode == PPS_MODE_ASYNC) {
       per_cu = 8;
