@@ -359,16 +359,25 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
 }
 
 // Whole sync loop in one workgroup of BLOCK threads, one particle per thread
-// held in registers; the winner hands its position over through LDS.
+// held in registers.  The reduction slots alternate by iteration parity, so a
+// slot is rewritten only after the barrier of the following iteration.  For
+// MAXD <= 8 every thread also writes its candidate position to a parity
+// buffer in LDS before the reduction and every thread reads the winner's row
+// after it: one barrier per iteration.  Wider problems hand the winner's
+// position over through s_gb (two barriers).
 template <int MAXD, bool EXACT, int OBJ, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepArgs A,
                                                    float w, float wdamp, uint32_t t0,
                                                    int32_t T) {
-  __shared__ uint64_t s_red[BLOCK / 64];
+  constexpr int NW = BLOCK / 64;
+  constexpr bool POS_LDS = MAXD <= 8;
+  __shared__ uint64_t s_red[2][NW];
   __shared__ float s_gb[MAXD];
+  __shared__ float s_pos[POS_LDS ? 2 : 1][POS_LDS ? MAXD : 1][POS_LDS ? BLOCK : 1];
   const int d = EXACT ? MAXD : P.d;
-  fetch_gbest<BLOCK>(S, t0 & 1u, d, s_gb, s_red);
+  fetch_gbest<BLOCK>(S, t0 & 1u, d, s_gb, s_red[0]);
   const int64_t i = threadIdx.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const bool act = i < S.n;
   float x[MAXD], v[MAXD], pb[MAXD], pc = 0.0f;
 #pragma unroll
@@ -380,14 +389,15 @@ __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepA
     pc = S.pc[i];
   }
   const bool cur = A.gbest_from == PPS_GBEST_CURRENT;
-  for (int32_t k = 1; k <= T; ++k) {
-    float gb[MAXD];
+  float gb[MAXD];
 #pragma unroll
-    for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
+  for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
+  for (int32_t k = 1; k <= T; ++k) {
+    const int b = k & 1;
     uint64_t mykey = KEY_NONE;
     if (act) {
       velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
-                              S.epoch, S.key0, S.key1, d, P);
+                                   S.epoch, S.key0, S.key1, d, P);
       const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2), pc);
       if (c < pc) {
 #pragma unroll
@@ -395,14 +405,32 @@ __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepA
         pc = c;
       }
       mykey = make_key(pc, (uint32_t)i);
-    }
-    const uint64_t bk = block_min_u64<BLOCK>(mykey, s_red);  // ends with a barrier
-    if (act && mykey == bk) {
+      if constexpr (POS_LDS) {
 #pragma unroll
-      for (int j = 0; j < MAXD; ++j)
-        if (j < d) s_gb[j] = cur ? x[j] : pb[j];
+        for (int j = 0; j < MAXD; ++j)
+          if (j < d) s_pos[b][j][i] = cur ? x[j] : pb[j];
+      }
     }
+    const uint64_t wk = wave_min_u64(mykey);
+    if (lane == 0) s_red[b][wid] = wk;
     __syncthreads();
+    uint64_t r = s_red[b][0];
+#pragma unroll
+    for (int q = 1; q < NW; ++q) r = umin64(r, s_red[b][q]);
+    if constexpr (POS_LDS) {
+      const uint32_t wi = (uint32_t)r;
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_pos[b][j][wi] : 0.0f;
+    } else {
+      if (act && mykey == r) {
+#pragma unroll
+        for (int j = 0; j < MAXD; ++j)
+          if (j < d) s_gb[j] = cur ? x[j] : pb[j];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
+    }
     w = w * wdamp;
   }
   if (act) {
@@ -411,11 +439,18 @@ __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepA
     store_vec<MAXD>(pb, S.pb, S.npad, i, d);
     S.pc[i] = pc;
   }
-  // Hand-over state for whatever runs next (parity of t0 + T).
-  const uint64_t bk = block_min_u64<BLOCK>(act ? make_key(pc, (uint32_t)i) : KEY_NONE, s_red);
+  // Hand-over state for whatever runs next (parity of t0 + T): the last
+  // winner's position (held by every thread in gb) and its key.  The
+  // reduction uses the slot set the last iteration did not.
+  const uint64_t bk = block_min_u64<BLOCK>(act ? make_key(pc, (uint32_t)i) : KEY_NONE,
+                                           s_red[(T + 1) & 1]);
   const uint32_t par = (t0 + (uint32_t)T) & 1u;
-  if ((int)threadIdx.x < d) S.cand[(int64_t)par * d + threadIdx.x] = s_gb[threadIdx.x];
-  if (threadIdx.x == 0) S.bkey[par] = bk;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j)
+      if (j < d) S.cand[(int64_t)par * d + j] = gb[j];
+    S.bkey[par] = bk;
+  }
 }
 
 // Async: reference semantics per particle (src/kernels.jl:205-221):
