@@ -331,7 +331,7 @@ def main():
     args = ap.parse_args()
     legs = set() if args.no_extras else {x.strip() for x in args.extras.split(",") if x.strip()}
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): "
               "the swarm path has no multi-GPU launch", file=sys.stderr)
         sys.exit(2)
 

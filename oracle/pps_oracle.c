@@ -1,8 +1,8 @@
 /*
  * pps_oracle.c — CPU restatement of ParallelParticleSwarms.jl's particle-swarm
- * hot path.  TEST INFRASTRUCTURE ONLY: the parity tests, the smoke check and
- * the bench's CPU-baseline leg call it as the checker.  Nothing in the
- * product links or calls it.
+ * hot path.  TEST INFRASTRUCTURE ONLY: the parity tests and the smoke check
+ * compare the kernels' results with it, and the bench's CPU-baseline leg times
+ * it.  Nothing in the product links or calls it.
  *
  * It follows the reference (v1.4.0) as text, function by function:
  *   - particle layout: SPSOParticle{SVector{d,Float32},Float32} as an array of

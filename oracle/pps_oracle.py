@@ -2,8 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  The oracle restates ParallelParticleSwarms.jl's
 particle-swarm path on the CPU (see pps_oracle.c for the file:line map); the
-parity tests, the smoke check and the bench's CPU-baseline leg use it as the
-checker.  The product never imports this module.
+parity tests and the smoke check compare the kernels' results with it, and the
+bench's CPU-baseline leg times it.  The product never imports this module.
 
 Particle state crosses this wrapper in the kernels' SoA convention
 (x, v, best: d x n float32; best_cost: n float32) and is converted to and from
