@@ -66,7 +66,7 @@ struct BatchItem {
 };
 
 bool batch_supported(int obj, int d);
-// Parts of a large problem of n particles (tiles of 1024 particles, at most
+// Parts of a large problem of n particles (tiles of 256 particles, at most
 // BATCH_MAXPARTS).
 int batch_parts(int64_t n);
 // One slice: items[0..count) each run in one workgroup of BATCH_BLOCK threads.

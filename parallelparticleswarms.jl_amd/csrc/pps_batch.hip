@@ -2,11 +2,13 @@
 //
 // A launch ("slice") is a list of work items, one workgroup each:
 //   SMALL  one problem in one workgroup: its iteration loop runs until the
-//          problem ends or the slice's device-clock budget is spent (checked
-//          after every iteration by thread 0, broadcast through LDS); the
-//          winner's position is handed over through LDS.
+//          problem ends or one more iteration as long as the last would end
+//          past the slice's device-clock budget (checked after every
+//          iteration by thread 0, broadcast through LDS); the winner's
+//          position is handed over through LDS.
 //   PART   part q of k of a large problem: one iteration t over the particle
-//          tiles q, q+k, q+2k, ... (1024 particles per tile).  The global best
+//          tiles q, q+k, q+2k, ... (256 particles per tile, one per thread,
+//          eight coordinates per round of loads).  The global best
 //          of iteration t-1 is the minimum of the k part keys the previous
 //          slice wrote to parity (t-1)&1 of the problem's exchange area; the
 //          part writes its own key and candidate position to parity t&1, so
@@ -28,8 +30,10 @@ namespace pps {
 namespace {
 
 constexpr uint64_t KEY_NONE = ~0ull;
-constexpr int PPT = 4;  // particles in flight per thread
+constexpr int PPT = 4;  // particles in flight per thread (SMALL items)
 constexpr int TILE = BATCH_BLOCK * PPT;
+constexpr int PTILE = BATCH_BLOCK;  // particles per tile of a PART item
+constexpr int PCOORD = 8;           // coordinates per round of loads (PART)
 
 template <int BLOCK>
 __device__ __forceinline__ uint64_t block_min(uint64_t k, uint64_t* s_red) {
@@ -85,6 +89,14 @@ struct Swarm {
   float* __restrict__ PC;
 };
 
+// Coordinate j of particle i of a [d][np] state array at byte offset
+// i * 4 + j * np * 4: every state array of a batched problem is below 2 GiB
+// (n <= 2^24, d <= 32), so the offset fits 32 bits and the array base stays
+// in scalar registers.
+__device__ __forceinline__ float& at32(float* base, uint32_t off) {
+  return *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off);
+}
+
 __device__ __forceinline__ Swarm load_swarm(const BatchProb* B, ProbLds* L) {
   Swarm S;
   S.d = B->P.d;
@@ -110,13 +122,14 @@ __device__ __forceinline__ Swarm load_swarm(const BatchProb* B, ProbLds* L) {
 }
 
 // Uniform init (as init_position + objective) of the tiles first,
-// first + stride, ...; returns this thread's minimum key.
-template <int OBJ>
+// first + stride, ... of Q * BATCH_BLOCK particles; returns this thread's
+// minimum key.
+template <int OBJ, int Q>
 __device__ uint64_t init_sweep(const Swarm& S, const ProbLds* L, int64_t first, int64_t stride) {
   uint64_t mykey = KEY_NONE;
-  for (int64_t tile = first; tile * TILE < S.n; tile += stride) {
-    for (int q = 0; q < PPT; ++q) {
-      const int64_t i = tile * TILE + (int64_t)q * BATCH_BLOCK + threadIdx.x;
+  for (int64_t tile = first; tile * (Q * BATCH_BLOCK) < S.n; tile += stride) {
+    for (int q = 0; q < Q; ++q) {
+      const int64_t i = tile * (Q * BATCH_BLOCK) + (int64_t)q * BATCH_BLOCK + threadIdx.x;
       if (i >= S.n) continue;
       Accum<OBJ> acc;
       acc.start(S.d);
@@ -127,10 +140,10 @@ __device__ uint64_t init_sweep(const Swarm& S, const ProbLds* L, int64_t first, 
                             S.k0, S.k1);
         const uint32_t rr = (j & 3) == 0 ? r.x : (j & 3) == 1 ? r.y : (j & 3) == 2 ? r.z : r.w;
         const float x = u01(rr) * (L->ub[j] - L->lb[j]) + L->lb[j];
-        const int64_t o = (int64_t)j * S.np + i;
-        S.X[o] = x;
-        S.V[o] = 0.0f;
-        S.PB[o] = x;
+        const uint32_t o = (uint32_t)i * 4u + (uint32_t)j * ((uint32_t)S.np * 4u);
+        at32(S.X, o) = x;
+        at32(S.V, o) = 0.0f;
+        at32(S.PB, o) = x;
         acc.add(j, S.has_shift ? x - L->shift[j] : x, S.p1, S.p2);
       }
       S.PC[i] = acc.res;
@@ -150,6 +163,7 @@ __device__ uint64_t step_sweep(const Swarm& S, const ProbLds* L, int64_t first, 
                                uint32_t t, float w) {
   uint64_t mykey = KEY_NONE;
   const int d = S.d;
+  const uint32_t row4 = (uint32_t)S.np * 4u;
   for (int64_t tile = first; tile * TILE < S.n; tile += stride) {
     const int64_t i0 = tile * TILE + threadIdx.x;
     Accum<OBJ> acc[PPT];
@@ -163,11 +177,11 @@ __device__ uint64_t step_sweep(const Swarm& S, const ProbLds* L, int64_t first, 
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int j = 2 * k + h;
-          const int64_t o = (int64_t)j * S.np + i;
+          const uint32_t o = (uint32_t)i * 4u + (uint32_t)j * row4;
           const bool ok = i < S.n && j < d;
-          x[q][h] = ok ? S.X[o] : 0.0f;
-          v[q][h] = ok ? S.V[o] : 0.0f;
-          pbv[q][h] = ok ? S.PB[o] : 0.0f;
+          x[q][h] = ok ? at32(S.X, o) : 0.0f;
+          v[q][h] = ok ? at32(S.V, o) : 0.0f;
+          pbv[q][h] = ok ? at32(S.PB, o) : 0.0f;
         }
       }
 #pragma unroll
@@ -188,9 +202,9 @@ __device__ uint64_t step_sweep(const Swarm& S, const ProbLds* L, int64_t first, 
             const float vn = (a + b) + c;
             float xn = x[q][h] + vn;
             xn = jl_min(jl_max(xn, L->lb[j]), L->ub[j]);
-            const int64_t o = (int64_t)j * S.np + i;
-            S.X[o] = xn;
-            S.V[o] = vn;
+            const uint32_t o = (uint32_t)i * 4u + (uint32_t)j * row4;
+            at32(S.X, o) = xn;
+            at32(S.V, o) = vn;
             acc[q].add(j, S.has_shift ? xn - L->shift[j] : xn, S.p1, S.p2);
           }
         }
@@ -202,7 +216,9 @@ __device__ uint64_t step_sweep(const Swarm& S, const ProbLds* L, int64_t first, 
       if (i >= S.n) continue;
       float pc = S.PC[i];
       if (acc[q].res < pc) {
-        for (int j = 0; j < d; ++j) S.PB[(int64_t)j * S.np + i] = S.X[(int64_t)j * S.np + i];
+        for (int j = 0; j < d; ++j)
+          at32(S.PB, (uint32_t)i * 4u + (uint32_t)j * row4) =
+              at32(S.X, (uint32_t)i * 4u + (uint32_t)j * row4);
         pc = acc[q].res;
         S.PC[i] = pc;
       }
@@ -212,13 +228,76 @@ __device__ uint64_t step_sweep(const Swarm& S, const ProbLds* L, int64_t first, 
   return mykey;
 }
 
+// One iteration t of a PART item over the tiles first, first + stride, ... of
+// PTILE particles, one particle per thread, PCOORD coordinates of x, v and
+// the best position loaded per round (the same arithmetic as step_sweep);
+// returns this thread's minimum key.
+template <int OBJ>
+__device__ uint64_t part_step_sweep(const Swarm& S, const ProbLds* L, int64_t first,
+                                    int64_t stride, uint32_t t, float w) {
+  uint64_t mykey = KEY_NONE;
+  const int d = S.d;
+  for (int64_t tile = first; tile * PTILE < S.n; tile += stride) {
+    const int64_t i = tile * PTILE + threadIdx.x;
+    if (i >= S.n) continue;
+    const uint32_t i4 = (uint32_t)i * 4u, row4 = (uint32_t)S.np * 4u;
+    Accum<OBJ> acc;
+    acc.start(d);
+    for (int g = 0; g < d; g += PCOORD) {
+      float x[PCOORD], v[PCOORD], pbv[PCOORD];
+#pragma unroll
+      for (int h = 0; h < PCOORD; ++h) {
+        const int j = g + h;
+        const uint32_t o = i4 + (uint32_t)j * row4;
+        const bool ok = j < d;
+        x[h] = ok ? at32(S.X, o) : 0.0f;
+        v[h] = ok ? at32(S.V, o) : 0.0f;
+        pbv[h] = ok ? at32(S.PB, o) : 0.0f;
+      }
+#pragma unroll
+      for (int h2 = 0; h2 < PCOORD; h2 += 2) {
+        const int j0 = g + h2;
+        if (j0 >= d) break;
+        const u4 r = philox4x32_10(
+            u4{(uint32_t)i, t, (uint32_t)(j0 >> 1) | (PPS_TAG_STEP << 16), 0u}, S.k0, S.k1);
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int h = h2 + e, j = j0 + e;
+          if (j < d) {
+            const float r1 = u01(e == 0 ? r.x : r.z);
+            const float r2 = u01(e == 0 ? r.y : r.w);
+            const float a = w * v[h];
+            const float b = (S.c1 * r1) * (pbv[h] - x[h]);
+            const float c = (S.c2 * r2) * (L->gb[j] - x[h]);
+            const float vn = (a + b) + c;
+            float xn = x[h] + vn;
+            xn = jl_min(jl_max(xn, L->lb[j]), L->ub[j]);
+            const uint32_t o = i4 + (uint32_t)j * row4;
+            at32(S.X, o) = xn;
+            at32(S.V, o) = vn;
+            acc.add(j, S.has_shift ? xn - L->shift[j] : xn, S.p1, S.p2);
+          }
+        }
+      }
+    }
+    float pc = S.PC[i];
+    if (acc.res < pc) {
+      for (int j = 0; j < d; ++j) at32(S.PB, i4 + (uint32_t)j * row4) = at32(S.X, i4 + (uint32_t)j * row4);
+      pc = acc.res;
+      S.PC[i] = pc;
+    }
+    mykey = umin64(mykey, make_key(pc, (uint32_t)i));
+  }
+  return mykey;
+}
+
 // The owning thread of the workgroup minimum copies that particle's best
 // position (which it wrote itself, or which is unchanged) to dst[0..d).
 __device__ __forceinline__ void copy_winner(const Swarm& S, uint64_t mykey, uint64_t bk,
                                             float* dst) {
   if (bk != KEY_NONE && mykey == bk) {
-    const int64_t wi = (int64_t)(uint32_t)bk;
-    for (int j = 0; j < S.d; ++j) dst[j] = S.PB[(int64_t)j * S.np + wi];
+    const uint32_t wi = (uint32_t)bk;
+    for (int j = 0; j < S.d; ++j) dst[j] = at32(S.PB, wi * 4u + (uint32_t)j * ((uint32_t)S.np * 4u));
   }
 }
 
@@ -229,7 +308,7 @@ __device__ void run_small(BatchProb* B, uint64_t deadline, uint64_t* s_red, Prob
   const Swarm S = load_swarm(B, L);
   __syncthreads();
   if (!B->initialized) {
-    const uint64_t mykey = init_sweep<OBJ>(S, L, 0, 1);
+    const uint64_t mykey = init_sweep<OBJ, PPT>(S, L, 0, 1);
     const uint64_t bk = block_min<BATCH_BLOCK>(mykey, s_red);
     copy_winner(S, mykey, bk, L->gb);
     __syncthreads();
@@ -251,12 +330,22 @@ __device__ void run_small(BatchProb* B, uint64_t deadline, uint64_t* s_red, Prob
   __syncthreads();
   uint64_t bk = KEY_NONE;
   int32_t done = 0;
+  uint64_t prev = wall_clock64();
   while (done < steps) {
     const uint32_t t = t0 + (uint32_t)(++done);
-    const uint64_t mykey = step_sweep<OBJ>(S, L, 0, 1, t, w);
+    // d > 4: eight coordinates of one particle per round of loads; else
+    // four particles of one coordinate pair
+    const uint64_t mykey = S.d > 4 ? part_step_sweep<OBJ>(S, L, 0, 1, t, w)
+                                   : step_sweep<OBJ>(S, L, 0, 1, t, w);
     bk = block_min<BATCH_BLOCK>(mykey, s_red);
     copy_winner(S, mykey, bk, L->gb);
-    if (threadIdx.x == 0) *s_stop = wall_clock64() > deadline;
+    if (threadIdx.x == 0) {
+      // stop when another iteration as long as this one would end past the
+      // deadline (the first iteration always runs)
+      const uint64_t now = wall_clock64();
+      *s_stop = now + (now - prev) > deadline;
+      prev = now;
+    }
     __syncthreads();
     w = w * wdamp;
     if (*s_stop) break;
@@ -272,7 +361,8 @@ __device__ void run_small(BatchProb* B, uint64_t deadline, uint64_t* s_red, Prob
 }
 
 // Global best of a large problem after iteration `t`: minimum of its part keys
-// of parity t & 1; the winning part is the one owning the winner's tile.
+// of parity t & 1; the winning part is the one owning the winner's tile
+// (tiles of PTILE particles dealt round robin to the k parts).
 __device__ __forceinline__ uint64_t part_gbest(const BatchXchg* XC, int k, uint32_t t, int d,
                                                float* dst, uint64_t* s_red) {
   const uint32_t par = t & 1u;
@@ -280,7 +370,7 @@ __device__ __forceinline__ uint64_t part_gbest(const BatchXchg* XC, int k, uint3
   for (int q = threadIdx.x; q < k; q += BATCH_BLOCK) key = umin64(key, XC->key[par][q]);
   key = block_min<BATCH_BLOCK>(key, s_red);
   const int64_t wi = (int64_t)(uint32_t)key;
-  const int wpart = (int)((wi / TILE) % k);
+  const int wpart = (int)((wi / PTILE) % k);
   if ((int)threadIdx.x < d) dst[threadIdx.x] = XC->cand[par][wpart][threadIdx.x];
   __syncthreads();
   return key;
@@ -294,10 +384,10 @@ __device__ void run_part(BatchProb* B, const BatchItem& it, BatchXchg* XC, uint6
   __syncthreads();
   uint64_t mykey;
   if (it.t == 0) {
-    mykey = init_sweep<OBJ>(S, L, it.part, it.nparts);
+    mykey = init_sweep<OBJ, 1>(S, L, it.part, it.nparts);
   } else {
     part_gbest(XC, it.nparts, it.t - 1u, S.d, L->gb, s_red);
-    mykey = step_sweep<OBJ>(S, L, it.part, it.nparts, it.t, it.w);
+    mykey = part_step_sweep<OBJ>(S, L, it.part, it.nparts, it.t, it.w);
   }
   const uint64_t bk = block_min<BATCH_BLOCK>(mykey, s_red);
   const uint32_t par = it.t & 1u;
@@ -368,7 +458,7 @@ bool batch_supported(int obj, int d) {
 }
 
 int batch_parts(int64_t n) {
-  const int64_t tiles = (n + TILE - 1) / TILE;
+  const int64_t tiles = (n + PTILE - 1) / PTILE;
   return (int)(tiles < BATCH_MAXPARTS ? tiles : BATCH_MAXPARTS);
 }
 

@@ -48,14 +48,19 @@ double now_s() {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Swarms with n * d at or above this run as `batch_parts(n)` workgroups, one
-// iteration per slice; smaller ones run their own loop in one workgroup.
-constexpr int64_t kLargeWork = 32768;
+// iteration per slice; smaller ones run their own loop in one workgroup.  A
+// slice lasts at least the longest single iteration of a SMALL item (about
+// n * d / 2048 dependent rounds of loads in one workgroup), and one slice is
+// one iteration of every large problem: at 32768 that was ~16 rounds and an
+// 80 us slice whatever the budget below; at 8192, with PART items of 256
+// particles, a slice follows the budget.
+constexpr int64_t kLargeWork = 8192;
 // Device-clock budget of SMALL items while large problems are resident, so a
 // slice (and with it one iteration of every large problem) stays short.  On
-// the 4096-request mix at half the saturating rate, 40 us against 100 us
-// brought the p99 latency from 140 to 83 ms with the same throughput; 20 and
-// 10 us raised the median and left the p99 where it was (slice overhead).
-constexpr int kSliceUsWithLarge = 40;
+// the 4096-request mix at half the saturating rate: p50/p99 13.5/30.8 ms at
+// 20 us, 17.0/47.2 at 40 us, 13.4/30.7 at 10 us (the slice's fixed cost
+// dominates there), with the same throughput.
+constexpr int kSliceUsWithLarge = 20;
 
 enum { R_PENDING = 0, R_RUNNING = 1, R_DONE = 2, R_CANCELLED = 3, R_FAILED = 4 };
 
