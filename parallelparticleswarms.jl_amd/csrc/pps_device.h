@@ -212,6 +212,52 @@ __device__ __forceinline__ float ode_sse_rk4(const float* th, const ProbArgs& P)
   return loss;
 }
 
+// The same Lotka-Volterra RK4 loss computed by two lanes of one particle:
+// the even lane of a lane pair carries u1, the odd lane u2, and each stage
+// reads both components of the pair by DPP.  Both components' derivatives
+// then have one form, f = X * own + (Y * u1) * u2:
+//   even lane: X = p1,  Y = -p2:  p1 u1 + ((-p2) u1) u2 = p1 u1 - (p2 u1) u2
+//   odd lane:  X = -p3, Y = p4:   (-p3) u2 + (p4 u1) u2
+// (negating a factor negates the rounded product exactly, and a - b is
+// a + (-b)), so every stage value, step and the loss (e1^2 + e2^2 in that
+// order, in both lanes) equal ode_sse_rk4's bit for bit.  Both lanes of each
+// pair must be active and hold the same th.
+__device__ __forceinline__ float pair_even(float x) {  // quad_perm [0,0,2,2]
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xA0, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float pair_odd(float x) {  // quad_perm [1,1,3,3]
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xF5, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float lv_pair_rhs(float X, float Y, float a) {
+  return X * a + (Y * pair_even(a)) * pair_odd(a);
+}
+__device__ __forceinline__ float ode_sse_rk4_lv_pair(const float* th, const ProbArgs& P) {
+  const bool odd = (threadIdx.x & 1u) != 0;
+  const float X = odd ? -th[2] : th[0];
+  const float Y = odd ? th[3] : -th[1];
+  float u = odd ? P.par[1] : P.par[0];
+  const float dt = P.par[2];
+  const float h2 = dt * 0.5f;
+  const float h6 = dt / 6.0f;
+  float loss = 0.0f;
+  const float* obs = P.obs + (odd ? 1 : 0);
+  for (int k = 0; k < P.n_obs; ++k) {
+    if (k > 0) {
+      for (int s = 0; s < P.steps_per_obs; ++s) {
+        const float k1 = lv_pair_rhs(X, Y, u);
+        const float k2 = lv_pair_rhs(X, Y, u + h2 * k1);
+        const float k3 = lv_pair_rhs(X, Y, u + h2 * k2);
+        const float k4 = lv_pair_rhs(X, Y, u + dt * k3);
+        u = u + h6 * (((k1 + 2.0f * k2) + 2.0f * k3) + k4);
+      }
+    }
+    const float e = obs[2 * k] - u;
+    const float sq = e * e;
+    loss = loss + (pair_even(sq) + pair_odd(sq));
+  }
+  return loss;
+}
+
 // ---------------------------------------------------------------------------
 // Tsitouras 5(4) (Ch. Tsitouras, Comput. Math. Appl. 62 (2011) 770-775), the
 // method of the reference's default ode_alg = GPUTsit5() (src/ode_pso.jl:54-170):

@@ -358,6 +358,87 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
   }
 }
 
+// Sync step of the Lotka-Volterra RK4 estimation with two lanes per particle:
+// a 64-particle tile (the tiling of k_init and k_sync_step, so block keys and
+// candidates mean the same) runs on 128 threads, lanes 2m and 2m+1 holding
+// particle m.  Both lanes load the particle and compute the same update; the
+// RK4 loss, a chain of ~1000 dependent steps per evaluation, is split between
+// them (ode_sse_rk4_lv_pair), which halves the chain each lane issues and
+// doubles the waves over the SIMDs.  The even lane writes the state.
+template <bool OFF32>
+__global__ __launch_bounds__(128) void k_sync_step_lv_pair(SwarmDev S, ProbArgs P, StepArgs A,
+                                                           float w, uint32_t k) {
+  constexpr int MAXD = 4, TP = 64, BLOCK = 2 * TP;
+  __shared__ uint64_t s_red[BLOCK / 64];
+  __shared__ float s_gb[MAXD];
+  const uint32_t t = *S.tbase + k;
+  const int d = MAXD;
+  const int G = S.G;
+  const int tid = threadIdx.x;
+  const bool cur = A.gbest_from == PPS_GBEST_CURRENT;
+  const uint64_t* keys = S.bkey + (int64_t)((t - 1u) & 1u) * G;
+  uint64_t gk = KEY_NONE;
+  for (int j = tid; j < G; j += BLOCK) gk = umin64(gk, keys[j]);
+  gk = block_min_u64<BLOCK>(gk, s_red);
+  {
+    const uint32_t gidx = (uint32_t)gk;
+    const int gblk = (int)((gidx / (uint32_t)S.block) % (uint32_t)G);
+    if (tid < d) s_gb[tid] = S.cand[((int64_t)((t - 1u) & 1u) * G + gblk) * d + tid];
+  }
+  __syncthreads();
+  float gb[MAXD];
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) gb[j] = s_gb[j];
+  const bool even = (tid & 1) == 0;
+  uint64_t mykey = KEY_NONE;
+  float hold[MAXD];
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) hold[j] = 0.0f;
+  for (int64_t tile = blockIdx.x; tile * TP < S.n; tile += gridDim.x) {
+    const int64_t i = tile * TP + (tid >> 1);
+    if (i < S.n) {  // the same for both lanes of a pair
+      const RowAddr<OFF32> R(i, S.npad);
+      float x[MAXD], v[MAXD], pb[MAXD];
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j) {
+        x[j] = *R.at(S.x, j);
+        v[j] = *R.at(S.v, j);
+        pb[j] = *R.at(S.pb, j);
+      }
+      float pc = S.pc[i];
+      velocity_position<MAXD, PPS_OBJ_LOTKA_VOLTERRA>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i,
+                                                      t, S.epoch, S.key0, S.key1, d, P);
+      float z[MAXD];
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j) z[j] = x[j] - P.shift[j];  // as objective()
+      const float c = ode_sse_rk4_lv_pair(z, P);
+      const bool imp = c < pc;
+      if (even) {
+#pragma unroll
+        for (int j = 0; j < MAXD; ++j) {
+          *R.at(S.x, j) = x[j];
+          *R.at(S.v, j) = v[j];
+          if (imp) *R.at(S.pb, j) = x[j];
+        }
+        if (imp) S.pc[i] = c;
+      }
+      if (imp) pc = c;
+      const uint64_t key = make_key(pc, (uint32_t)i);
+      const bool win = key < mykey;
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j)
+        if (win) hold[j] = (cur || imp) ? x[j] : pb[j];
+      mykey = umin64(mykey, key);
+    }
+  }
+  const uint64_t bk = block_min_u64<BLOCK>(mykey, s_red);
+  if (bk != KEY_NONE && mykey == bk && even) {
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j) S.cand[((int64_t)(t & 1u) * G + blockIdx.x) * d + j] = hold[j];
+  }
+  if (tid == 0) S.bkey[(int64_t)(t & 1u) * G + blockIdx.x] = bk;
+}
+
 // Whole sync loop in one workgroup of BLOCK threads, one particle per thread
 // held in registers.  The reduction slots alternate by iteration parity, so a
 // slot is rewritten only after the barrier of the following iteration.  For
@@ -615,6 +696,15 @@ struct Launch {
     const uint64_t d = (uint64_t)(EXACT ? MAXD : P.d);
     const bool off32 = (uint64_t)S.npad * d * 4u < (1ull << 32);
     const bool nt = (uint64_t)S.npad * (3u * d + 1u) * 4u > PPS_NT_MIN_STATE_BYTES;
+    if constexpr (OBJ == PPS_OBJ_LOTKA_VOLTERRA) {
+      if (!(S.block == WGB && S.G == 1) && S.block == 64) {
+        if (off32)
+          hipLaunchKernelGGL(k_sync_step_lv_pair<true>, dim3(S.G), dim3(128), 0, st, S, P, A, w, k);
+        else
+          hipLaunchKernelGGL(k_sync_step_lv_pair<false>, dim3(S.G), dim3(128), 0, st, S, P, A, w, k);
+        return hipGetLastError();
+      }
+    }
     if (S.block == WGB && S.G == 1)
       hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, WGB, true, false>), dim3(1), dim3(WGB), 0,
                          st, S, P, A, w, k);
