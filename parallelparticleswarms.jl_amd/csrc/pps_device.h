@@ -293,53 +293,90 @@ __device__ __forceinline__ float ode_sse_rk4_lv_pair(const float* th, const Prob
 #define TS_E6 -0.45808210592918697f
 #define TS_E7 0.015151515151515152f
 
-// One Tsit5 step of the ODE model from (u1, u2) with k1 = f(u) in
-// ka[0], kb[0]: stage sums left to right, y_i = u + h * sum_j a_ij k_j; fills
-// ka/kb[1..6] with k2..k7 (k7 = f(new state), the next step's k1) and returns
-// the new state and, when ERR, the error estimate h * sum_j btilde_j k_j.
-template <int MODEL, bool ERR>
-__device__ __forceinline__ void tsit5_stages(float u1, float u2, float h, const float* th,
-                                             float (&ka)[7], float (&kb)[7], float& n1, float& n2,
-                                             float& e1, float& e2) {
-  const float p1 = th[0], p2 = th[1], p3 = th[2], p4 = th[3];
-  ode_rhs<MODEL>(u1 + h * (TS_A21 * ka[0]), u2 + h * (TS_A21 * kb[0]), p1, p2, p3, p4, ka[1], kb[1]);
-  ode_rhs<MODEL>(u1 + h * (TS_A31 * ka[0] + TS_A32 * ka[1]), u2 + h * (TS_A31 * kb[0] + TS_A32 * kb[1]),
-         p1, p2, p3, p4, ka[2], kb[2]);
-  ode_rhs<MODEL>(u1 + h * ((TS_A41 * ka[0] + TS_A42 * ka[1]) + TS_A43 * ka[2]),
-         u2 + h * ((TS_A41 * kb[0] + TS_A42 * kb[1]) + TS_A43 * kb[2]), p1, p2, p3, p4, ka[3],
-         kb[3]);
-  ode_rhs<MODEL>(u1 + h * (((TS_A51 * ka[0] + TS_A52 * ka[1]) + TS_A53 * ka[2]) + TS_A54 * ka[3]),
-         u2 + h * (((TS_A51 * kb[0] + TS_A52 * kb[1]) + TS_A53 * kb[2]) + TS_A54 * kb[3]), p1, p2,
-         p3, p4, ka[4], kb[4]);
-  ode_rhs<MODEL>(u1 + h * ((((TS_A61 * ka[0] + TS_A62 * ka[1]) + TS_A63 * ka[2]) + TS_A64 * ka[3]) +
-                   TS_A65 * ka[4]),
-         u2 + h * ((((TS_A61 * kb[0] + TS_A62 * kb[1]) + TS_A63 * kb[2]) + TS_A64 * kb[3]) +
-                   TS_A65 * kb[4]),
-         p1, p2, p3, p4, ka[5], kb[5]);
-  n1 = u1 + h * (((((TS_B1 * ka[0] + TS_B2 * ka[1]) + TS_B3 * ka[2]) + TS_B4 * ka[3]) +
-                  TS_B5 * ka[4]) + TS_B6 * ka[5]);
-  n2 = u2 + h * (((((TS_B1 * kb[0] + TS_B2 * kb[1]) + TS_B3 * kb[2]) + TS_B4 * kb[3]) +
-                  TS_B5 * kb[4]) + TS_B6 * kb[5]);
-  ode_rhs<MODEL>(n1, n2, p1, p2, p3, p4, ka[6], kb[6]);
-  if (ERR) {
-    e1 = h * ((((((TS_E1 * ka[0] + TS_E2 * ka[1]) + TS_E3 * ka[2]) + TS_E4 * ka[3]) +
-                TS_E5 * ka[4]) + TS_E6 * ka[5]) + TS_E7 * ka[6]);
-    e2 = h * ((((((TS_E1 * kb[0] + TS_E2 * kb[1]) + TS_E3 * kb[2]) + TS_E4 * kb[3]) +
-                TS_E5 * kb[4]) + TS_E6 * kb[5]) + TS_E7 * kb[6]);
+// The components an ODE loss integrates in one lane.  OdeLane: both
+// components of a two-component model.  LvPair: Lotka-Volterra on two lanes
+// per particle (ode_sse_rk4_lv_pair), the lane's own component.  Component c
+// of sample k is obs[2 k + first() + c]; sumsq() is e1^2 + e2^2 over the
+// particle's two components, in that order.
+template <int MODEL>
+struct OdeLane {
+  static constexpr int NC = 2;
+  float p1, p2, p3, p4;
+  __device__ __forceinline__ explicit OdeLane(const float* th)
+      : p1(th[0]), p2(th[1]), p3(th[2]), p4(th[3]) {}
+  __device__ __forceinline__ int first() const { return 0; }
+  __device__ __forceinline__ void start(const ProbArgs& P, float (&u)[NC]) const {
+    u[0] = P.par[0];
+    u[1] = P.par[1];
   }
-}
+  __device__ __forceinline__ void rhs(const float (&u)[NC], float (&f)[NC]) const {
+    ode_rhs<MODEL>(u[0], u[1], p1, p2, p3, p4, f[0], f[1]);
+  }
+  __device__ __forceinline__ float sumsq(const float (&e)[NC]) const {
+    return e[0] * e[0] + e[1] * e[1];
+  }
+};
+struct LvPair {
+  static constexpr int NC = 1;
+  bool odd;
+  float X, Y;
+  __device__ __forceinline__ explicit LvPair(const float* th)
+      : odd((threadIdx.x & 1u) != 0), X(odd ? -th[2] : th[0]), Y(odd ? th[3] : -th[1]) {}
+  __device__ __forceinline__ int first() const { return odd ? 1 : 0; }
+  __device__ __forceinline__ void start(const ProbArgs& P, float (&u)[NC]) const {
+    u[0] = odd ? P.par[1] : P.par[0];
+  }
+  __device__ __forceinline__ void rhs(const float (&u)[NC], float (&f)[NC]) const {
+    f[0] = lv_pair_rhs(X, Y, u[0]);
+  }
+  __device__ __forceinline__ float sumsq(const float (&e)[NC]) const {
+    const float q = e[0] * e[0];
+    return pair_even(q) + pair_odd(q);
+  }
+};
 
-// The same step with k1 passed and k7 returned as scalars.
-template <int MODEL, bool ERR>
-__device__ __forceinline__ void tsit5_step(float u1, float u2, float k1a, float k1b, float h,
-                                           const float* th, float& n1, float& n2, float& k7a,
-                                           float& k7b, float& e1, float& e2) {
-  float ka[7], kb[7];
-  ka[0] = k1a;
-  kb[0] = k1b;
-  tsit5_stages<MODEL, ERR>(u1, u2, h, th, ka, kb, n1, n2, e1, e2);
-  k7a = ka[6];
-  k7b = kb[6];
+// One Tsit5 step from u with k1 = f(u) in k[0]: stage sums left to right,
+// y_i = u + h * sum_j a_ij k_j, per component; fills k[1..6] with k2..k7
+// (k7 = f(new state), the next step's k1) and returns the new state and,
+// when ERR, the error estimate h * sum_j btilde_j k_j.
+template <class SYS, bool ERR>
+__device__ __forceinline__ void tsit5_stages(const SYS& sys, const float (&u)[SYS::NC], float h,
+                                             float (&k)[7][SYS::NC], float (&n)[SYS::NC],
+                                             float (&e)[SYS::NC]) {
+  constexpr int NC = SYS::NC;
+  float y[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) y[c] = u[c] + h * (TS_A21 * k[0][c]);
+  sys.rhs(y, k[1]);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) y[c] = u[c] + h * (TS_A31 * k[0][c] + TS_A32 * k[1][c]);
+  sys.rhs(y, k[2]);
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    y[c] = u[c] + h * ((TS_A41 * k[0][c] + TS_A42 * k[1][c]) + TS_A43 * k[2][c]);
+  sys.rhs(y, k[3]);
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    y[c] = u[c] + h * (((TS_A51 * k[0][c] + TS_A52 * k[1][c]) + TS_A53 * k[2][c]) +
+                       TS_A54 * k[3][c]);
+  sys.rhs(y, k[4]);
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    y[c] = u[c] + h * ((((TS_A61 * k[0][c] + TS_A62 * k[1][c]) + TS_A63 * k[2][c]) +
+                        TS_A64 * k[3][c]) + TS_A65 * k[4][c]);
+  sys.rhs(y, k[5]);
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    n[c] = u[c] + h * (((((TS_B1 * k[0][c] + TS_B2 * k[1][c]) + TS_B3 * k[2][c]) +
+                         TS_B4 * k[3][c]) + TS_B5 * k[4][c]) + TS_B6 * k[5][c]);
+  sys.rhs(n, k[6]);
+  if (ERR) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      e[c] = h * ((((((TS_E1 * k[0][c] + TS_E2 * k[1][c]) + TS_E3 * k[2][c]) +
+                     TS_E4 * k[3][c]) + TS_E5 * k[4][c]) + TS_E6 * k[5][c]) +
+                  TS_E7 * k[6][c]);
+  }
 }
 
 // Tsit5's continuous extension (Tsitouras 2011, Sec. 4; 4th order):
@@ -370,9 +407,9 @@ __device__ __forceinline__ void tsit5_step(float u1, float u2, float k1a, float 
 #define TS_R73 -4.0f
 #define TS_R74 2.5f
 
-__device__ __forceinline__ void tsit5_interp(float u1, float u2, float h, float tt,
-                                             const float (&ka)[7], const float (&kb)[7],
-                                             float& v1, float& v2) {
+template <int NC>
+__device__ __forceinline__ void tsit5_interp(const float (&u)[NC], float h, float tt,
+                                             const float (&k)[7][NC], float (&v)[NC]) {
   const float b1 = tt * (TS_R11 + tt * (TS_R12 + tt * (TS_R13 + tt * TS_R14)));
   const float b2 = tt * (tt * (TS_R22 + tt * (TS_R23 + tt * TS_R24)));
   const float b3 = tt * (tt * (TS_R32 + tt * (TS_R33 + tt * TS_R34)));
@@ -380,10 +417,10 @@ __device__ __forceinline__ void tsit5_interp(float u1, float u2, float h, float 
   const float b5 = tt * (tt * (TS_R52 + tt * (TS_R53 + tt * TS_R54)));
   const float b6 = tt * (tt * (TS_R62 + tt * (TS_R63 + tt * TS_R64)));
   const float b7 = tt * (tt * (TS_R72 + tt * (TS_R73 + tt * TS_R74)));
-  v1 = u1 + h * ((((((b1 * ka[0] + b2 * ka[1]) + b3 * ka[2]) + b4 * ka[3]) + b5 * ka[4]) +
-                  b6 * ka[5]) + b7 * ka[6]);
-  v2 = u2 + h * ((((((b1 * kb[0] + b2 * kb[1]) + b3 * kb[2]) + b4 * kb[3]) + b5 * kb[4]) +
-                  b6 * kb[5]) + b7 * kb[6]);
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    v[c] = u[c] + h * ((((((b1 * k[0][c] + b2 * k[1][c]) + b3 * k[2][c]) + b4 * k[3][c]) +
+                         b5 * k[4][c]) + b6 * k[5][c]) + b7 * k[6][c]);
 }
 
 // x^(1/5) for x > 0 from basic IEEE operations only (an exponent-halving seed
@@ -426,33 +463,37 @@ __device__ __forceinline__ float root5(float x) {
 // turns stiff from running to the step limit.  A NaN bound never stops the
 // sum.  (With fixed steps every lane of a wavefront integrates equally long,
 // and stopping early was measured to gain nothing.)
-template <int MODEL, bool ADAPTIVE>
-__device__ __forceinline__ float ode_sse_tsit5(const float* th, const ProbArgs& P, float bound) {
-  float u1 = P.par[0], u2 = P.par[1];
-  float k1a, k1b;
-  ode_rhs<MODEL>(u1, u2, th[0], th[1], th[2], th[3], k1a, k1b);
+template <class SYS, bool ADAPTIVE>
+__device__ __forceinline__ float ode_sse_tsit5(const SYS& sys, const ProbArgs& P, float bound) {
+  constexpr int NC = SYS::NC;
+  float u[NC], k[7][NC];
+  sys.start(P, u);
+  sys.rhs(u, k[0]);
   float loss = 0.0f;
-  const float* obs = P.obs;
+  const float* obs = P.obs + sys.first();
   if (P.n_obs <= 0) return loss;
   {
-    const float e1 = obs[0] - u1;
-    const float e2 = obs[1] - u2;
-    loss = loss + (e1 * e1 + e2 * e2);
+    float e[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) e[c] = obs[c] - u[c];
+    loss = loss + sys.sumsq(e);
   }
   if (!ADAPTIVE) {
     const float dt = P.par[2];
-    for (int k = 1; k < P.n_obs; ++k) {
+    for (int o = 1; o < P.n_obs; ++o) {
       for (int s = 0; s < P.steps_per_obs; ++s) {
-        float n1, n2, k7a, k7b, e1, e2;
-        tsit5_step<MODEL, false>(u1, u2, k1a, k1b, dt, th, n1, n2, k7a, k7b, e1, e2);
-        u1 = n1;
-        u2 = n2;
-        k1a = k7a;
-        k1b = k7b;
+        float n[NC], e[NC];
+        tsit5_stages<SYS, false>(sys, u, dt, k, n, e);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          u[c] = n[c];
+          k[0][c] = k[6][c];
+        }
       }
-      const float e1 = obs[2 * k] - u1;
-      const float e2 = obs[2 * k + 1] - u2;
-      loss = loss + (e1 * e1 + e2 * e2);
+      float e[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) e[c] = obs[2 * o + c] - u[c];
+      loss = loss + sys.sumsq(e);
     }
     return loss;
   }
@@ -464,27 +505,31 @@ __device__ __forceinline__ float ode_sse_tsit5(const float* th, const ProbArgs& 
   const int32_t max_steps = P.par[6] > 0.0f ? (int32_t)P.par[6] : ODE_MAX_STEPS;
   float t = 0.0f, dt = P.par[3];
   int32_t steps = 0;
-  int k = 1;
-  while (k < P.n_obs) {
-    const float t_next = (float)k * P.par[2];
+  int o = 1;
+  while (o < P.n_obs) {
+    const float t_next = (float)o * P.par[2];
     if (!(t < t_next)) {
-      const float e1 = obs[2 * k] - u1;
-      const float e2 = obs[2 * k + 1] - u2;
-      loss = loss + (e1 * e1 + e2 * e2);
+      float e[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) e[c] = obs[2 * o + c] - u[c];
+      loss = loss + sys.sumsq(e);
       if (loss >= bound) return loss;
-      ++k;
+      ++o;
       continue;
     }
     const bool hit = !(t + dt < t_next);
     const float h = hit ? t_next - t : dt;
     if (++steps > max_steps || !(t + h > t)) return __builtin_inff();
-    float n1, n2, k7a, k7b, e1, e2;
-    tsit5_step<MODEL, true>(u1, u2, k1a, k1b, h, th, n1, n2, k7a, k7b, e1, e2);
-    const float a1 = __builtin_fabsf(u1), b1 = __builtin_fabsf(n1);
-    const float a2 = __builtin_fabsf(u2), b2 = __builtin_fabsf(n2);
-    const float s1 = e1 / (atol + (a1 > b1 ? a1 : b1) * rtol);
-    const float s2 = e2 / (atol + (a2 > b2 ? a2 : b2) * rtol);
-    const float eest = sqrtf((s1 * s1 + s2 * s2) * 0.5f);
+    float kk[7][NC], n[NC], e[NC], sc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) kk[0][c] = k[0][c];
+    tsit5_stages<SYS, true>(sys, u, h, kk, n, e);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const float a = __builtin_fabsf(u[c]), b = __builtin_fabsf(n[c]);
+      sc[c] = e[c] / (atol + (a > b ? a : b) * rtol);
+    }
+    const float eest = sqrtf(sys.sumsq(sc) * 0.5f);
     float q = 5.0f;
     if (eest <= 3.0e38f) {
       q = root5(eest) / 0.9f;
@@ -493,10 +538,11 @@ __device__ __forceinline__ float ode_sse_tsit5(const float* th, const ProbArgs& 
     const float hnew = h / q;
     if (eest <= 1.0f) {
       t = hit ? t_next : t + h;
-      u1 = n1;
-      u2 = n2;
-      k1a = k7a;
-      k1b = k7b;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        u[c] = n[c];
+        k[0][c] = kk[6][c];
+      }
       dt = hit ? (hnew > dt ? hnew : dt) : hnew;
     } else {
       dt = hnew;
@@ -512,34 +558,42 @@ __device__ __forceinline__ float ode_sse_tsit5(const float* th, const ProbArgs& 
 // the continuous extension at theta = (t_k - t) / h.  Adaptive control, step
 // limit par[6] and the +Inf results as ode_sse_tsit5; fixed steps carry no
 // limit (the host bounds their number).  Loss as ode_sse_rk4.
-template <int MODEL, bool ADAPTIVE>
-__device__ __forceinline__ float ode_sse_tsit5_saveat(const float* th, const ProbArgs& P,
+template <class SYS, bool ADAPTIVE>
+__device__ __forceinline__ float ode_sse_tsit5_saveat(const SYS& sys, const ProbArgs& P,
                                                       float bound) {
-  float u1 = P.par[0], u2 = P.par[1];
-  float ka[7], kb[7];
-  ode_rhs<MODEL>(u1, u2, th[0], th[1], th[2], th[3], ka[0], kb[0]);
-  const float* obs = P.obs;
+  constexpr int NC = SYS::NC;
+  float u[NC], k[7][NC];
+  sys.start(P, u);
+  sys.rhs(u, k[0]);
+  const float* obs = P.obs + sys.first();
   const float tau = P.par[2];
   const float t_end = (float)(P.n_obs - 1) * tau;
-  const float o1 = obs[0] - u1, o2 = obs[1] - u2;
-  float loss = o1 * o1 + o2 * o2;
-  int k = 1;
+  float loss;
+  {
+    float e[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) e[c] = obs[c] - u[c];
+    loss = sys.sumsq(e);
+  }
+  int o = 1;
   float t = 0.0f, dt = P.par[3];
   int32_t steps = 0;
   const int32_t max_steps =
       ADAPTIVE ? (P.par[6] > 0.0f ? (int32_t)P.par[6] : ODE_MAX_STEPS) : 0x7fffffff;
-  while (k < P.n_obs) {
+  while (o < P.n_obs) {
     const bool last = !(t + dt < t_end);
     const float h = last ? t_end - t : dt;
     if (++steps > max_steps || !(t + h > t)) return __builtin_inff();
-    float n1, n2, e1, e2;
-    tsit5_stages<MODEL, ADAPTIVE>(u1, u2, h, th, ka, kb, n1, n2, e1, e2);
+    float n[NC], e[NC];
+    tsit5_stages<SYS, ADAPTIVE>(sys, u, h, k, n, e);
     if (ADAPTIVE) {
-      const float a1 = __builtin_fabsf(u1), b1 = __builtin_fabsf(n1);
-      const float a2 = __builtin_fabsf(u2), b2 = __builtin_fabsf(n2);
-      const float s1 = e1 / (P.par[4] + (a1 > b1 ? a1 : b1) * P.par[5]);
-      const float s2 = e2 / (P.par[4] + (a2 > b2 ? a2 : b2) * P.par[5]);
-      const float eest = sqrtf((s1 * s1 + s2 * s2) * 0.5f);
+      float sc[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const float a = __builtin_fabsf(u[c]), b = __builtin_fabsf(n[c]);
+        sc[c] = e[c] / (P.par[4] + (a > b ? a : b) * P.par[5]);
+      }
+      const float eest = sqrtf(sys.sumsq(sc) * 0.5f);
       float q = 5.0f;
       if (eest <= 3.0e38f) {
         q = root5(eest) / 0.9f;
@@ -549,23 +603,41 @@ __device__ __forceinline__ float ode_sse_tsit5_saveat(const float* th, const Pro
       if (!(eest <= 1.0f)) continue;
     }
     const float t_new = last ? t_end : t + h;
-    while (k < P.n_obs) {
-      const float tk = (float)k * tau;
+    while (o < P.n_obs) {
+      const float tk = (float)o * tau;
       if (tk > t_new) break;
-      float v1, v2;
-      tsit5_interp(u1, u2, h, (tk - t) / h, ka, kb, v1, v2);
-      const float d1 = obs[2 * k] - v1, d2 = obs[2 * k + 1] - v2;
-      loss = loss + (d1 * d1 + d2 * d2);
+      float v[NC], dv[NC];
+      tsit5_interp<NC>(u, h, (tk - t) / h, k, v);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) dv[c] = obs[2 * o + c] - v[c];
+      loss = loss + sys.sumsq(dv);
       if (ADAPTIVE && loss >= bound) return loss;
-      ++k;
+      ++o;
     }
     t = t_new;
-    u1 = n1;
-    u2 = n2;
-    ka[0] = ka[6];
-    kb[0] = kb[6];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      u[c] = n[c];
+      k[0][c] = k[6][c];
+    }
   }
   return loss;
+}
+
+// Lotka-Volterra loss of objective OBJ on two lanes per particle (LvPair,
+// ode_sse_rk4_lv_pair): the value objective<4, OBJ> returns for z, bit for
+// bit.  Both lanes of each pair must be active and hold the same z.
+template <int OBJ>
+__device__ __forceinline__ float lv_pair_objective(const float (&z)[4], const ProbArgs& P,
+                                                   float bound) {
+  static_assert(is_ode_objective(OBJ) && ode_model(OBJ) == PPS_ODE_LOTKA_VOLTERRA, "LV only");
+  constexpr int S = ode_solver(OBJ);
+  if constexpr (S == PPS_ODE_RK4) return ode_sse_rk4_lv_pair(z, P);
+  else if constexpr (S == PPS_ODE_TSIT5) return ode_sse_tsit5<LvPair, false>(LvPair(z), P, bound);
+  else if constexpr (S == PPS_ODE_ATSIT5) return ode_sse_tsit5<LvPair, true>(LvPair(z), P, bound);
+  else if constexpr (S == PPS_ODE_TSIT5_SAVEAT)
+    return ode_sse_tsit5_saveat<LvPair, false>(LvPair(z), P, bound);
+  else return ode_sse_tsit5_saveat<LvPair, true>(LvPair(z), P, bound);
 }
 
 // Objective f(x - shift) over a register array (compile-time capacity MAXD,
@@ -637,10 +709,11 @@ __device__ __forceinline__ float objective(const float (&xin)[MAXD], const int d
   } else if constexpr (is_ode_objective(OBJ)) {
     constexpr int M = ode_model(OBJ), S = ode_solver(OBJ);
     if constexpr (S == PPS_ODE_RK4) return ode_sse_rk4<M>(z, P);
-    else if constexpr (S == PPS_ODE_TSIT5) return ode_sse_tsit5<M, false>(z, P, bound);
-    else if constexpr (S == PPS_ODE_ATSIT5) return ode_sse_tsit5<M, true>(z, P, bound);
-    else if constexpr (S == PPS_ODE_TSIT5_SAVEAT) return ode_sse_tsit5_saveat<M, false>(z, P, bound);
-    else return ode_sse_tsit5_saveat<M, true>(z, P, bound);
+    else if constexpr (S == PPS_ODE_TSIT5) return ode_sse_tsit5<OdeLane<M>, false>(OdeLane<M>(z), P, bound);
+    else if constexpr (S == PPS_ODE_ATSIT5) return ode_sse_tsit5<OdeLane<M>, true>(OdeLane<M>(z), P, bound);
+    else if constexpr (S == PPS_ODE_TSIT5_SAVEAT)
+      return ode_sse_tsit5_saveat<OdeLane<M>, false>(OdeLane<M>(z), P, bound);
+    else return ode_sse_tsit5_saveat<OdeLane<M>, true>(OdeLane<M>(z), P, bound);
   } else {
     return __uint_as_float(0x7FC00000u);
   }
