@@ -315,6 +315,9 @@ struct OdeLane {
   __device__ __forceinline__ float sumsq(const float (&e)[NC]) const {
     return e[0] * e[0] + e[1] * e[1];
   }
+  // u(t + tt h) from Tsit5's continuous extension (tsit5_interp)
+  __device__ __forceinline__ void interp(const float (&u)[NC], float h, float tt,
+                                         const float (&k)[7][NC], float (&v)[NC]) const;
 };
 struct LvPair {
   static constexpr int NC = 1;
@@ -333,6 +336,9 @@ struct LvPair {
     const float q = e[0] * e[0];
     return pair_even(q) + pair_odd(q);
   }
+  // As OdeLane::interp, with the weights b2..b7 split between the lanes
+  __device__ __forceinline__ void interp(const float (&u)[NC], float h, float tt,
+                                         const float (&k)[7][NC], float (&v)[NC]) const;
 };
 
 // One Tsit5 step from u with k1 = f(u) in k[0]: stage sums left to right,
@@ -421,6 +427,30 @@ __device__ __forceinline__ void tsit5_interp(const float (&u)[NC], float h, floa
   for (int c = 0; c < NC; ++c)
     v[c] = u[c] + h * ((((((b1 * k[0][c] + b2 * k[1][c]) + b3 * k[2][c]) + b4 * k[3][c]) +
                          b5 * k[4][c]) + b6 * k[5][c]) + b7 * k[6][c]);
+}
+
+template <int MODEL>
+__device__ __forceinline__ void OdeLane<MODEL>::interp(const float (&u)[NC], float h, float tt,
+                                                       const float (&k)[7][NC],
+                                                       float (&v)[NC]) const {
+  tsit5_interp<NC>(u, h, tt, k, v);
+}
+
+// The even lane evaluates b2, b3, b4 and the odd lane b5, b6, b7 (the same
+// polynomials, in the same form), and each product b_i k_i reads b_i from the
+// lane that has it by DPP; the sum is tsit5_interp's, bit for bit.
+__device__ __forceinline__ void LvPair::interp(const float (&u)[NC], float h, float tt,
+                                               const float (&k)[7][NC], float (&v)[NC]) const {
+  const float r02 = odd ? TS_R52 : TS_R22, r03 = odd ? TS_R53 : TS_R23, r04 = odd ? TS_R54 : TS_R24;
+  const float r12 = odd ? TS_R62 : TS_R32, r13 = odd ? TS_R63 : TS_R33, r14 = odd ? TS_R64 : TS_R34;
+  const float r22 = odd ? TS_R72 : TS_R42, r23 = odd ? TS_R73 : TS_R43, r24 = odd ? TS_R74 : TS_R44;
+  const float b1 = tt * (TS_R11 + tt * (TS_R12 + tt * (TS_R13 + tt * TS_R14)));
+  const float s0 = tt * (tt * (r02 + tt * (r03 + tt * r04)));  // b2 | b5
+  const float s1 = tt * (tt * (r12 + tt * (r13 + tt * r14)));  // b3 | b6
+  const float s2 = tt * (tt * (r22 + tt * (r23 + tt * r24)));  // b4 | b7
+  v[0] = u[0] + h * ((((((b1 * k[0][0] + pair_even(s0) * k[1][0]) + pair_even(s1) * k[2][0]) +
+                        pair_even(s2) * k[3][0]) + pair_odd(s0) * k[4][0]) +
+                      pair_odd(s1) * k[5][0]) + pair_odd(s2) * k[6][0]);
 }
 
 // x^(1/5) for x > 0 from basic IEEE operations only (an exponent-halving seed
@@ -607,7 +637,7 @@ __device__ __forceinline__ float ode_sse_tsit5_saveat(const SYS& sys, const Prob
       const float tk = (float)o * tau;
       if (tk > t_new) break;
       float v[NC], dv[NC];
-      tsit5_interp<NC>(u, h, (tk - t) / h, k, v);
+      sys.interp(u, h, (tk - t) / h, k, v);
 #pragma unroll
       for (int c = 0; c < NC; ++c) dv[c] = obs[2 * o + c] - v[c];
       loss = loss + sys.sumsq(dv);
