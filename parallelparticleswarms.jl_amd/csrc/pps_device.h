@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "pps_shared.h"
 
 
@@ -180,82 +182,23 @@ __device__ __forceinline__ void ode_rhs(float u1, float u2, float p1, float p2, 
   }
 }
 
-// Sum of squared errors of a fixed-step RK4 trajectory (dt = par[2], sampled
-// every steps_per_obs steps, starting at u0 = par[0..1] at sample 0) against
-// the observations; the per-sample error is e1^2 + e2^2 and samples are added
-// in time order (reference src/ode_pso.jl:47-52).
-template <int MODEL>
-__device__ __forceinline__ float ode_sse_rk4(const float* th, const ProbArgs& P) {
-  const float p1 = th[0], p2 = th[1], p3 = th[2], p4 = th[3];
-  float u1 = P.par[0], u2 = P.par[1];
-  const float dt = P.par[2];
-  const float h2 = dt * 0.5f;
-  const float h6 = dt / 6.0f;
-  float loss = 0.0f;
-  const float* obs = P.obs;
-  for (int k = 0; k < P.n_obs; ++k) {
-    if (k > 0) {
-      for (int s = 0; s < P.steps_per_obs; ++s) {
-        float k1a, k1b, k2a, k2b, k3a, k3b, k4a, k4b;
-        ode_rhs<MODEL>(u1, u2, p1, p2, p3, p4, k1a, k1b);
-        ode_rhs<MODEL>(u1 + h2 * k1a, u2 + h2 * k1b, p1, p2, p3, p4, k2a, k2b);
-        ode_rhs<MODEL>(u1 + h2 * k2a, u2 + h2 * k2b, p1, p2, p3, p4, k3a, k3b);
-        ode_rhs<MODEL>(u1 + dt * k3a, u2 + dt * k3b, p1, p2, p3, p4, k4a, k4b);
-        u1 = u1 + h6 * (((k1a + 2.0f * k2a) + 2.0f * k3a) + k4a);
-        u2 = u2 + h6 * (((k1b + 2.0f * k2b) + 2.0f * k3b) + k4b);
-      }
-    }
-    const float e1 = obs[2 * k] - u1;
-    const float e2 = obs[2 * k + 1] - u2;
-    loss = loss + (e1 * e1 + e2 * e2);
-  }
-  return loss;
-}
-
-// The same Lotka-Volterra RK4 loss computed by two lanes of one particle:
-// the even lane of a lane pair carries u1, the odd lane u2, and each stage
-// reads both components of the pair by DPP.  Both components' derivatives
-// then have one form, f = X * own + (Y * u1) * u2:
-//   even lane: X = p1,  Y = -p2:  p1 u1 + ((-p2) u1) u2 = p1 u1 - (p2 u1) u2
-//   odd lane:  X = -p3, Y = p4:   (-p3) u2 + (p4 u1) u2
-// (negating a factor negates the rounded product exactly, and a - b is
-// a + (-b)), so every stage value, step and the loss (e1^2 + e2^2 in that
-// order, in both lanes) equal ode_sse_rk4's bit for bit.  Both lanes of each
-// pair must be active and hold the same th.
+// Two lanes per particle (the ODE estimation's pair kernel): the even lane of
+// a lane pair carries u1, the odd lane u2, and each right-hand side reads both
+// components of the pair by DPP.  Both lanes of each pair must be active.
 __device__ __forceinline__ float pair_even(float x) {  // quad_perm [0,0,2,2]
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xA0, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float pair_odd(float x) {  // quad_perm [1,1,3,3]
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xF5, 0xF, 0xF, true));
 }
+// Lotka-Volterra on a pair: both derivatives have one form,
+// f = X * own + (Y * u1) * u2:
+//   even lane: X = p1,  Y = -p2:  p1 u1 + ((-p2) u1) u2 = p1 u1 - (p2 u1) u2
+//   odd lane:  X = -p3, Y = p4:   (-p3) u2 + (p4 u1) u2
+// (negating a factor negates the rounded product exactly, and a - b is
+// a + (-b)), so each lane's derivative equals ode_rhs's bit for bit.
 __device__ __forceinline__ float lv_pair_rhs(float X, float Y, float a) {
   return X * a + (Y * pair_even(a)) * pair_odd(a);
-}
-__device__ __forceinline__ float ode_sse_rk4_lv_pair(const float* th, const ProbArgs& P) {
-  const bool odd = (threadIdx.x & 1u) != 0;
-  const float X = odd ? -th[2] : th[0];
-  const float Y = odd ? th[3] : -th[1];
-  float u = odd ? P.par[1] : P.par[0];
-  const float dt = P.par[2];
-  const float h2 = dt * 0.5f;
-  const float h6 = dt / 6.0f;
-  float loss = 0.0f;
-  const float* obs = P.obs + (odd ? 1 : 0);
-  for (int k = 0; k < P.n_obs; ++k) {
-    if (k > 0) {
-      for (int s = 0; s < P.steps_per_obs; ++s) {
-        const float k1 = lv_pair_rhs(X, Y, u);
-        const float k2 = lv_pair_rhs(X, Y, u + h2 * k1);
-        const float k3 = lv_pair_rhs(X, Y, u + h2 * k2);
-        const float k4 = lv_pair_rhs(X, Y, u + dt * k3);
-        u = u + h6 * (((k1 + 2.0f * k2) + 2.0f * k3) + k4);
-      }
-    }
-    const float e = obs[2 * k] - u;
-    const float sq = e * e;
-    loss = loss + (pair_even(sq) + pair_odd(sq));
-  }
-  return loss;
 }
 
 // ---------------------------------------------------------------------------
@@ -294,10 +237,10 @@ __device__ __forceinline__ float ode_sse_rk4_lv_pair(const float* th, const Prob
 #define TS_E7 0.015151515151515152f
 
 // The components an ODE loss integrates in one lane.  OdeLane: both
-// components of a two-component model.  LvPair: Lotka-Volterra on two lanes
-// per particle (ode_sse_rk4_lv_pair), the lane's own component.  Component c
+// components of a two-component model.  LvPair / FhnPair: a model on two lanes
+// per particle (pair_even, pair_odd), the lane's own component.  Component c
 // of sample k is obs[2 k + first() + c]; sumsq() is e1^2 + e2^2 over the
-// particle's two components, in that order.
+// particle's two components, in that order (in both lanes of a pair).
 template <int MODEL>
 struct OdeLane {
   static constexpr int NC = 2;
@@ -340,6 +283,60 @@ struct LvPair {
   __device__ __forceinline__ void interp(const float (&u)[NC], float h, float tt,
                                          const float (&k)[7][NC], float (&v)[NC]) const;
 };
+// FitzHugh-Nagumo on a pair: the two derivatives differ in form, so both
+// lanes evaluate ode_rhs's two expressions on the pair's (u1, u2) and keep
+// their own; stage sums, steps, interpolation and loss split as for LvPair.
+struct FhnPair : LvPair {
+  float p1, p2, p3, p4;
+  __device__ __forceinline__ explicit FhnPair(const float* th)
+      : LvPair(th), p1(th[0]), p2(th[1]), p3(th[2]), p4(th[3]) {}
+  __device__ __forceinline__ void rhs(const float (&u)[NC], float (&f)[NC]) const {
+    float f1, f2;
+    ode_rhs<PPS_ODE_FITZHUGH_NAGUMO>(pair_even(u[0]), pair_odd(u[0]), p1, p2, p3, p4, f1, f2);
+    f[0] = odd ? f2 : f1;
+  }
+};
+
+// Sum of squared errors of a fixed-step RK4 trajectory (dt = par[2], sampled
+// every steps_per_obs steps, starting at u0 = par[0..1] at sample 0) against
+// the observations; the per-sample error is e1^2 + e2^2 and samples are added
+// in time order (reference src/ode_pso.jl:47-52).
+template <class SYS>
+__device__ __forceinline__ float ode_sse_rk4(const SYS& sys, const ProbArgs& P) {
+  constexpr int NC = SYS::NC;
+  float u[NC];
+  sys.start(P, u);
+  const float dt = P.par[2];
+  const float h2 = dt * 0.5f;
+  const float h6 = dt / 6.0f;
+  float loss = 0.0f;
+  const float* obs = P.obs + sys.first();
+  for (int o = 0; o < P.n_obs; ++o) {
+    if (o > 0) {
+      for (int s = 0; s < P.steps_per_obs; ++s) {
+        float k1[NC], k2[NC], k3[NC], k4[NC], y[NC];
+        sys.rhs(u, k1);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) y[c] = u[c] + h2 * k1[c];
+        sys.rhs(y, k2);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) y[c] = u[c] + h2 * k2[c];
+        sys.rhs(y, k3);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) y[c] = u[c] + dt * k3[c];
+        sys.rhs(y, k4);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          u[c] = u[c] + h6 * (((k1[c] + 2.0f * k2[c]) + 2.0f * k3[c]) + k4[c]);
+      }
+    }
+    float e[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) e[c] = obs[2 * o + c] - u[c];
+    loss = loss + sys.sumsq(e);
+  }
+  return loss;
+}
 
 // One Tsit5 step from u with k1 = f(u) in k[0]: stage sums left to right,
 // y_i = u + h * sum_j a_ij k_j, per component; fills k[1..6] with k2..k7
@@ -654,20 +651,22 @@ __device__ __forceinline__ float ode_sse_tsit5_saveat(const SYS& sys, const Prob
   return loss;
 }
 
-// Lotka-Volterra loss of objective OBJ on two lanes per particle (LvPair,
-// ode_sse_rk4_lv_pair): the value objective<4, OBJ> returns for z, bit for
-// bit.  Both lanes of each pair must be active and hold the same z.
+// ODE loss of objective OBJ on two lanes per particle (LvPair or FhnPair):
+// the value objective<4, OBJ> returns for z, bit for bit.  Both lanes of each
+// pair must be active and hold the same z.
 template <int OBJ>
-__device__ __forceinline__ float lv_pair_objective(const float (&z)[4], const ProbArgs& P,
-                                                   float bound) {
-  static_assert(is_ode_objective(OBJ) && ode_model(OBJ) == PPS_ODE_LOTKA_VOLTERRA, "LV only");
+__device__ __forceinline__ float pair_objective(const float (&z)[4], const ProbArgs& P,
+                                                float bound) {
+  static_assert(is_ode_objective(OBJ), "ODE objectives only");
+  using SYS = typename std::conditional<ode_model(OBJ) == PPS_ODE_LOTKA_VOLTERRA, LvPair,
+                                        FhnPair>::type;
+  const SYS sys(z);
   constexpr int S = ode_solver(OBJ);
-  if constexpr (S == PPS_ODE_RK4) return ode_sse_rk4_lv_pair(z, P);
-  else if constexpr (S == PPS_ODE_TSIT5) return ode_sse_tsit5<LvPair, false>(LvPair(z), P, bound);
-  else if constexpr (S == PPS_ODE_ATSIT5) return ode_sse_tsit5<LvPair, true>(LvPair(z), P, bound);
-  else if constexpr (S == PPS_ODE_TSIT5_SAVEAT)
-    return ode_sse_tsit5_saveat<LvPair, false>(LvPair(z), P, bound);
-  else return ode_sse_tsit5_saveat<LvPair, true>(LvPair(z), P, bound);
+  if constexpr (S == PPS_ODE_RK4) return ode_sse_rk4<SYS>(sys, P);
+  else if constexpr (S == PPS_ODE_TSIT5) return ode_sse_tsit5<SYS, false>(sys, P, bound);
+  else if constexpr (S == PPS_ODE_ATSIT5) return ode_sse_tsit5<SYS, true>(sys, P, bound);
+  else if constexpr (S == PPS_ODE_TSIT5_SAVEAT) return ode_sse_tsit5_saveat<SYS, false>(sys, P, bound);
+  else return ode_sse_tsit5_saveat<SYS, true>(sys, P, bound);
 }
 
 // Objective f(x - shift) over a register array (compile-time capacity MAXD,
@@ -738,7 +737,7 @@ __device__ __forceinline__ float objective(const float (&xin)[MAXD], const int d
     return a * a + P.par[1] * (b * b);
   } else if constexpr (is_ode_objective(OBJ)) {
     constexpr int M = ode_model(OBJ), S = ode_solver(OBJ);
-    if constexpr (S == PPS_ODE_RK4) return ode_sse_rk4<M>(z, P);
+    if constexpr (S == PPS_ODE_RK4) return ode_sse_rk4<OdeLane<M>>(OdeLane<M>(z), P);
     else if constexpr (S == PPS_ODE_TSIT5) return ode_sse_tsit5<OdeLane<M>, false>(OdeLane<M>(z), P, bound);
     else if constexpr (S == PPS_ODE_ATSIT5) return ode_sse_tsit5<OdeLane<M>, true>(OdeLane<M>(z), P, bound);
     else if constexpr (S == PPS_ODE_TSIT5_SAVEAT)

@@ -358,16 +358,16 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
   }
 }
 
-// Sync step of the Lotka-Volterra estimation (any of its five solvers) with
-// two lanes per particle: a 64-particle tile (the tiling of k_init and
+// Sync step of the ODE estimation (either model, any of the five solvers)
+// with two lanes per particle: a 64-particle tile (the tiling of k_init and
 // k_sync_step, so block keys and candidates mean the same) runs on 128
 // threads, lanes 2m and 2m+1 holding particle m.  Both lanes load the
 // particle and compute the same update; the loss, a chain of hundreds to
 // thousands of dependent solver steps per evaluation, is split between them
-// (lv_pair_objective), which halves the chain each lane issues and doubles
+// (pair_objective), which halves the chain each lane issues and doubles
 // the waves over the SIMDs.  The even lane writes the state.
 template <int OBJ, bool OFF32>
-__global__ __launch_bounds__(128) void k_sync_step_lv_pair(SwarmDev S, ProbArgs P, StepArgs A,
+__global__ __launch_bounds__(128) void k_sync_step_pair(SwarmDev S, ProbArgs P, StepArgs A,
                                                            float w, uint32_t k) {
   constexpr int MAXD = 4, TP = 64, BLOCK = 2 * TP;
   __shared__ uint64_t s_red[BLOCK / 64];
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(128) void k_sync_step_lv_pair(SwarmDev S, ProbArgs 
 #pragma unroll
       for (int j = 0; j < MAXD; ++j) z[j] = x[j] - P.shift[j];  // as objective()
       // as cost_at(): the particle's best cost bounds an adaptive loss
-      const float c = lv_pair_objective<OBJ>(z, P, P.ode_stop ? pc : NO_BOUND);
+      const float c = pair_objective<OBJ>(z, P, P.ode_stop ? pc : NO_BOUND);
       const bool imp = c < pc;
       if (even) {
 #pragma unroll
@@ -698,13 +698,13 @@ struct Launch {
     const uint64_t d = (uint64_t)(EXACT ? MAXD : P.d);
     const bool off32 = (uint64_t)S.npad * d * 4u < (1ull << 32);
     const bool nt = (uint64_t)S.npad * (3u * d + 1u) * 4u > PPS_NT_MIN_STATE_BYTES;
-    if constexpr (is_ode_objective(OBJ) && ode_model(OBJ) == PPS_ODE_LOTKA_VOLTERRA) {
+    if constexpr (is_ode_objective(OBJ)) {
       if (!(S.block == WGB && S.G == 1) && S.block == 64) {
         if (off32)
-          hipLaunchKernelGGL((k_sync_step_lv_pair<OBJ, true>), dim3(S.G), dim3(128), 0, st, S, P,
+          hipLaunchKernelGGL((k_sync_step_pair<OBJ, true>), dim3(S.G), dim3(128), 0, st, S, P,
                              A, w, k);
         else
-          hipLaunchKernelGGL((k_sync_step_lv_pair<OBJ, false>), dim3(S.G), dim3(128), 0, st, S,
+          hipLaunchKernelGGL((k_sync_step_pair<OBJ, false>), dim3(S.G), dim3(128), 0, st, S,
                              P, A, w, k);
         return hipGetLastError();
       }
