@@ -317,7 +317,10 @@ typedef struct pps_server_config {
   int64_t hbm_budget;      /* bytes of swarm state admitted at once; 0 = 80%
                               of free HBM at create time, at most 16 GiB    */
   int32_t max_active;      /* problems resident at once (0 = 4096)         */
-  int32_t slice_us;        /* device time per launch slice (0 = 1000 us)   */
+  int32_t slice_us;        /* device time per launch slice (0 = 1000 us);
+                              while a problem with n * d >= 8192 is
+                              resident (one iteration per slice) at most
+                              20 us                                         */
   int32_t reserved;
 } pps_server_config;
 
