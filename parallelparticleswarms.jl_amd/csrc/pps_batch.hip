@@ -8,7 +8,7 @@
 //          position is handed over through LDS.
 //   PART   part q of k of a large problem: one iteration t over the particle
 //          tiles q, q+k, q+2k, ... (256 particles per tile, one per thread,
-//          eight coordinates per round of loads).  The global best
+//          sixteen coordinates per round of loads).  The global best
 //          of iteration t-1 is the minimum of the k part keys the previous
 //          slice wrote to parity (t-1)&1 of the problem's exchange area; the
 //          part writes its own key and candidate position to parity t&1, so
@@ -33,7 +33,7 @@ constexpr uint64_t KEY_NONE = ~0ull;
 constexpr int PPT = 4;  // particles in flight per thread (SMALL items)
 constexpr int TILE = BATCH_BLOCK * PPT;
 constexpr int PTILE = BATCH_BLOCK;  // particles per tile of a PART item
-constexpr int PCOORD = 8;           // coordinates per round of loads (PART)
+constexpr int PCOORD = 16;          // coordinates per round of loads (PART)
 
 template <int BLOCK>
 __device__ __forceinline__ uint64_t block_min(uint64_t k, uint64_t* s_red) {
@@ -333,7 +333,7 @@ __device__ void run_small(BatchProb* B, uint64_t deadline, uint64_t* s_red, Prob
   uint64_t prev = wall_clock64();
   while (done < steps) {
     const uint32_t t = t0 + (uint32_t)(++done);
-    // d > 4: eight coordinates of one particle per round of loads; else
+    // d > 4: sixteen coordinates of one particle per round of loads; else
     // four particles of one coordinate pair
     const uint64_t mykey = S.d > 4 ? part_step_sweep<OBJ>(S, L, 0, 1, t, w)
                                    : step_sweep<OBJ>(S, L, 0, 1, t, w);
