@@ -28,7 +28,7 @@ def objective_marker(obj, obs=None, u0=(1.0, 1.0), dt=0.01, spo=10):
 
 def make_pair(obj, d, n, seed, mode=L.MODE_SYNC, *, lb=-5.12, ub=5.12, gbest_from=0,
               launch=L.LAUNCH_AUTO, shift=None, clamp=0, obs=None, u0=(1.0, 1.0), dt=0.01,
-              spo=10, min_improvers=2, nthreads=0, cons=None):
+              spo=10, min_improvers=2, nthreads=0, cons=None, grid_per_cu=0):
     lbv = np.broadcast_to(np.asarray(lb, np.float32), (d,)).copy()
     ubv = np.broadcast_to(np.asarray(ub, np.float32), (d,)).copy()
     params = PARAMS.get(obj, ())
@@ -39,7 +39,7 @@ def make_pair(obj, d, n, seed, mode=L.MODE_SYNC, *, lb=-5.12, ub=5.12, gbest_fro
                                  lb=lbv, ub=ubv, shift=shift, cons=cons)
     ctx = P.context()
     sp = P.swarm_params(n, mode, seed=seed, gbest_from=gbest_from, clamp=clamp, launch=launch,
-                        qlock_min_improvers=min_improvers)
+                        qlock_min_improvers=min_improvers, grid_per_cu=grid_per_cu)
     g = P.Swarm(ctx, prob, sp)
     oprob = O.Problem(obj, d, lbv, ubv, np.asarray(params, np.float32), shift, clamp,
                       None if obs is None else np.asarray(obs, np.float32),

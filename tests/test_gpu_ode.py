@@ -33,6 +33,21 @@ def test_lv_full_shape_parity():
     assert_same_gbest(g, o)
 
 
+@pytest.mark.parametrize("gbest_from", [L.GBEST_PBEST, L.GBEST_CURRENT])
+def test_lv_pair_lanes_ragged_multi_tile(gbest_from):
+    # The two-lanes-per-particle kernel with a partial last tile (20001 is not
+    # a multiple of 64) and several tiles per workgroup (one workgroup per
+    # CU), under both global-best sources
+    note(f"LV RK4 20001 particles, grid_per_cu=1, gbest_from={gbest_from}")
+    g, o = make_pair(O.LOTKA_VOLTERRA, 4, 20001, seed=91, lb=0.0, ub=5.0, obs=OBS,
+                     clamp=L.CLAMP_LEXICOGRAPHIC, launch=L.LAUNCH_GRAPH, grid_per_cu=1,
+                     gbest_from=gbest_from)
+    g.run(2)
+    o.run_sync(2, gbest_from=gbest_from)
+    assert_same_state(g, o, "after 2 iterations")
+    assert_same_gbest(g, o)
+
+
 def test_lv_estimation_recovers_parameters():
     note("LV estimation 300 iterations")
     sol, sw = P.parameter_estim_ode(OBS, np.zeros(4), np.full(4, 5.0), 1 << 14, 300, seed=5)
