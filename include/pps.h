@@ -269,7 +269,8 @@ int pps_swarm_get_state(pps_swarm* s, float* x, float* v, float* best,
 int pps_swarm_set_state(pps_swarm* s, const float* x, const float* v,
                         const float* best, const float* best_cost,
                         uint32_t iter);
-/* Device time of the last run (hipEvents on the swarm's stream). */
+/* Device time of the last run (hipEvents on the swarm's stream around its
+ * launches; the capture of a new iteration graph happens before the first). */
 int pps_swarm_elapsed_ms(pps_swarm* s, float* ms);
 int pps_swarm_free(pps_swarm* s);
 

