@@ -439,7 +439,13 @@ int pps_server_create(pps_ctx* ctx, const pps_server_config* cfg, pps_server** o
   for (int b = 0; b < 2; ++b) {
     SVCHK(hipHostMalloc(&sv->h_items[b], sizeof(BatchItem) * sv->max_items,
                         hipHostMallocCoherent | hipHostMallocMapped));
-    SVCHK(hipEventCreateWithFlags(&sv->slice_done[b], hipEventDisableTiming));
+    // Without the event's system-scope fence (a cache writeback and
+    // invalidation after every slice): the host waits on these events only
+    // for completion order, to rewrite a slice's work list, and every result
+    // and done flag it reads is published by the kernel itself with a
+    // system-scope release.
+    SVCHK(hipEventCreateWithFlags(&sv->slice_done[b],
+                                  hipEventDisableTiming | hipEventDisableSystemFence));
   }
   SVCHK(hipMalloc(&sv->d_xchg, sizeof(BatchXchg) * max_active));
   memset(sv->h_out, 0, sizeof(BatchOut) * max_active);
