@@ -302,12 +302,18 @@ __device__ __forceinline__ void copy_winner(const Swarm& S, uint64_t mykey, uint
 }
 
 // SMALL item: run iterations until the problem ends or the deadline passes.
+// Returns true (in every thread) when the problem ended in this item: it ran
+// the last iteration, or it has none and was initialised here.  Only that
+// item publishes the result, so no kernel writes a result slot after the host
+// may have seen its done flag (a finished problem can still be listed in a
+// slice the host built while the finishing slice ran).
 template <int OBJ>
-__device__ void run_small(BatchProb* B, uint64_t deadline, uint64_t* s_red, ProbLds* L,
+__device__ bool run_small(BatchProb* B, uint64_t deadline, uint64_t* s_red, ProbLds* L,
                           int* s_stop) {
   const Swarm S = load_swarm(B, L);
+  const bool was_initialized = B->initialized != 0;
   __syncthreads();
-  if (!B->initialized) {
+  if (!was_initialized) {
     const uint64_t mykey = init_sweep<OBJ, PPT>(S, L, 0, 1);
     const uint64_t bk = block_min<BATCH_BLOCK>(mykey, s_red);
     copy_winner(S, mykey, bk, L->gb);
@@ -321,7 +327,7 @@ __device__ void run_small(BatchProb* B, uint64_t deadline, uint64_t* s_red, Prob
     __syncthreads();
   }
   const int32_t steps = B->remaining;
-  if (steps <= 0) return;
+  if (steps <= 0) return !was_initialized;
   if ((int)threadIdx.x < S.d) L->gb[threadIdx.x] = B->gb[threadIdx.x];
   const uint32_t t0 = B->iter;
   float w = B->w;
@@ -358,6 +364,7 @@ __device__ void run_small(BatchProb* B, uint64_t deadline, uint64_t* s_red, Prob
     B->remaining -= done;
     B->w = w;
   }
+  return done >= steps;
 }
 
 // Global best of a large problem after iteration `t`: minimum of its part keys
@@ -416,12 +423,11 @@ __global__ __launch_bounds__(BATCH_BLOCK) void k_slice(BatchProb* probs, const B
         dst[k] = src[k];
       __syncthreads();
     }
-    if (B->P.obj == PPS_OBJ_RASTRIGIN)
-      run_small<PPS_OBJ_RASTRIGIN>(B, deadline, s_red, &s_lds, &s_stop);
-    else
-      run_small<PPS_OBJ_ROSENBROCK>(B, deadline, s_red, &s_lds, &s_stop);
+    const bool finished = B->P.obj == PPS_OBJ_RASTRIGIN
+                              ? run_small<PPS_OBJ_RASTRIGIN>(B, deadline, s_red, &s_lds, &s_stop)
+                              : run_small<PPS_OBJ_ROSENBROCK>(B, deadline, s_red, &s_lds, &s_stop);
     __syncthreads();
-    if (threadIdx.x == 0 && B->remaining <= 0) {
+    if (threadIdx.x == 0 && finished) {
       BatchOut* o = out + it.slot;
       o->gb_cost = B->gb_cost;
       o->gb_idx = B->gb_idx;

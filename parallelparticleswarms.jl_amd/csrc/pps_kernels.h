@@ -21,7 +21,8 @@ struct KernelSet {
   // Recompute block keys/candidates from the current state (no step).
   hipError_t (*rekey)(const SwarmDev&, const ProbArgs&, uint32_t par, int32_t gbest_from,
                       hipStream_t);
-  // One sync iteration t = *tbase + k (reads parity (t-1)&1, writes t&1).
+  // One sync iteration t = tbase[0] + k of epoch tbase[1] (reads parity
+  // (t-1)&1, writes t&1).
   hipError_t (*sync_step)(const SwarmDev&, const ProbArgs&, const StepArgs&, float w,
                           uint32_t k, hipStream_t);
   // Sync iterations t0+1 .. t0+T in one workgroup with the swarm in registers
@@ -31,7 +32,7 @@ struct KernelSet {
   // Async: every particle runs T iterations against its running best.
   hipError_t (*async_run)(const SwarmDev&, const ProbArgs&, const StepArgs&, float w0,
                           float wdamp, uint32_t t0, int32_t T, hipStream_t);
-  // Queue-lock: one iteration t = *tbase + k against the live global best
+  // Queue-lock: one iteration t = tbase[0] + k against the live global best
   // (workgroups of 1024 particles).
   hipError_t (*qlock_step)(const SwarmDev&, const ProbArgs&, const StepArgs&, float w,
                            uint32_t k, hipStream_t);
@@ -43,8 +44,9 @@ const KernelSet* find_kernels(int obj, int d, bool constrained = false);
 
 // Reduce the block keys of parity `par` into swarm.gbrec (cost, idx, pos).
 hipError_t launch_finalize(const SwarmDev&, int d, uint32_t par, hipStream_t);
-// *p = v on the stream (a one-lane kernel, so the write can sit in a graph).
-hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t);
+// p[0] = v0, p[1] = v1 on the stream (a one-lane kernel, so the write can sit
+// in a graph).
+hipError_t launch_set_u32x2(uint32_t* p, uint32_t v0, uint32_t v1, hipStream_t);
 // Write gbrec and a single-candidate parity `par` from host values.
 hipError_t launch_set_gbest(const SwarmDev&, int d, uint32_t par, const float* pos_dev,
                             float cost, hipStream_t);

@@ -57,12 +57,15 @@ hipError_t launch_finalize(const SwarmDev& S, int d, uint32_t par, hipStream_t s
   return hipGetLastError();
 }
 
-__global__ void k_set_u32(uint32_t* p, uint32_t v) {
-  if (threadIdx.x == 0) *p = v;
+__global__ void k_set_u32x2(uint32_t* p, uint32_t v0, uint32_t v1) {
+  if (threadIdx.x == 0) {
+    p[0] = v0;
+    p[1] = v1;
+  }
 }
 
-hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t st) {
-  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, p, v);
+hipError_t launch_set_u32x2(uint32_t* p, uint32_t v0, uint32_t v1, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_u32x2, dim3(1), dim3(64), 0, st, p, v0, v1);
   return hipGetLastError();
 }
 

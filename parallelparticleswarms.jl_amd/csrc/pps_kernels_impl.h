@@ -246,7 +246,8 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
   constexpr bool HOLD = MAXD * BLOCK <= 32 * 256;  // block winner's position in registers
   __shared__ uint64_t s_red[BLOCK / 64];
   __shared__ float s_gb[MAXD];
-  const uint32_t t = *S.tbase + k;
+  const uint32_t t = S.tbase[0] + k;
+  const uint32_t epoch = S.tbase[1];
   const int d = EXACT ? MAXD : P.d;
   const int G = S.G;
   const int tid = threadIdx.x;
@@ -277,7 +278,7 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
   if (i < S.n) load_tile();
   // Keep the loads ahead of everything below.
   __builtin_amdgcn_sched_barrier(0);
-  if (PRE && i < S.n) step_draws<MAXD>(r1, r2, (uint32_t)i, t, S.epoch, S.key0, S.key1, d);
+  if (PRE && i < S.n) step_draws<MAXD>(r1, r2, (uint32_t)i, t, epoch, S.key0, S.key1, d);
   for (int j = tid + BLOCK; j < G; j += BLOCK) gk = umin64(gk, keys[j]);
   gk = block_min_u64<BLOCK>(gk, s_red);
   {
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
       if constexpr (PRE) {
         velocity_position_drawn<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, r1, r2, d, P);
       } else {
-        velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch,
+        velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, epoch,
                                      S.key0, S.key1, d, P);
       }
       const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2), pc);
@@ -342,7 +343,7 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
       // scheduler interleaves them with the arithmetic pair by pair and the
       // wave waits for memory several times per tile.
       __builtin_amdgcn_sched_barrier(0);
-      if (PRE) step_draws<MAXD>(r1, r2, (uint32_t)i, t, S.epoch, S.key0, S.key1, d);
+      if (PRE) step_draws<MAXD>(r1, r2, (uint32_t)i, t, epoch, S.key0, S.key1, d);
     }
   }
   if constexpr (HOLD) {
@@ -372,7 +373,8 @@ __global__ __launch_bounds__(128) void k_sync_step_pair(SwarmDev S, ProbArgs P, 
   constexpr int MAXD = 4, TP = 64, BLOCK = 2 * TP;
   __shared__ uint64_t s_red[BLOCK / 64];
   __shared__ float s_gb[MAXD];
-  const uint32_t t = *S.tbase + k;
+  const uint32_t t = S.tbase[0] + k;
+  const uint32_t epoch = S.tbase[1];
   const int d = MAXD;
   const int G = S.G;
   const int tid = threadIdx.x;
@@ -407,7 +409,7 @@ __global__ __launch_bounds__(128) void k_sync_step_pair(SwarmDev S, ProbArgs P, 
         pb[j] = *R.at(S.pb, j);
       }
       float pc = S.pc[i];
-      velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch,
+      velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, epoch,
                                    S.key0, S.key1, d, P);
       float z[MAXD];
 #pragma unroll
@@ -603,7 +605,8 @@ __device__ __forceinline__ float ld_acq(const float* p) {
 template <int MAXD, bool EXACT, int OBJ, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArgs A, float w,
                                                  uint32_t k) {
-  const uint32_t t = *S.tbase + k;
+  const uint32_t t = S.tbase[0] + k;
+  const uint32_t epoch = S.tbase[1];
   __shared__ uint64_t s_red[BLOCK / 64];
   __shared__ int s_cnt[BLOCK / 64];
   __shared__ float s_gb[MAXD];
@@ -624,7 +627,7 @@ __global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArg
     load_vec<MAXD>(v, S.v, S.npad, i, d);
     load_vec<MAXD>(pb, S.pb, S.npad, i, d);
     pc = S.pc[i];
-    velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, S.epoch, S.key0,
+    velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, epoch, S.key0,
                             S.key1, d, P);
     const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2), pc);
     store_vec<MAXD>(x, S.x, S.npad, i, d);

@@ -497,7 +497,9 @@ int pps_swarm_reinit_positions(pps_swarm* s, const float* positions) {
 }
 
 // Capture (or reuse) the graph of T sync launches; launched by pps_swarm_run
-// after its start event, so the run's device time excludes the capture.
+// after its start event, so the run's device time excludes the capture.  The
+// kernels read their stream position (iteration base and epoch) from device
+// memory, so a graph stays valid across runs and reinitialisations.
 static int prepare_sync_graph(pps_swarm* s, int32_t T, float w, float wdamp,
                               const StepArgs& A) {
   hipStream_t st = s->ctx->stream;
@@ -550,7 +552,11 @@ int pps_swarm_run(pps_swarm* s, int32_t maxiters, float w, float wdamp) {
     int rc = prepare_sync_graph(s, T, w, wdamp, A);
     if (rc != PPS_OK) return rc;
   }
-  HIPCHK(launch_set_u32((uint32_t*)s->S.tbase, t0, st));
+  // Stream position of this run (read by the kernels, so a captured graph
+  // replays after a reinit too); a queue-lock run starts with a clear
+  // give-up flag.
+  HIPCHK(launch_set_u32x2((uint32_t*)s->S.tbase, t0, s->S.epoch, st));
+  if (s->prm.mode == PPS_MODE_QLOCK) HIPCHK(hipMemsetAsync(&s->S.gbrec->err, 0, sizeof(uint32_t), st));
   HIPCHK(hipEventRecord(s->ev0, st));
   if (T > 0) {
     switch (s->prm.mode) {

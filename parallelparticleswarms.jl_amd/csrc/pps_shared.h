@@ -122,12 +122,13 @@ struct SwarmDev {
   uint64_t* bkey;  // [2][G]
   float* cand;     // [2][G][d]
   GbRec* gbrec;
-  // Iteration counter base: launch k of a run executes random-stream
-  // iteration t = *tbase + k, so a captured iteration loop can be replayed
-  // from any starting iteration.
+  // Position in the random stream, in device memory: launch k of a run
+  // executes iteration t = tbase[0] + k of epoch tbase[1], so a captured
+  // iteration loop can be replayed from any starting iteration and after any
+  // number of reinitialisations.  The host writes both words before each run.
   const uint32_t* tbase;
   uint32_t key0, key1;  // Philox key (seed)
-  uint32_t epoch;
+  uint32_t epoch;       // epoch of the kernels launched outside a graph
   int32_t G;
   int32_t block;
   int32_t sampler;   // PPS_SAMPLER_* of the next initialisation

@@ -63,6 +63,27 @@ def test_sync_multiblock_trajectory(launch, obj, d, n):
         assert_same_gbest(g, o, f"after {done} iterations")
 
 
+@pytest.mark.parametrize("obj,d,n", [(O.RASTRIGIN, 10, 5000), (O.ROSENBROCK, 3, 70000)])
+def test_graph_replay_after_reinit(obj, d, n):
+    # init -> solve! -> reinit! -> solve! with the iteration loop in a captured
+    # graph (same T, so the graph is replayed, not recaptured): every run after
+    # a reinit must draw from the new epoch's random stream
+    note(f"graph replay after reinit obj={obj} d={d} n={n}")
+    g, o = make_pair(obj, d, n, seed=31 + n, launch=L.LAUNCH_GRAPH)
+    for rnd in range(3):
+        g.run(12)
+        o.run_sync(12)
+        assert_same_state(g, o, f"round {rnd}")
+        assert_same_gbest(g, o, f"round {rnd}")
+        g.reinit()
+        o.reinit()
+        assert_same_state(g, o, f"reinit {rnd}")
+    g.run(12)
+    o.run_sync(12)
+    assert_same_state(g, o, "after the last reinit")
+    assert_same_gbest(g, o, "after the last reinit")
+
+
 def test_sync_current_mode_multiblock():
     note("sync multiblock gbest_from=CURRENT")
     g, o = make_pair(O.ROSENBROCK, 4, 4000, seed=99, lb=-1.0, ub=10.0,
