@@ -11,7 +11,10 @@ the reference's benchmarks/CPU_vs_GPU/benchmark.jl:107-121).
 Also reported on the same line:
   roofline      HBM roofline of the dominant kernel (the fused sync step):
                 algorithmic bytes per launch / average launch time (hipEvents
-                on the swarm's stream over the timed region).
+                on the swarm's stream over the timed region).  At 2^20 x 10-D
+                the 130 MB state sits inside the 256 MB Infinity Cache.
+  roofline_hbm  the same kernel with the state past the cache (2^22
+                particles x 10-D, 520 MB), both objectives.
   async         ParallelPSOKernel(global_update=false) at the same shape.
   qlock         ParallelPSOKernel(global_update=true), the reference's default:
                 one launch per iteration against the live global best, at the
@@ -153,7 +156,12 @@ def c4_mix(P, L, ctx, count=4096, plain_sample=256, served_qps=None):
                "speedup_vs_plain_loop": (count / wall) / plain_rate,
                "bitwise_mismatches_vs_plain": mismatch,
                "max_abs_f64_diff_vs_plain": max_df64,
-               "max_abs_f32_vs_f64": max(abs(r.cost_f64 - r.cost_f32) for r in res)}
+               "max_abs_f32_vs_f64": max(abs(r.cost_f64 - r.cost_f32) for r in res),
+               "max_rel_f32_vs_f64": max(abs(r.cost_f64 - r.cost_f32) / max(1.0, abs(r.cost_f64))
+                                         for r in res),
+               "objective_tolerance": "|f32(u) - f64(u)| <= 1e-4 * max(1, |f64(u)|) for every "
+                                      "returned u; f64 of the packed u equals f64 of the plain "
+                                      "u (max_abs_f64_diff_vs_plain)"}
     # served: Poisson arrivals at half the packed throughput
     qps = served_qps or 0.5 * count / wall
     reqs2, arr = P.served_mix(count, seed=2024, qps=qps)
@@ -317,6 +325,27 @@ def n_sweep(P, L):
     return rows
 
 
+def beyond_cache(P, L, lg=22, iters=200):
+    """The sync step with the state past the 256 MB Infinity Cache (2^22
+    particles x 10-D: 520 MB of state), both objectives: average launch time
+    over `iters` graph-replayed iterations after 1000 untimed ones."""
+    n = 1 << lg
+    us = []
+    for obj, seed in (("rastrigin", 21), ("rosenbrock", 22)):
+        s = make_swarm(P, obj, n, D, seed, L.MODE_SYNC)
+        s.run(1000)
+        s.run(iters)
+        us.append(s.elapsed_ms() * 1e3 / iters)
+        s.free()
+    avg = sum(us) / len(us)
+    rate = n * BYTES_PER_PARTICLE_ITER / (avg * 1e-6)
+    return {"particles": n, "dim": D, "state_mb": round(n * 4 * (3 * D + 1) / 1e6, 1),
+            "iterations_timed": iters, "us_per_launch": [round(u, 2) for u in us],
+            "bytes_per_launch": n * BYTES_PER_PARTICLE_ITER, "achieved": rate / 1e9,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": rate / HBM_PEAK,
+            "particle_iters_per_s": n / (avg * 1e-6)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,7 +353,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--extras", default="async,qlock,c0,c4,hybrid,ode",
+    ap.add_argument("--extras", default="hbm,async,qlock,c0,c4,hybrid,ode",
                     help="comma list of the legs to run besides the headline")
     ap.add_argument("--sweep", action="store_true",
                     help="also time the sync step at 2^10..2^24 particles (adds ~1 min)")
@@ -383,6 +412,9 @@ def main():
     }
     for s in swarms:
         s.free()
+    if "hbm" in legs:
+        out["roofline_hbm"] = beyond_cache(P, L)
+        log(f"[bench] beyond-cache leg done: {out['roofline_hbm']['frac']:.3f} of HBM peak")
 
     if "async" in legs:
         # async (ParallelPSOKernel(global_update=false)) at the same shape
