@@ -293,6 +293,22 @@ int pps_swarm_lbfgs(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x,
                     float* out_f, float* best_x, float* best_f,
                     int64_t* best_idx);
 
+/* Allocation-free form of pps_swarm_lbfgs (same arguments, same results bit
+ * for bit).  It has no counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Its device scratch (the local
+ * search's SoA results, their argmin key and the AoS staging of out_x) is a
+ * file-scope device array of PPS_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends; the call needs about 4 * n * (2d + 1) bytes of it
+ * (4 * n * (d + 1) with out_x NULL) and returns PPS_E_INVALID beyond that.
+ * It calls no device or host allocator.  Every call on one GPU shares that
+ * GPU's array and leaves its contents there: calls that overlap on one GPU
+ * (several threads, several contexts) race. */
+#define PPS_NOALLOC_SCRATCH_BYTES (64 << 20)
+int pps_swarm_lbfgs_noalloc(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x,
+                            float* out_f, float* best_x, float* best_f,
+                            int64_t* best_idx);
+
 /* ---- autotuning ------------------------------------------------------- */
 /* Time the multi-workgroup sync step for grid_per_cu in {1, 2, 3, 4, 6, 8} on a
  * scratch swarm of `n` particles (`iters` timed iterations each), remember the

@@ -56,6 +56,9 @@ hipError_t launch_set_gbest(const SwarmDev&, int d, uint32_t par, const float* p
 bool lbfgs_supported(int obj, int d);
 hipError_t launch_lbfgs(const SwarmDev&, const ProbArgs&, const LbfgsArgs&, float* out_x,
                         float* out_f, hipStream_t);
+// dst[i * d + j] = src[j * npad + i] for i < n, j < d (SoA rows -> AoS).
+hipError_t launch_soa_to_aos(const float* src, int64_t npad, int64_t n, int d, float* dst,
+                             hipStream_t);
 // Lowest-index argmin key of vals[0..n) (NaN never wins) into *out_key.
 hipError_t launch_argmin_f32(const float* vals, int64_t n, uint64_t* out_key, hipStream_t);
 

@@ -88,6 +88,23 @@ hipError_t launch_argmin_f32(const float* vals, int64_t n, uint64_t* out_key, hi
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void k_soa_to_aos(const float* src, int64_t npad, int64_t n,
+                                                    int d, float* dst) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k < n * d) {
+    const int64_t i = k / d, j = k - i * d;
+    dst[k] = src[j * npad + i];
+  }
+}
+
+hipError_t launch_soa_to_aos(const float* src, int64_t npad, int64_t n, int d, float* dst,
+                             hipStream_t st) {
+  const int64_t blocks = (n * d + 255) / 256;
+  if (blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)blocks), dim3(256), 0, st, src, npad, n, d, dst);
+  return hipGetLastError();
+}
+
 const KernelSet* kernels_rosenbrock(int d, bool constrained);
 const KernelSet* kernels_rastrigin(int d, bool constrained);
 const KernelSet* kernels_sphere(int d, bool constrained);

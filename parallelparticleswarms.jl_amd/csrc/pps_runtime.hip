@@ -174,6 +174,35 @@ struct pps_swarm {
     if (e_ != hipSuccess) return hip_fail(e_, #expr, __FILE__, __LINE__); \
   } while (0)
 
+// Device scratch of the allocation-free local search (pps_swarm_lbfgs_noalloc).
+// Every GPU the library runs on holds its own copy until the process ends.
+// In host code `scratch` only names the symbol: kernels and copies use the
+// device address looked up once per device (scratch_setup), never the name.
+static __device__ __align__(256) unsigned char scratch[PPS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 256;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// With `device` current: resolve this device's copy of `scratch` (once).
+int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(PPS_E_INVALID, "device %d", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return PPS_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+  g_scratch_dev[device] = p;
+  return PPS_OK;
+}
+
+char* scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? (char*)g_scratch_dev[device] : nullptr;
+}
+}  // namespace
+
 extern "C" {
 
 const char* pps_last_error(void) { return g_err.c_str(); }
@@ -205,6 +234,8 @@ int pps_ctx_create(int device, pps_ctx** out) {
   HIPCHK(hipSetDevice(device));
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
+  const int src = scratch_setup(device);
+  if (src != PPS_OK) return src;
   pps_ctx* c = new pps_ctx();
   c->device = device;
   c->num_cus = prop.multiProcessorCount;
@@ -753,8 +784,9 @@ void pps_lbfgs_params_default(pps_lbfgs_params* p) {
   p->max_backtracks = 20;
 }
 
-int pps_swarm_lbfgs(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x, float* out_f,
-                    float* best_x, float* best_f, int64_t* best_idx) {
+}  // extern "C"
+
+static int lbfgs_check(const pps_swarm* s, const pps_lbfgs_params* lp) {
   if (!s || !lp) return fail(PPS_E_INVALID, "NULL argument");
   if (!lbfgs_supported(s->P.obj, s->P.d))
     return fail(PPS_E_INVALID, "no analytic gradient for objective %d at d=%d", s->P.obj, s->P.d);
@@ -765,6 +797,15 @@ int pps_swarm_lbfgs(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x, floa
   if (lp->history < 1 || lp->history > 16) return fail(PPS_E_INVALID, "history must be 1..16");
   if (lp->maxiters < 0 || lp->max_backtracks < 0) return fail(PPS_E_INVALID, "negative iteration count");
   if (!(lp->armijo_rho > 0.0f && lp->armijo_rho < 1.0f)) return fail(PPS_E_INVALID, "armijo_rho must be in (0,1)");
+  return PPS_OK;
+}
+
+extern "C" {
+
+int pps_swarm_lbfgs(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x, float* out_f,
+                    float* best_x, float* best_f, int64_t* best_idx) {
+  const int chk = lbfgs_check(s, lp);
+  if (chk != PPS_OK) return chk;
   HIPCHK(hipSetDevice(s->ctx->device));
   const int d = s->P.d;
   const int64_t n = s->prm.n, npad = s->S.npad;
@@ -802,6 +843,55 @@ int pps_swarm_lbfgs(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x, floa
     for (int64_t i = 0; i < n; ++i)
       for (int j = 0; j < d; ++j) out_x[i * d + j] = tmp[(size_t)j * n + i];
   }
+  return PPS_OK;
+}
+
+// The kernels of pps_swarm_lbfgs with their outputs in this device's
+// `scratch` instead of library-allocated buffers: SoA results (d x npad),
+// objective values (npad), the argmin key, and the AoS staging of out_x.
+// Nothing there needs clearing: every word read is written by this call.
+int pps_swarm_lbfgs_noalloc(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x,
+                            float* out_f, float* best_x, float* best_f, int64_t* best_idx) {
+  const int chk = lbfgs_check(s, lp);
+  if (chk != PPS_OK) return chk;
+  const int d = s->P.d;
+  const int64_t n = s->prm.n, npad = s->S.npad;
+  const size_t o_f = align256(sizeof(float) * d * npad);
+  const size_t o_key = o_f + align256(sizeof(float) * npad);
+  const size_t o_aos = o_key + 256;
+  const size_t need = o_aos + (out_x ? align256(sizeof(float) * d * n) : 0);
+  if (need > (size_t)PPS_NOALLOC_SCRATCH_BYTES)
+    return fail(PPS_E_INVALID,
+                "the allocation-free local search needs %zu bytes of scratch for n=%lld, d=%d; "
+                "it has %zu (about 4 n (2d + 1) bytes fit)",
+                need, (long long)n, d, (size_t)PPS_NOALLOC_SCRATCH_BYTES);
+  char* base = scratch_of(s->ctx->device);
+  if (!base) return fail(PPS_E_STATE, "no scratch for device %d", s->ctx->device);
+  float* lx = (float*)base;
+  float* lf = (float*)(base + o_f);
+  uint64_t* lkey = (uint64_t*)(base + o_key);
+  float* aos = (float*)(base + o_aos);
+  HIPCHK(hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  LbfgsArgs A{lp->maxiters, lp->history, lp->abstol, lp->armijo_c, lp->armijo_rho,
+              lp->max_backtracks};
+  HIPCHK(hipEventRecord(s->ev0, st));
+  HIPCHK(launch_lbfgs(s->S, s->P, A, lx, lf, st));
+  HIPCHK(launch_argmin_f32(lf, n, lkey, st));
+  HIPCHK(hipEventRecord(s->ev1, st));
+  if (out_x) HIPCHK(launch_soa_to_aos(lx, npad, n, d, aos, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipEventElapsedTime(&s->last_ms, s->ev0, s->ev1));
+  uint64_t key = 0;
+  HIPCHK(hipMemcpy(&key, lkey, sizeof key, hipMemcpyDeviceToHost));
+  const int64_t bi = (int64_t)(uint32_t)key;
+  if (best_idx) *best_idx = bi;
+  if (best_f) HIPCHK(hipMemcpy(best_f, lf + bi, sizeof(float), hipMemcpyDeviceToHost));
+  if (best_x)
+    HIPCHK(hipMemcpy2D(best_x, sizeof(float), lx + bi, sizeof(float) * npad, sizeof(float), d,
+                       hipMemcpyDeviceToHost));
+  if (out_f) HIPCHK(hipMemcpy(out_f, lf, sizeof(float) * n, hipMemcpyDeviceToHost));
+  if (out_x) HIPCHK(hipMemcpy(out_x, aos, sizeof(float) * d * n, hipMemcpyDeviceToHost));
   return PPS_OK;
 }
 

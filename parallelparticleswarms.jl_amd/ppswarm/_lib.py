@@ -129,6 +129,7 @@ def lib():
                                   C.POINTER(C.c_int32), fp])
     _sig(L, "pps_tuned_grid_per_cu", [C.c_int32, C.c_int32, C.c_int64])
     _sig(L, "pps_swarm_lbfgs", [vp, C.POINTER(pps_lbfgs_params), fp, fp, fp, fp, i64p])
+    _sig(L, "pps_swarm_lbfgs_noalloc", [vp, C.POINTER(pps_lbfgs_params), fp, fp, fp, fp, i64p])
     _sig(L, "pps_server_create", [vp, C.POINTER(pps_server_config), C.POINTER(vp)])
     _sig(L, "pps_server_submit", [vp, C.POINTER(pps_problem), C.POINTER(pps_swarm_params),
                                   C.c_int32, C.c_float, C.c_float, u64p])
