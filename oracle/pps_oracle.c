@@ -824,16 +824,20 @@ int ora_num_threads(void) {
 }
 
 /* ===================================================================== */
-/* HybridPSO local phase: projected L-BFGS with Armijo backtracking.
+/* HybridPSO local phase: projected L-BFGS with a Strong-Wolfe or an Armijo
+ * line search.
  *
  * The reference runs SimpleOptimization._lbfgs (projected L-BFGS with a
- * Strong-Wolfe line search) from every particle's best_position
- * (src/lbfgs.jl:9-21, src/hybrid.jl:100-158); that package is not vendored
- * under the reference, so this restates the published algorithm (Nocedal &
- * Wright, Numerical Optimization, Alg. 7.4 two-loop recursion and Alg. 7.5)
- * with the configuration the benchmark names: history m, analytic gradient,
- * Armijo backtracking on the projected path.  Non-finite final values become
- * +Inf (src/lbfgs.jl:20).
+ * Strong-Wolfe line search from LineSearch.jl, whose c1, c2, alpha_init,
+ * alpha_max, maxiters and zoom_maxiters it forwards, src/hybrid.jl:129-136)
+ * from every particle's best_position (src/lbfgs.jl:9-21,
+ * src/hybrid.jl:100-158); neither package is vendored under the reference,
+ * so this restates the published algorithms (Nocedal & Wright, Numerical
+ * Optimization: Alg. 7.4 two-loop recursion, Alg. 7.5 L-BFGS, Alg. 3.5 line
+ * search for the strong Wolfe conditions with its zoom, Alg. 3.6) on the
+ * projected path x(a) = P(x + a p), with an analytic gradient.  The Armijo
+ * backtracking search is the benchmark configuration's.  Non-finite final
+ * values become +Inf (src/lbfgs.jl:20).
  *
  * The kernel gives each particle a group of G = 2^ceil(log2 d) lanes, one
  * coordinate per lane; every dot product and objective sum is an xor-butterfly
@@ -842,10 +846,16 @@ int ora_num_threads(void) {
  * ===================================================================== */
 #define ORA_LB_MAXH 16
 
+#define ORA_LS_ARMIJO 0
+#define ORA_LS_STRONG_WOLFE 1
+
 typedef struct ora_lbfgs_params {
   int32_t maxiters, history;
   float abstol, armijo_c, armijo_rho;
   int32_t max_backtracks;
+  int32_t linesearch;    /* ORA_LS_* */
+  float wolfe_c2, alpha_init, alpha_max;
+  int32_t ls_maxiters, zoom_maxiters;
 } ora_lbfgs_params;
 
 static int group_size(int d) {
@@ -966,6 +976,145 @@ static void lane_fg(const ora_problem* P, const float* xin, int G, float* f, flo
 
 static inline float proj(float x, float lo, float hi) { return jl_min(jl_max(x, lo), hi); }
 
+/* Line search on phi(a) = f(P(x + a p)).  phi'(0) counts the lanes the
+ * projection lets move along p (not on a bound that p points out of);
+ * phi'(a), a > 0, the lanes whose unprojected x + a p lies strictly inside
+ * the box (the one-sided derivative of the projected path). */
+static float dphi_at0(const float* x, const float* g, const float* p, const float* lo,
+                      const float* hi, int G) {
+  float t[ORA_MAXD];
+  for (int l = 0; l < G; ++l)
+    t[l] = ((x[l] <= lo[l] && p[l] < 0.0f) || (x[l] >= hi[l] && p[l] > 0.0f)) ? 0.0f
+                                                                              : g[l] * p[l];
+  return tsum(t, G);
+}
+static float dphi_at(const float* x, const float* gt, const float* p, const float* lo,
+                     const float* hi, float a, int G) {
+  float t[ORA_MAXD];
+  for (int l = 0; l < G; ++l) {
+    float u = x[l] + a * p[l];
+    t[l] = (lo[l] < u && u < hi[l]) ? gt[l] * p[l] : 0.0f;
+  }
+  return tsum(t, G);
+}
+static void ls_eval(const ora_problem* P, const float* x, const float* p, const float* lo,
+                    const float* hi, float a, int G, float* xt, float* ft, float* gt) {
+  for (int l = 0; l < G; ++l) xt[l] = proj(x[l] + a * p[l], lo[l], hi[l]);
+  lane_fg(P, xt, G, ft, gt);
+}
+
+/* Strong-Wolfe search (Nocedal & Wright Alg. 3.5/3.6): accept a with
+ *   phi(a) <= phi(0) + c1 a phi'(0)   and   |phi'(a)| <= c2 |phi'(0)|.
+ * Trial steps alpha_init, doubled up to alpha_max (at most ls_maxiters of
+ * them) until one of them brackets an acceptable step; zoom's trial is the
+ * minimiser of the quadratic through phi(lo), phi'(lo) and phi(hi) when it
+ * lies in the middle 80 % of the bracket, else its midpoint.  When no step
+ * meets both conditions within the iteration limits, the last step of
+ * sufficient decrease found is taken (none: the search fails, returns 0).
+ * Returns 1 with the accepted point in xn, fn, gn. */
+int ora_wolfe_search(const ora_problem* P, const ora_lbfgs_params* lp, int G, const float* x,
+                     float f, const float* g, const float* p, const float* lo, const float* hi,
+                     float* xn, float* fn, float* gn, float* alpha) {
+  const float d0 = dphi_at0(x, g, p, lo, hi, G);
+  if (!(d0 < 0.0f)) return 0;
+  const float c1 = lp->armijo_c, curv = -(lp->wolfe_c2 * d0);
+  float xt[ORA_MAXD], gt[ORA_MAXD], ft;
+  float xl[ORA_MAXD], gl[ORA_MAXD];   /* point at a_lo (a_prev before a bracket) */
+  memcpy(xl, x, sizeof(float) * G);
+  memcpy(gl, g, sizeof(float) * G);
+  float a_lo = 0.0f, f_lo = f, d_lo = d0, a_hi = 0.0f, f_hi = 0.0f;
+  float a = lp->alpha_init;
+  int bracket = 0;
+  for (int i = 0; i < lp->ls_maxiters && !bracket; ++i) {
+    ls_eval(P, x, p, lo, hi, a, G, xt, &ft, gt);
+    if (ft > f + (c1 * a) * d0 || (i > 0 && ft >= f_lo)) {
+      a_hi = a;
+      f_hi = ft;
+      bracket = 1;
+      break;
+    }
+    const float da = dphi_at(x, gt, p, lo, hi, a, G);
+    if (fabsf(da) <= curv) goto accept_t;
+    if (da >= 0.0f) {
+      a_hi = a_lo;
+      f_hi = f_lo;
+      a_lo = a;
+      f_lo = ft;
+      d_lo = da;
+      memcpy(xl, xt, sizeof(float) * G);
+      memcpy(gl, gt, sizeof(float) * G);
+      bracket = 1;
+      break;
+    }
+    a_lo = a;
+    f_lo = ft;
+    d_lo = da;
+    memcpy(xl, xt, sizeof(float) * G);
+    memcpy(gl, gt, sizeof(float) * G);
+    if (!(a < lp->alpha_max)) break;
+    a = a + a;
+    if (a > lp->alpha_max) a = lp->alpha_max;
+  }
+  if (bracket) {
+    for (int j = 0; j < lp->zoom_maxiters; ++j) {
+      const float delta = a_hi - a_lo;
+      const float den = 2.0f * ((f_hi - f_lo) - d_lo * delta);
+      a = a_lo - ((d_lo * delta) * delta) / den;
+      const float b0 = fminf(a_lo, a_hi), b1 = fmaxf(a_lo, a_hi), w = b1 - b0;
+      if (!(a >= b0 + 0.1f * w && a <= b1 - 0.1f * w)) a = a_lo + 0.5f * delta;
+      ls_eval(P, x, p, lo, hi, a, G, xt, &ft, gt);
+      if (ft > f + (c1 * a) * d0 || ft >= f_lo) {
+        a_hi = a;
+        f_hi = ft;
+      } else {
+        const float da = dphi_at(x, gt, p, lo, hi, a, G);
+        if (fabsf(da) <= curv) goto accept_t;
+        if (da * (a_hi - a_lo) >= 0.0f) {
+          a_hi = a_lo;
+          f_hi = f_lo;
+        }
+        a_lo = a;
+        f_lo = ft;
+        d_lo = da;
+        memcpy(xl, xt, sizeof(float) * G);
+        memcpy(gl, gt, sizeof(float) * G);
+      }
+    }
+  }
+  if (!(a_lo > 0.0f)) return 0;
+  memcpy(xn, xl, sizeof(float) * G);
+  memcpy(gn, gl, sizeof(float) * G);
+  *fn = f_lo;
+  if (alpha) *alpha = a_lo;
+  return 1;
+accept_t:
+  memcpy(xn, xt, sizeof(float) * G);
+  memcpy(gn, gt, sizeof(float) * G);
+  *fn = ft;
+  if (alpha) *alpha = a;
+  return 1;
+}
+
+/* Armijo backtracking: the first a = rho^k (k <= max_backtracks) with
+ * f(P(x + a p)) <= f + c g'(P(x + a p) - x). */
+static int armijo_search(const ora_problem* P, const ora_lbfgs_params* lp, int G, const float* x,
+                         float f, const float* g, const float* p, const float* lo,
+                         const float* hi, float* xn, float* fn, float* gn) {
+  float alpha = 1.0f;
+  for (int bt = 0; bt <= lp->max_backtracks; ++bt) {
+    float dx[ORA_MAXD];
+    for (int l = 0; l < G; ++l) {
+      xn[l] = proj(x[l] + alpha * p[l], lo[l], hi[l]);
+      dx[l] = xn[l] - x[l];
+    }
+    lane_fg(P, xn, G, fn, gn);
+    float decr = tdot(g, dx, G);
+    if (*fn <= f + lp->armijo_c * decr) return 1;
+    alpha = alpha * lp->armijo_rho;
+  }
+  return 0;
+}
+
 /* One particle: x (d floats, in/out), returns the final value. */
 float ora_lbfgs_one(const ora_problem* P, const ora_lbfgs_params* lp, float* xio) {
   int d = P->d, G = group_size(d), m = lp->history;
@@ -1002,8 +1151,8 @@ float ora_lbfgs_one(const ora_problem* P, const ora_lbfgs_params* lp, float* xio
       for (int l = 0; l < G; ++l) p[l] = -g[l];
       k = 0;
     }
-    /* Armijo backtracking; if the quasi-Newton direction fails, forget the
-     * history and search once more along -g */
+    /* line search; if the quasi-Newton direction fails, forget the history
+     * and search once more along -g */
     float fn = 0.0f, xn[ORA_MAXD], gn[ORA_MAXD];
     int accepted = 0;
     for (int attempt = 0; attempt < 2 && !accepted; ++attempt) {
@@ -1012,21 +1161,9 @@ float ora_lbfgs_one(const ora_problem* P, const ora_lbfgs_params* lp, float* xio
         for (int l = 0; l < G; ++l) p[l] = -g[l];
         k = 0;
       }
-      float alpha = 1.0f;
-      for (int bt = 0; bt <= lp->max_backtracks; ++bt) {
-        float dx[ORA_MAXD];
-        for (int l = 0; l < G; ++l) {
-          xn[l] = proj(x[l] + alpha * p[l], lo[l], hi[l]);
-          dx[l] = xn[l] - x[l];
-        }
-        lane_fg(P, xn, G, &fn, gn);
-        float decr = tdot(g, dx, G);
-        if (fn <= f + lp->armijo_c * decr) {
-          accepted = 1;
-          break;
-        }
-        alpha = alpha * lp->armijo_rho;
-      }
+      accepted = lp->linesearch == ORA_LS_STRONG_WOLFE
+                     ? ora_wolfe_search(P, lp, G, x, f, g, p, lo, hi, xn, &fn, gn, NULL)
+                     : armijo_search(P, lp, G, x, f, g, p, lo, hi, xn, &fn, gn);
     }
     if (!accepted) break;
     float s[ORA_MAXD], y[ORA_MAXD];
@@ -1057,6 +1194,14 @@ float ora_lbfgs_one(const ora_problem* P, const ora_lbfgs_params* lp, float* xio
   }
   memcpy(xio, x, sizeof(float) * d);
   return isfinite(f) ? f : INFINITY;
+}
+
+/* Value and gradient as the local search evaluates them (lane-group form). */
+void ora_value_grad(const ora_problem* P, const float* x, float* f, float* g) {
+  int G = group_size(P->d);
+  float xs[ORA_MAXD];
+  for (int l = 0; l < G; ++l) xs[l] = l < P->d ? x[l] : 0.0f;
+  lane_fg(P, xs, G, f, g);
 }
 
 /* All particles from their best positions (AoS swarm); out_x n x d, out_f n. */

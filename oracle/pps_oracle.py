@@ -45,16 +45,27 @@ class _Prob(C.Structure):
     ]
 
 
+LS_ARMIJO, LS_STRONG_WOLFE = 0, 1
+
+
 class _LbfgsParams(C.Structure):
     _fields_ = [("maxiters", C.c_int32), ("history", C.c_int32), ("abstol", C.c_float),
                 ("armijo_c", C.c_float), ("armijo_rho", C.c_float),
-                ("max_backtracks", C.c_int32)]
+                ("max_backtracks", C.c_int32), ("linesearch", C.c_int32),
+                ("wolfe_c2", C.c_float), ("alpha_init", C.c_float), ("alpha_max", C.c_float),
+                ("ls_maxiters", C.c_int32), ("zoom_maxiters", C.c_int32)]
 
 
 def lbfgs_params(maxiters=100, history=10, abstol=0.0, armijo_c=1e-4, armijo_rho=0.5,
-                 max_backtracks=20):
+                 max_backtracks=20, linesearch=LS_ARMIJO, wolfe_c2=0.9, alpha_init=1.0,
+                 alpha_max=65536.0, ls_maxiters=20, zoom_maxiters=30):
+    """Local-search settings.  linesearch LS_ARMIJO (armijo_c, armijo_rho,
+    max_backtracks) or LS_STRONG_WOLFE (c1 = armijo_c, c2 = wolfe_c2,
+    alpha_init, alpha_max, ls_maxiters, zoom_maxiters)."""
     return _LbfgsParams(int(maxiters), int(history), float(abstol), float(armijo_c),
-                        float(armijo_rho), int(max_backtracks))
+                        float(armijo_rho), int(max_backtracks), int(linesearch),
+                        float(wolfe_c2), float(alpha_init), float(alpha_max), int(ls_maxiters),
+                        int(zoom_maxiters))
 
 
 _lib = None
@@ -110,6 +121,10 @@ def lib():
         L.ora_lbfgs_one.restype = C.c_float
         L.ora_lbfgs_run.argtypes = [C.POINTER(_Prob), C.POINTER(_LbfgsParams), C.c_int64, fp, fp,
                                     fp, C.c_int]
+        L.ora_wolfe_search.argtypes = [C.POINTER(_Prob), C.POINTER(_LbfgsParams), C.c_int, fp,
+                                       C.c_float, fp, fp, fp, fp, fp, fp, fp, fp]
+        L.ora_wolfe_search.restype = C.c_int
+        L.ora_value_grad.argtypes = [C.POINTER(_Prob), fp, fp, fp]
         _lib = L
     return _lib
 
@@ -338,6 +353,50 @@ def lbfgs_one(prob: Problem, x0, params=None):
     lp = params or lbfgs_params()
     f = lib().ora_lbfgs_one(C.byref(p), C.byref(lp), _fp(x))
     return x[: prob.d].copy(), np.float32(f)
+
+
+def group_size(d: int) -> int:
+    G = 1
+    while G < d:
+        G <<= 1
+    return G
+
+
+def wolfe_search(prob: Problem, x, p, params=None):
+    """One Strong-Wolfe search from x along p (d floats each) inside the
+    problem's bounds: (ok, alpha, xn, fn, gn, f, g) with f, g at x."""
+    d = prob.d
+    G = group_size(d)
+    lp = params or lbfgs_params(linesearch=LS_STRONG_WOLFE)
+    xs = np.zeros(MAXD, np.float32)
+    ps = np.zeros(MAXD, np.float32)
+    lo = np.zeros(MAXD, np.float32)
+    hi = np.zeros(MAXD, np.float32)
+    xs[:d], ps[:d] = np.asarray(x, np.float32), np.asarray(p, np.float32)
+    lo[:d], hi[:d] = np.asarray(prob.lb, np.float32), np.asarray(prob.ub, np.float32)
+    f, g = value_grad(prob, xs[:d])
+    gs = np.zeros(MAXD, np.float32)
+    gs[:d] = g
+    xn = np.zeros(MAXD, np.float32)
+    gn = np.zeros(MAXD, np.float32)
+    fn = C.c_float()
+    a = C.c_float()
+    pc = prob.c()
+    ok = lib().ora_wolfe_search(C.byref(pc), C.byref(lp), G, _fp(xs), float(f), _fp(gs), _fp(ps),
+                                _fp(lo), _fp(hi), _fp(xn), C.byref(fn), _fp(gn), C.byref(a))
+    return bool(ok), np.float32(a.value), xn[:d].copy(), np.float32(fn.value), gn[:d].copy(), f, g
+
+
+def value_grad(prob: Problem, x):
+    """FP32 value and analytic gradient the local search uses at x."""
+    d = prob.d
+    xs = np.zeros(MAXD, np.float32)
+    xs[:d] = np.asarray(x, np.float32)
+    g = np.zeros(MAXD, np.float32)
+    f = C.c_float()
+    pc = prob.c()
+    lib().ora_value_grad(C.byref(pc), _fp(xs), C.byref(f), _fp(g))
+    return np.float32(f.value), g[:d].copy()
 
 
 def lbfgs_swarm(swarm: "Swarm", params=None):
