@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PPS_ABI_VERSION 2
+#define PPS_ABI_VERSION 3
 #define PPS_MAXCONS 8  /* constraints per problem */
 
 /* ---- status codes ------------------------------------------------------ */
@@ -275,14 +275,43 @@ int pps_swarm_elapsed_ms(pps_swarm* s, float* ms);
 int pps_swarm_free(pps_swarm* s);
 
 /* ---- HybridPSO local phase (src/hybrid.jl:100-158, src/lbfgs.jl:9-21) --- */
+/* Line searches of the local phase, on the projected path
+ * phi(a) = f(P(x + a p)). */
+enum {
+  PPS_LINESEARCH_ARMIJO = 0,      /* first a = rho^k with
+                                     f(P(x + a p)) <= f + c g'(P(x + a p) - x)  */
+  PPS_LINESEARCH_STRONG_WOLFE = 1 /* phi(a) <= phi(0) + c1 a phi'(0) and
+                                     |phi'(a)| <= c2 |phi'(0)| (Nocedal & Wright
+                                     Alg. 3.5/3.6; the reference's only accepted
+                                     search, LineSearch.jl's StrongWolfeLineSearch,
+                                     src/hybrid.jl:129-136); phi' counts the
+                                     coordinates the projection lets move      */
+};
 typedef struct pps_lbfgs_params {
   int32_t maxiters;        /* local_maxiters                                */
   int32_t history;         /* m, 1..16                                      */
   float abstol;            /* stop when max |P(x - grad f) - x| <= abstol   */
-  float armijo_c;          /* sufficient-decrease constant (1e-4)           */
-  float armijo_rho;        /* backtracking factor (0.5)                     */
-  int32_t max_backtracks;  /* step reductions per line search (20)          */
+  float armijo_c;          /* sufficient-decrease constant c1 (1e-4)        */
+  float armijo_rho;        /* Armijo: backtracking factor (0.5)             */
+  int32_t max_backtracks;  /* Armijo: step reductions per search (20)       */
+  int32_t linesearch;      /* PPS_LINESEARCH_*                              */
+  float wolfe_c2;          /* Strong-Wolfe: curvature constant c2 (0.9),
+                              armijo_c < wolfe_c2 < 1                      */
+  float alpha_init;        /* Strong-Wolfe: first trial step (1)            */
+  float alpha_max;         /* Strong-Wolfe: largest trial step (65536); trial
+                              steps double from alpha_init up to it         */
+  int32_t ls_maxiters;     /* Strong-Wolfe: trial steps before a bracket (20) */
+  int32_t zoom_maxiters;   /* Strong-Wolfe: zoom iterations (30); zoom's trial
+                              is the minimiser of the quadratic through
+                              phi(lo), phi'(lo), phi(hi) when inside the
+                              middle 80 % of the bracket, else its midpoint;
+                              without a step meeting both conditions the
+                              last step of sufficient decrease is taken     */
+  float reltol;            /* also stop after a step with
+                              |f - f_new| <= reltol |f| (0: off)            */
 } pps_lbfgs_params;
+/* The reference's defaults: Strong-Wolfe (c1 1e-4, c2 0.9), m = 10, 100
+ * iterations; the Armijo fields at c 1e-4, rho 0.5, 20 reductions. */
 void pps_lbfgs_params_default(pps_lbfgs_params* p);
 /* Projected L-BFGS from every particle's best position (objectives with an
  * analytic gradient: Rosenbrock, Rastrigin, Sphere and the 2-D functions).

@@ -856,6 +856,7 @@ typedef struct ora_lbfgs_params {
   int32_t linesearch;    /* ORA_LS_* */
   float wolfe_c2, alpha_init, alpha_max;
   int32_t ls_maxiters, zoom_maxiters;
+  float reltol;          /* stop after a step with |f - f_new| <= reltol |f| (0: off) */
 } ora_lbfgs_params;
 
 static int group_size(int d) {
@@ -1187,10 +1188,13 @@ float ora_lbfgs_one(const ora_problem* P, const ora_lbfgs_params* lp, float* xio
     } else {
       k = 0; /* no positive curvature along s: restart from a scaled gradient */
     }
+    int stall = lp->reltol > 0.0f && fabsf(f - fn) <= lp->reltol * fabsf(f);
     memcpy(x, xn, sizeof(float) * G);
     memcpy(g, gn, sizeof(float) * G);
     f = fn;
-    if (!(ss > 0.0f)) break; /* projection pinned the step: no progress possible */
+    /* projection pinned the step (no progress possible), or a relative
+     * decrease below reltol */
+    if (!(ss > 0.0f) || stall) break;
   }
   memcpy(xio, x, sizeof(float) * d);
   return isfinite(f) ? f : INFINITY;

@@ -53,19 +53,21 @@ class _LbfgsParams(C.Structure):
                 ("armijo_c", C.c_float), ("armijo_rho", C.c_float),
                 ("max_backtracks", C.c_int32), ("linesearch", C.c_int32),
                 ("wolfe_c2", C.c_float), ("alpha_init", C.c_float), ("alpha_max", C.c_float),
-                ("ls_maxiters", C.c_int32), ("zoom_maxiters", C.c_int32)]
+                ("ls_maxiters", C.c_int32), ("zoom_maxiters", C.c_int32),
+                ("reltol", C.c_float)]
 
 
 def lbfgs_params(maxiters=100, history=10, abstol=0.0, armijo_c=1e-4, armijo_rho=0.5,
                  max_backtracks=20, linesearch=LS_ARMIJO, wolfe_c2=0.9, alpha_init=1.0,
-                 alpha_max=65536.0, ls_maxiters=20, zoom_maxiters=30):
+                 alpha_max=65536.0, ls_maxiters=20, zoom_maxiters=30, reltol=0.0):
     """Local-search settings.  linesearch LS_ARMIJO (armijo_c, armijo_rho,
     max_backtracks) or LS_STRONG_WOLFE (c1 = armijo_c, c2 = wolfe_c2,
-    alpha_init, alpha_max, ls_maxiters, zoom_maxiters)."""
+    alpha_init, alpha_max, ls_maxiters, zoom_maxiters); reltol > 0 also stops
+    after a step with |f - f_new| <= reltol |f|."""
     return _LbfgsParams(int(maxiters), int(history), float(abstol), float(armijo_c),
                         float(armijo_rho), int(max_backtracks), int(linesearch),
                         float(wolfe_c2), float(alpha_init), float(alpha_max), int(ls_maxiters),
-                        int(zoom_maxiters))
+                        int(zoom_maxiters), float(reltol))
 
 
 _lib = None

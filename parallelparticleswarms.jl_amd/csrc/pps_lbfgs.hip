@@ -12,19 +12,19 @@
 // and all branch decisions are group-uniform.
 //
 // Algorithm (Nocedal & Wright Alg. 7.4 two-loop recursion, projected onto the
-// box, Armijo backtracking; the reference's SimpleOptimization._lbfgs is not
+// box; the reference's SimpleOptimization._lbfgs and LineSearch.jl are not
 // vendored under the reference):
 //   x = P(x0); f, g = f(x), grad f(x)
 //   repeat maxiters: stop if max|P(x - g) - x| <= abstol
 //     p = -H g (two-loop; gamma = s'y / y'y of the newest pair); if g'p >= 0:
 //       p = -g and forget the history
-//     alpha = 1; accept the first xn = P(x + alpha p) with
-//       f(xn) <= f + c g'(xn - x), else alpha *= rho (<= max_backtracks times);
-//       stop if none is accepted
+//     line search on phi(a) = f(P(x + a p)) (wolfe_search or armijo_search
+//       below); stop if it fails
 //       (a failed search with history is retried once along -g)
 //     s = xn - x, y = grad f(xn) - g; keep the pair if s'y > 1e-10 y'y, s'y > 0,
 //       otherwise restart the history
-//     stop if the projected step is zero
+//     stop if the projected step is zero, or (reltol > 0) if
+//       |f - f(xn)| <= reltol |f|
 //   non-finite final value -> +Inf (src/lbfgs.jl:20)
 #include <hip/hip_runtime.h>
 
@@ -183,6 +183,127 @@ __device__ __forceinline__ void lane_fg(float xl, int l, int d, const ProbArgs& 
   }
 }
 
+// Armijo backtracking: the first a = rho^k (k <= max_backtracks) with
+// f(P(x + a p)) <= f + c g'(P(x + a p) - x).  Group-uniform result.
+template <int G, int OBJ>
+__device__ __forceinline__ bool armijo_search(float x, float f, float g, float p, float lo,
+                                              float hi, int l, int d, const ProbArgs& P,
+                                              const LbfgsArgs& A, float& xn, float& fn,
+                                              float& gn) {
+  float alpha = 1.0f;
+  for (int bt = 0; bt <= A.max_backtracks; ++bt) {
+    xn = projc(x + alpha * p, lo, hi);
+    const float dx = xn - x;
+    lane_fg<G, OBJ>(xn, l, d, P, fn, gn);
+    const float decr = gsum<G>(g * dx);
+    if (fn <= f + A.armijo_c * decr) return true;
+    alpha = alpha * A.armijo_rho;
+  }
+  return false;
+}
+
+// Strong-Wolfe search (Nocedal & Wright Alg. 3.5 with zoom, Alg. 3.6) on
+// phi(a) = f(P(x + a p)): accept a with phi(a) <= phi(0) + c1 a phi'(0) and
+// |phi'(a)| <= c2 |phi'(0)|.  phi'(0) counts the lanes the projection lets
+// move along p (not on a bound p points out of), phi'(a) the lanes whose
+// unprojected x + a p lies strictly inside the box.  Trial steps alpha_init,
+// doubled up to alpha_max (ls_maxiters of them) until one brackets an
+// acceptable step; zoom's trial is the minimiser of the quadratic through
+// phi(lo), phi'(lo), phi(hi) inside the middle 80 % of the bracket, else the
+// midpoint; without a step meeting both conditions the last step of
+// sufficient decrease is taken (none: false).  Every branch depends on group
+// sums only, so the group stays together.
+template <int G, int OBJ>
+__device__ __forceinline__ bool wolfe_search(float x, float f, float g, float p, float lo,
+                                             float hi, int l, int d, const ProbArgs& P,
+                                             const LbfgsArgs& A, float& xn, float& fn,
+                                             float& gn) {
+  const float d0 = gsum<G>(((x <= lo && p < 0.0f) || (x >= hi && p > 0.0f)) ? 0.0f : g * p);
+  if (!(d0 < 0.0f)) return false;
+  const float c1 = A.armijo_c, curv = -(A.wolfe_c2 * d0);
+  float xl = x, gl = g;  // point at a_lo
+  float a_lo = 0.0f, f_lo = f, d_lo = d0, a_hi = 0.0f, f_hi = 0.0f;
+  float a = A.alpha_init;
+  float xt = x, ft = f, gt = g;
+  bool bracket = false;
+  for (int i = 0; i < A.ls_maxiters; ++i) {
+    const float u = x + a * p;
+    xt = projc(u, lo, hi);
+    lane_fg<G, OBJ>(xt, l, d, P, ft, gt);
+    if (ft > f + (c1 * a) * d0 || (i > 0 && ft >= f_lo)) {
+      a_hi = a;
+      f_hi = ft;
+      bracket = true;
+      break;
+    }
+    const float da = gsum<G>((lo < u && u < hi) ? gt * p : 0.0f);
+    if (__builtin_fabsf(da) <= curv) {
+      xn = xt;
+      fn = ft;
+      gn = gt;
+      return true;
+    }
+    if (da >= 0.0f) {
+      a_hi = a_lo;
+      f_hi = f_lo;
+      a_lo = a;
+      f_lo = ft;
+      d_lo = da;
+      xl = xt;
+      gl = gt;
+      bracket = true;
+      break;
+    }
+    a_lo = a;
+    f_lo = ft;
+    d_lo = da;
+    xl = xt;
+    gl = gt;
+    if (!(a < A.alpha_max)) break;
+    a = a + a;
+    if (a > A.alpha_max) a = A.alpha_max;
+  }
+  if (bracket) {
+    for (int j = 0; j < A.zoom_maxiters; ++j) {
+      const float delta = a_hi - a_lo;
+      const float den = 2.0f * ((f_hi - f_lo) - d_lo * delta);
+      a = a_lo - ((d_lo * delta) * delta) / den;
+      const float b0 = __builtin_fminf(a_lo, a_hi), b1 = __builtin_fmaxf(a_lo, a_hi);
+      const float w = b1 - b0;
+      if (!(a >= b0 + 0.1f * w && a <= b1 - 0.1f * w)) a = a_lo + 0.5f * delta;
+      const float u = x + a * p;
+      xt = projc(u, lo, hi);
+      lane_fg<G, OBJ>(xt, l, d, P, ft, gt);
+      if (ft > f + (c1 * a) * d0 || ft >= f_lo) {
+        a_hi = a;
+        f_hi = ft;
+      } else {
+        const float da = gsum<G>((lo < u && u < hi) ? gt * p : 0.0f);
+        if (__builtin_fabsf(da) <= curv) {
+          xn = xt;
+          fn = ft;
+          gn = gt;
+          return true;
+        }
+        if (da * (a_hi - a_lo) >= 0.0f) {
+          a_hi = a_lo;
+          f_hi = f_lo;
+        }
+        a_lo = a;
+        f_lo = ft;
+        d_lo = da;
+        xl = xt;
+        gl = gt;
+      }
+    }
+  }
+  if (!(a_lo > 0.0f)) return false;
+  xn = xl;
+  fn = f_lo;
+  gn = gl;
+  return true;
+}
+
 template <int G, int OBJ>
 __global__ __launch_bounds__(256) void k_lbfgs(SwarmDev S, ProbArgs P, LbfgsArgs A, float* out_x,
                                                float* out_f) {
@@ -230,8 +351,8 @@ __global__ __launch_bounds__(256) void k_lbfgs(SwarmDev S, ProbArgs P, LbfgsArgs
       p = -g;
       k = 0;
     }
-    // Armijo backtracking; if the quasi-Newton direction fails, forget the
-    // history and search once more along -g.
+    // Line search; if the quasi-Newton direction fails, forget the history
+    // and search once more along -g.
     float fn = 0.0f, gn = 0.0f, xn = x;
     bool accepted = false;
     for (int attempt = 0; attempt < 2 && !accepted; ++attempt) {
@@ -240,18 +361,9 @@ __global__ __launch_bounds__(256) void k_lbfgs(SwarmDev S, ProbArgs P, LbfgsArgs
         p = -g;
         k = 0;
       }
-      float alpha = 1.0f;
-      for (int bt = 0; bt <= A.max_backtracks; ++bt) {
-        xn = projc(x + alpha * p, lo, hi);
-        const float dx = xn - x;
-        lane_fg<G, OBJ>(xn, l, d, P, fn, gn);
-        const float decr = gsum<G>(g * dx);
-        if (fn <= f + A.armijo_c * decr) {
-          accepted = true;
-          break;
-        }
-        alpha = alpha * A.armijo_rho;
-      }
+      accepted = A.linesearch == PPS_LINESEARCH_STRONG_WOLFE
+                     ? wolfe_search<G, OBJ>(x, f, g, p, lo, hi, l, d, P, A, xn, fn, gn)
+                     : armijo_search<G, OBJ>(x, f, g, p, lo, hi, l, d, P, A, xn, fn, gn);
     }
     if (!accepted) break;
     const float s = xn - x, y = gn - g;
@@ -274,10 +386,11 @@ __global__ __launch_bounds__(256) void k_lbfgs(SwarmDev S, ProbArgs P, LbfgsArgs
     } else {
       k = 0;  // no positive curvature along s: restart from a scaled gradient
     }
+    const bool stall = A.reltol > 0.0f && __builtin_fabsf(f - fn) <= A.reltol * __builtin_fabsf(f);
     x = xn;
     g = gn;
     f = fn;
-    if (!(ss > 0.0f)) break;
+    if (!(ss > 0.0f) || stall) break;
   }
   if (act) out_x[(int64_t)l * S.npad + i] = x;
   if (l == 0) out_f[i] = __builtin_isfinite(f) ? f : __builtin_inff();

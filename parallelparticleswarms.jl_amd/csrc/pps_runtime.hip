@@ -782,6 +782,13 @@ void pps_lbfgs_params_default(pps_lbfgs_params* p) {
   p->armijo_c = 1e-4f;
   p->armijo_rho = 0.5f;
   p->max_backtracks = 20;
+  p->linesearch = PPS_LINESEARCH_STRONG_WOLFE;
+  p->wolfe_c2 = 0.9f;
+  p->alpha_init = 1.0f;
+  p->alpha_max = 65536.0f;
+  p->ls_maxiters = 20;
+  p->zoom_maxiters = 30;
+  p->reltol = 0.0f;
 }
 
 }  // extern "C"
@@ -796,8 +803,30 @@ static int lbfgs_check(const pps_swarm* s, const pps_lbfgs_params* lp) {
                 "handling (the swarm has %d constraints)", s->P.n_cons);
   if (lp->history < 1 || lp->history > 16) return fail(PPS_E_INVALID, "history must be 1..16");
   if (lp->maxiters < 0 || lp->max_backtracks < 0) return fail(PPS_E_INVALID, "negative iteration count");
-  if (!(lp->armijo_rho > 0.0f && lp->armijo_rho < 1.0f)) return fail(PPS_E_INVALID, "armijo_rho must be in (0,1)");
+  if (!(lp->reltol >= 0.0f && std::isfinite(lp->reltol)))
+    return fail(PPS_E_INVALID, "reltol must be finite and >= 0");
+  if (lp->linesearch == PPS_LINESEARCH_ARMIJO) {
+    if (!(lp->armijo_rho > 0.0f && lp->armijo_rho < 1.0f))
+      return fail(PPS_E_INVALID, "armijo_rho must be in (0,1)");
+  } else if (lp->linesearch == PPS_LINESEARCH_STRONG_WOLFE) {
+    if (!(lp->armijo_c > 0.0f && lp->armijo_c < lp->wolfe_c2 && lp->wolfe_c2 < 1.0f))
+      return fail(PPS_E_INVALID, "Strong-Wolfe constants need 0 < c1 < c2 < 1 (c1 %g, c2 %g)",
+                  (double)lp->armijo_c, (double)lp->wolfe_c2);
+    if (!(lp->alpha_init > 0.0f && lp->alpha_max >= lp->alpha_init && std::isfinite(lp->alpha_max)))
+      return fail(PPS_E_INVALID, "Strong-Wolfe steps need 0 < alpha_init <= alpha_max < Inf");
+    if (lp->ls_maxiters < 1 || lp->zoom_maxiters < 0)
+      return fail(PPS_E_INVALID, "Strong-Wolfe iteration limits: ls_maxiters >= 1, zoom_maxiters >= 0");
+  } else {
+    return fail(PPS_E_INVALID, "unknown line search %d", lp->linesearch);
+  }
   return PPS_OK;
+}
+
+static LbfgsArgs lbfgs_args(const pps_lbfgs_params* lp) {
+  return LbfgsArgs{lp->maxiters,   lp->history,    lp->abstol,      lp->armijo_c,
+                   lp->armijo_rho, lp->max_backtracks, lp->linesearch, lp->wolfe_c2,
+                   lp->alpha_init, lp->alpha_max,  lp->ls_maxiters, lp->zoom_maxiters,
+                   lp->reltol};
 }
 
 extern "C" {
@@ -819,8 +848,7 @@ int pps_swarm_lbfgs(pps_swarm* s, const pps_lbfgs_params* lp, float* out_x, floa
     s->lf = (float*)((char*)m + bx);
     s->lkey = (uint64_t*)((char*)m + bx + bf);
   }
-  LbfgsArgs A{lp->maxiters, lp->history, lp->abstol, lp->armijo_c, lp->armijo_rho,
-              lp->max_backtracks};
+  const LbfgsArgs A = lbfgs_args(lp);
   HIPCHK(hipEventRecord(s->ev0, st));
   HIPCHK(launch_lbfgs(s->S, s->P, A, s->lx, s->lf, st));
   HIPCHK(launch_argmin_f32(s->lf, n, s->lkey, st));
@@ -873,8 +901,7 @@ int pps_swarm_lbfgs_noalloc(pps_swarm* s, const pps_lbfgs_params* lp, float* out
   float* aos = (float*)(base + o_aos);
   HIPCHK(hipSetDevice(s->ctx->device));
   hipStream_t st = s->ctx->stream;
-  LbfgsArgs A{lp->maxiters, lp->history, lp->abstol, lp->armijo_c, lp->armijo_rho,
-              lp->max_backtracks};
+  const LbfgsArgs A = lbfgs_args(lp);
   HIPCHK(hipEventRecord(s->ev0, st));
   HIPCHK(launch_lbfgs(s->S, s->P, A, lx, lf, st));
   HIPCHK(launch_argmin_f32(lf, n, lkey, st));

@@ -143,6 +143,13 @@ struct LbfgsArgs {
   float armijo_c;
   float armijo_rho;
   int32_t max_backtracks;
+  int32_t linesearch;  // PPS_LINESEARCH_*
+  float wolfe_c2;
+  float alpha_init;
+  float alpha_max;
+  int32_t ls_maxiters;
+  int32_t zoom_maxiters;
+  float reltol;
 };
 
 }  // namespace pps

@@ -8,8 +8,8 @@
 #   solve!(cache, alg)      src/solve.jl:11-59
 #   reinit!(cache)          src/init.jl:117-141
 #   solve(prob, alg)        src/solve.jl:61-67
-#   solve!(::HybridPSOCache, ::HybridPSO{<:MI355XBackend,<:SimpleLBFGS})
-#                           src/hybrid.jl:100-158
+#   init / solve! / reinit! / solve for HybridPSO{<:MI355XBackend,<:SimpleLBFGS}
+#                           src/init.jl:143-168, src/hybrid.jl:100-158
 # Every call crosses into libpps through `ccall`; the C declarations are in
 # include/pps.h.  User code changes only the backend:
 #     solve(prob, ParallelSyncPSOKernel(1000; backend = MI355XBackend()), maxiters = 500)
@@ -21,6 +21,8 @@ using SciMLBase, StaticArrays, QuasiMonteCarlo
 import ParallelParticleSwarms
 import ParallelParticleSwarms: ParallelPSOKernel, ParallelSyncPSOKernel, HybridPSO,
     SimpleLBFGS
+import SimpleOptimization
+import LineSearch: StrongWolfeLineSearch
 import SciMLBase: init, solve!, reinit!, solve, OptimizationProblem, OptimizationStats
 
 const libpps = get(ENV, "PPS_LIBPPS", joinpath(@__DIR__, "..", "lib", "libpps.so"))
@@ -69,6 +71,13 @@ struct CLbfgsParams        # pps_lbfgs_params
     armijo_c::Float32
     armijo_rho::Float32
     max_backtracks::Int32
+    linesearch::Int32      # 0 Armijo, 1 Strong-Wolfe
+    wolfe_c2::Float32
+    alpha_init::Float32
+    alpha_max::Float32
+    ls_maxiters::Int32
+    zoom_maxiters::Int32
+    reltol::Float32
 end
 
 function check(rc)
@@ -96,7 +105,7 @@ objective_code(f) = throw(ArgumentError(
     "the MI355X backend runs a closed objective set; wrap it as PPSObjective(:name), got $(typeof(f))"))
 
 const CTX = Ref{Ptr{Cvoid}}(C_NULL)
-const ABI_VERSION = Int32(2)   # include/pps.h PPS_ABI_VERSION mirrored by CProblem & co.
+const ABI_VERSION = Int32(3)   # include/pps.h PPS_ABI_VERSION mirrored by CProblem & co.
 function context(b::MI355XBackend)
     if CTX[] == C_NULL
         v = ccall((:pps_abi_version, libpps), Cint, ())
@@ -374,15 +383,77 @@ end
 parameter_estim_fhn(data::AbstractMatrix, lb, ub; kwargs...) =
     parameter_estim_lv(data, lb, ub; model = :fitzhugh_nagumo, kwargs...)
 
-# HybridPSO{MI355XBackend, SimpleLBFGS} (src/hybrid.jl:100-158): PSO, then
-# projected L-BFGS from every particle's best position, keep the better.
-function solve(prob::OptimizationProblem, alg::HybridPSO{<:MI355XBackend, <:SimpleLBFGS},
-        args...; maxiters = 100, local_maxiters = 50, abstol = nothing, kwargs...)
-    cache = init(prob, alg.pso; kwargs...)
-    sol_pso = solve!(cache; maxiters)
+# ---- HybridPSO{MI355XBackend, SimpleLBFGS} ----------------------------------
+# init builds the PSO stage's cache (src/init.jl:143-159), reinit! renews it
+# (src/init.jl:160-168), solve! runs the PSO stage, then projected L-BFGS from
+# every particle's best position on the device, and keeps the local argmin when
+# it beats the PSO result (src/hybrid.jl:100-158).
+
+struct MI355XHybridCache{C, A}
+    pso_cache::C
+    alg::A
+end
+
+# Other properties read through to the PSO cache, as the reference's
+# HybridPSOCache does (src/init.jl:170-176).
+Base.getproperty(c::MI355XHybridCache, name::Symbol) =
+    name in (:pso_cache, :alg) ? getfield(c, name) : getproperty(getfield(c, :pso_cache), name)
+
+function init(prob::OptimizationProblem, alg::HybridPSO{<:MI355XBackend, <:SimpleLBFGS},
+        args...; kwargs...)
+    return MI355XHybridCache(init(prob, alg.pso, args...; kwargs...), alg)
+end
+
+function reinit!(cache::MI355XHybridCache)
+    reinit!(cache.pso_cache)
+    return nothing
+end
+
+"""
+    PPSArmijo(; c = 1f-4, rho = 0.5f0, max_backtracks = 20)
+
+Armijo backtracking for the local phase (the HybridPSO benchmark
+configuration's line search; the reference itself accepts only
+`StrongWolfeLineSearch`, whose c1, c2, α_init, α_max, maxiters and
+zoom_maxiters are forwarded to the kernel as the reference forwards them,
+src/hybrid.jl:131-136).
+"""
+struct PPSArmijo
+    c::Float32
+    rho::Float32
+    max_backtracks::Int32
+end
+PPSArmijo(; c = 1.0f-4, rho = 0.5f0, max_backtracks = 20) = PPSArmijo(c, rho, max_backtracks)
+
+history_size(::Val{N}) where {N} = Int32(N)
+history_size(n::Integer) = Int32(n)
+
+function lbfgs_params(local_maxiters, history, abstol, reltol, ls::StrongWolfeLineSearch)
+    return CLbfgsParams(Int32(local_maxiters), history,
+        abstol === nothing ? 0.0f0 : Float32(abstol), Float32(ls.c1), 0.5f0, Int32(20),
+        Int32(1), Float32(ls.c2), Float32(ls.α_init), Float32(ls.α_max), Int32(ls.maxiters),
+        Int32(ls.zoom_maxiters), reltol === nothing ? 0.0f0 : Float32(reltol))
+end
+function lbfgs_params(local_maxiters, history, abstol, reltol, ls::PPSArmijo)
+    return CLbfgsParams(Int32(local_maxiters), history,
+        abstol === nothing ? 0.0f0 : Float32(abstol), ls.c, ls.rho, ls.max_backtracks,
+        Int32(0), 0.9f0, 1.0f0, 65536.0f0, Int32(20), Int32(30),
+        reltol === nothing ? 0.0f0 : Float32(reltol))
+end
+lbfgs_params(local_maxiters, history, abstol, reltol, ls) = throw(ArgumentError(
+    "HybridPSO with SimpleLBFGS on MI355XBackend requires a StrongWolfeLineSearch " *
+    "(or PPSArmijo), got $(typeof(ls))"))
+
+function solve!(cache::MI355XHybridCache, args...; maxiters = 100, local_maxiters = 50,
+        abstol = nothing, reltol = nothing,
+        linesearch = cache.alg.local_opt.linesearch, w = 0.7298f0, wdamp = 1.0f0, kwargs...)
+    opt = cache.alg
+    lp = lbfgs_params(local_maxiters,
+        history_size(SimpleOptimization.__get_threshold(opt.local_opt)), abstol, reltol,
+        linesearch)
+    sol_pso = solve!(cache.pso_cache; maxiters, w, wdamp)
+    prob = cache.pso_cache.prob
     d = length(prob.u0)
-    lp = CLbfgsParams(Int32(local_maxiters), Int32(10),
-        abstol === nothing ? 0.0f0 : Float32(abstol), 1.0f-4, 0.5f0, Int32(20))
     bx = zeros(Float32, d)
     bf = Ref{Float32}(0)
     bi = Ref{Int64}(0)
@@ -390,18 +461,26 @@ function solve(prob::OptimizationProblem, alg::HybridPSO{<:MI355XBackend, <:Simp
     check(ccall((:pps_swarm_lbfgs, libpps), Cint,
         (Ptr{Cvoid}, Ref{CLbfgsParams}, Ptr{Float32}, Ptr{Float32}, Ptr{Float32},
             Ref{Float32}, Ref{Int64}),
-        cache.handle, lp, C_NULL, C_NULL, bx, bf, bi))
+        cache.pso_cache.handle, lp, C_NULL, C_NULL, bx, bf, bi))
     best_u, best_obj = sol_pso.u, sol_pso.objective
-    local_obj = prob.f(typeof(prob.u0)(bx), prob.p)
-    if local_obj < best_obj
-        best_u, best_obj = typeof(prob.u0)(bx), local_obj
+    if isfinite(bf[])   # non-finite local results are +Inf (src/lbfgs.jl:20)
+        local_u = typeof(prob.u0)(bx)
+        local_obj = prob.f(local_u, prob.p)
+        if local_obj < best_obj
+            best_u, best_obj = local_u, local_obj
+        end
     end
     return SciMLBase.build_solution(
-        SciMLBase.DefaultOptimizationCache(prob.f, prob.p), alg, best_u, best_obj;
+        SciMLBase.DefaultOptimizationCache(prob.f, prob.p), opt, best_u, best_obj;
         stats = OptimizationStats(; time = (time() - t0) + sol_pso.stats.time))
 end
 
-export MI355XBackend, PPSObjective, PPSQuadraticConstraints, PPSLatinHypercube,
+function solve(prob::OptimizationProblem, alg::HybridPSO{<:MI355XBackend, <:SimpleLBFGS},
+        args...; maxiters = 100, kwargs...)
+    return solve!(init(prob, alg; kwargs...); maxiters, kwargs...)
+end
+
+export MI355XBackend, PPSObjective, PPSQuadraticConstraints, PPSLatinHypercube, PPSArmijo,
     parameter_estim_lv, parameter_estim_fhn
 
 end # module

@@ -7,10 +7,10 @@ swarm resident in HBM and every iteration in the HIP kernels of libpps.
 from ._lib import PPSError, declared_functions, lib  # noqa: F401
 from .api import (  # noqa: F401
     Beale, Booth, BoundaryQuadratic, Context, GBest, GPUUnboundedSampler, GPUUniformSampler,
-    Himmelblau, HybridPSO, LatinHypercubeSample, MI355XBackend,
+    ArmijoBacktracking, Himmelblau, HybridPSO, HybridPSOCache, LatinHypercubeSample, MI355XBackend,
     Objective, OptimizationProblem, OptimizationSolution, ParallelPSOKernel, QuadraticConstraints,
     ParallelSyncPSOKernel, PSOCache, Quadratic, Rastrigin, Rosen2, Rosenbrock, SimpleLBFGS,
-    Sphere, Swarm, autotune_sync, context, eval_f64, init, lotka_volterra_sse, fitzhugh_nagumo_sse, ode_sse, reinit_, seed_, solve, solve_,
+    Sphere, StrongWolfeLineSearch, Swarm, autotune_sync, context, eval_f64, init, lotka_volterra_sse, fitzhugh_nagumo_sse, ode_sse, reinit_, seed_, solve, solve_,
     swarm_params)
 from .serve import Request, Result, Server, batch_solve, solve_alone  # noqa: F401
 from .workloads import served_mix  # noqa: F401

@@ -46,7 +46,7 @@ class PPSError(RuntimeError):
         self.code = code
 
 
-ABI_VERSION = 2  # include/pps.h PPS_ABI_VERSION mirrored by the structures below
+ABI_VERSION = 3  # include/pps.h PPS_ABI_VERSION mirrored by the structures below
 
 
 class pps_problem(C.Structure):
@@ -66,10 +66,16 @@ class pps_swarm_params(C.Structure):
     ]
 
 
+LINESEARCH_ARMIJO, LINESEARCH_STRONG_WOLFE = 0, 1
+
+
 class pps_lbfgs_params(C.Structure):
     _fields_ = [("maxiters", C.c_int32), ("history", C.c_int32), ("abstol", C.c_float),
                 ("armijo_c", C.c_float), ("armijo_rho", C.c_float),
-                ("max_backtracks", C.c_int32)]
+                ("max_backtracks", C.c_int32), ("linesearch", C.c_int32),
+                ("wolfe_c2", C.c_float), ("alpha_init", C.c_float), ("alpha_max", C.c_float),
+                ("ls_maxiters", C.c_int32), ("zoom_maxiters", C.c_int32),
+                ("reltol", C.c_float)]
 
 
 class pps_server_config(C.Structure):

@@ -389,16 +389,63 @@ class ParallelSyncPSOKernel:
 
 
 @dataclass
+class StrongWolfeLineSearch:
+    """Strong-Wolfe line search on the projected path (Nocedal & Wright
+    Alg. 3.5/3.6): the fields the reference forwards from LineSearch.jl's
+    StrongWolfeLineSearch to the local search (src/hybrid.jl:131-136)."""
+
+    c1: float = 1e-4
+    c2: float = 0.9
+    alpha_init: float = 1.0
+    alpha_max: float = 65536.0
+    maxiters: int = 20
+    zoom_maxiters: int = 30
+
+
+@dataclass
+class ArmijoBacktracking:
+    """Armijo backtracking: the first a = rho^k (k <= max_backtracks) with
+    f(P(x + a p)) <= f + c g'(P(x + a p) - x).  Not a reference line search;
+    the HybridPSO benchmark configuration names it."""
+
+    c: float = 1e-4
+    rho: float = 0.5
+    max_backtracks: int = 20
+
+
+@dataclass
 class SimpleLBFGS:
     """Projected L-BFGS local search: history ``threshold`` (the reference's
-    history size, SimpleOptimization.__get_threshold), Armijo backtracking
-    with constant ``armijo_c``, factor ``armijo_rho``, at most
-    ``max_backtracks`` reductions per search."""
+    history size, SimpleOptimization.__get_threshold) and ``linesearch``,
+    StrongWolfeLineSearch() by default as in the reference, or
+    ArmijoBacktracking()."""
 
     threshold: int = 10
-    armijo_c: float = 1e-4
-    armijo_rho: float = 0.5
-    max_backtracks: int = 20
+    linesearch: Any = field(default_factory=StrongWolfeLineSearch)
+
+
+def lbfgs_params(maxiters: int, history: int, abstol: float, linesearch,
+                 reltol: float = 0.0) -> "L.pps_lbfgs_params":
+    """pps_lbfgs_params for a line-search marker; anything else is refused
+    (the reference throws ArgumentError for a line search other than
+    StrongWolfeLineSearch, src/hybrid.jl:129-130)."""
+    lp = L.pps_lbfgs_params()
+    L.lib().pps_lbfgs_params_default(C.byref(lp))
+    lp.maxiters, lp.history, lp.abstol = int(maxiters), int(history), float(abstol)
+    lp.reltol = float(reltol)
+    if isinstance(linesearch, StrongWolfeLineSearch):
+        lp.linesearch = L.LINESEARCH_STRONG_WOLFE
+        lp.armijo_c, lp.wolfe_c2 = float(linesearch.c1), float(linesearch.c2)
+        lp.alpha_init, lp.alpha_max = float(linesearch.alpha_init), float(linesearch.alpha_max)
+        lp.ls_maxiters, lp.zoom_maxiters = int(linesearch.maxiters), int(linesearch.zoom_maxiters)
+    elif isinstance(linesearch, ArmijoBacktracking):
+        lp.linesearch = L.LINESEARCH_ARMIJO
+        lp.armijo_c, lp.armijo_rho = float(linesearch.c), float(linesearch.rho)
+        lp.max_backtracks = int(linesearch.max_backtracks)
+    else:
+        raise ValueError("the MI355X local search takes linesearch=StrongWolfeLineSearch(...) or "
+                         f"ArmijoBacktracking(...), got {type(linesearch).__name__}")
+    return lp
 
 
 @dataclass
@@ -514,11 +561,16 @@ class Swarm:
                                             int(it)))
 
     def lbfgs(self, maxiters=100, history=10, abstol=0.0, armijo_c=1e-4, armijo_rho=0.5,
-              max_backtracks=20, want_all=True):
-        """Projected L-BFGS from every particle's best position.  Returns
-        (x n x d or None, f n or None, best_x, best_f, best_idx)."""
-        lp = L.pps_lbfgs_params(int(maxiters), int(history), float(abstol), float(armijo_c),
-                                float(armijo_rho), int(max_backtracks))
+              max_backtracks=20, want_all=True, linesearch=None, reltol=0.0):
+        """Projected L-BFGS from every particle's best position, with
+        ``linesearch`` (a StrongWolfeLineSearch or ArmijoBacktracking marker;
+        None: Armijo backtracking with armijo_c, armijo_rho, max_backtracks).
+        reltol > 0 also stops a particle after a step with |f - f_new| <=
+        reltol |f|.  Returns (x n x d or None, f n or None, best_x, best_f,
+        best_idx)."""
+        if linesearch is None:
+            linesearch = ArmijoBacktracking(armijo_c, armijo_rho, max_backtracks)
+        lp = lbfgs_params(maxiters, history, abstol, linesearch, reltol)
         ox = np.zeros((self.n, self.d), np.float32) if want_all else None
         of = np.zeros(self.n, np.float32) if want_all else None
         bx = np.zeros(self.d, np.float32)
@@ -592,13 +644,32 @@ def _host_sample(sampler, prob: OptimizationProblem, n: int) -> np.ndarray:
     return np.ascontiguousarray(x.T)
 
 
+class HybridPSOCache:
+    """``init(prob, HybridPSO)`` (src/init.jl:143-159): the PSO stage's cache
+    and the algorithm; other attributes read through to the PSO cache, as the
+    reference's getproperty does (src/init.jl:170-176)."""
+
+    def __init__(self, pso_cache: PSOCache, alg: "HybridPSO"):
+        self.pso_cache = pso_cache
+        self.alg = alg
+
+    def __getattr__(self, name):
+        if name in ("pso_cache", "alg"):
+            raise AttributeError(name)
+        return getattr(self.pso_cache, name)
+
+
 def init(prob: OptimizationProblem, alg, *, seed: Optional[int] = None, sampler=None,
-         **kw) -> PSOCache:
+         **kw):
     """``init(prob, alg; sampler)`` (src/init.jl:56-115): sampler is
     GPUUniformSampler() (default), GPUUnboundedSampler(), LatinHypercubeSample()
-    or a host sampler with ``sample(n, lb, ub)``."""
+    or a host sampler with ``sample(n, lb, ub)``.  For HybridPSO: a
+    HybridPSOCache around the PSO stage's cache."""
     if isinstance(alg, HybridPSO):
-        return init(prob, alg.pso, seed=seed, sampler=sampler, **kw)
+        if not isinstance(alg.local_opt, SimpleLBFGS):
+            raise ValueError("HybridPSO on the MI355X backend supports local_opt=SimpleLBFGS() "
+                             f"(got {type(alg.local_opt).__name__})")
+        return HybridPSOCache(init(prob, alg.pso, seed=seed, sampler=sampler, **kw), alg)
     seed = _next_seed() if seed is None else int(seed)
     ctx = context(alg.backend)
     code = getattr(sampler, "code", L.SAMPLER_UNIFORM)
@@ -609,8 +680,12 @@ def init(prob: OptimizationProblem, alg, *, seed: Optional[int] = None, sampler=
     return PSOCache(prob, alg, sw, sw.gbest(), seed, sampler if host else None)
 
 
-def reinit_(cache: PSOCache) -> None:
-    """``reinit!``: a fresh initial swarm from the cache's sampler."""
+def reinit_(cache) -> None:
+    """``reinit!``: a fresh initial swarm from the cache's sampler (a
+    HybridPSOCache reinitialises its PSO stage, src/init.jl:160-168)."""
+    if isinstance(cache, HybridPSOCache):
+        reinit_(cache.pso_cache)
+        return
     if cache.sampler is not None:
         cache.swarm.reinit(_host_sample(cache.sampler, cache.prob, cache.swarm.n))
     else:
@@ -618,11 +693,19 @@ def reinit_(cache: PSOCache) -> None:
     cache.gbest = cache.swarm.gbest()
 
 
-def solve_(cache: PSOCache, maxiters: int = 100, *, w: float = 0.7298, wdamp: float = 1.0,
-           original: bool = True) -> OptimizationSolution:
+def solve_(cache, maxiters: int = 100, *, w: float = 0.7298, wdamp: float = 1.0,
+           original: bool = True, **kw) -> OptimizationSolution:
     """``solve!``.  For ParallelSyncPSOKernel the attractor of the first
     iteration is the cache's initial global best on every call, as in the
-    reference (its solve! does not store the result back, src/solve.jl:36-59)."""
+    reference (its solve! does not store the result back, src/solve.jl:36-59).
+    A HybridPSOCache runs the PSO stage, then the local phase
+    (local_maxiters, abstol, reltol, linesearch; src/hybrid.jl:100-158)."""
+    if isinstance(cache, HybridPSOCache):
+        from .hybrid import solve_hybrid_cache
+
+        return solve_hybrid_cache(cache, maxiters, w=w, wdamp=wdamp, **kw)
+    if kw:
+        raise TypeError(f"unexpected keyword arguments {sorted(kw)}")
     sw = cache.swarm
     if isinstance(cache.alg, ParallelSyncPSOKernel):
         sw.set_gbest(cache.gbest.position, cache.gbest.cost)
@@ -640,8 +723,6 @@ def solve_(cache: PSOCache, maxiters: int = 100, *, w: float = 0.7298, wdamp: fl
 def solve(prob: OptimizationProblem, alg, maxiters: int = 100, *, seed: Optional[int] = None,
           w: float = 0.7298, wdamp: float = 1.0, **kw) -> OptimizationSolution:
     if isinstance(alg, HybridPSO):
-        from .hybrid import solve_hybrid
-
-        return solve_hybrid(prob, alg, maxiters, seed=seed, w=w, wdamp=wdamp, **kw)
+        return solve_(init(prob, alg, seed=seed), maxiters, w=w, wdamp=wdamp, **kw)
     cache = init(prob, alg, seed=seed)
     return solve_(cache, maxiters, w=w, wdamp=wdamp)

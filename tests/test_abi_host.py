@@ -26,7 +26,7 @@ def test_abi_version_and_defaults(pps):
     from ppswarm import _lib
 
     L = _lib.lib()
-    assert L.pps_abi_version() == _lib.ABI_VERSION == 2
+    assert L.pps_abi_version() == _lib.ABI_VERSION == 3
     p = _lib.pps_swarm_params()
     L.pps_swarm_params_default(C.byref(p))
     # reference defaults: c1 = c2 = 1.4962f0 (src/kernels.jl:41-42), queue
@@ -119,3 +119,26 @@ def test_served_mix_generator(pps):
     assert all((np.abs(q.prob.shift) <= 1).all() for q in reqs)
     # Poisson arrivals: mean gap 1/qps
     assert abs(np.diff(arr).mean() - 1e-3) < 1e-4
+
+
+def test_hybrid_refuses_what_the_local_search_does_not_implement(pps):
+    from ppswarm import api, hybrid
+
+    # line searches other than Strong-Wolfe / Armijo (the reference throws
+    # ArgumentError for anything but StrongWolfeLineSearch, src/hybrid.jl:129-130)
+    with pytest.raises(ValueError):
+        api.lbfgs_params(10, 10, 0.0, object())
+    lp = api.lbfgs_params(10, 10, 0.0, pps.StrongWolfeLineSearch(c2=0.5), reltol=1e-6)
+    assert lp.linesearch == 1 and abs(lp.wolfe_c2 - 0.5) < 1e-7 and lp.reltol > 0
+    lp = api.lbfgs_params(10, 10, 0.0, pps.ArmijoBacktracking(rho=0.25))
+    assert lp.linesearch == 0 and lp.armijo_rho == 0.25
+    # the reference's defaults: SimpleLBFGS() searches with Strong-Wolfe
+    assert isinstance(pps.SimpleLBFGS().linesearch, pps.StrongWolfeLineSearch)
+    prob = _prob(pps, pps.Sphere(), 3)
+    with pytest.raises(ValueError):   # local optimisers other than SimpleLBFGS
+        pps.init(prob, pps.HybridPSO(local_opt=object()))
+    cache = api.HybridPSOCache(None, pps.HybridPSO())
+    with pytest.raises(ValueError):
+        hybrid.solve_hybrid_cache(cache, 10, reltol=-1.0)
+    with pytest.raises(ValueError):
+        hybrid.solve_hybrid_cache(cache, 10, linesearch="hager-zhang")

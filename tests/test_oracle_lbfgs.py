@@ -113,3 +113,23 @@ def test_lbfgs_history_one_and_zero_iterations(oracle):
     x, f = o.lbfgs_one(P, x0, o.lbfgs_params(maxiters=1, history=1))
     np.testing.assert_array_equal(x, np.zeros(4, np.float32))
     assert f == 0.0
+
+
+@pytest.mark.parametrize("ls", [0, 1])
+def test_lbfgs_reltol_stops_at_the_first_small_relative_decrease(oracle, ls):
+    # the run with reltol r equals the plain run cut at the first iteration k
+    # whose step decreased f by at most r |f| (plain runs of increasing
+    # maxiters are prefixes of one another)
+    o = oracle
+    P = _P(o, o.ROSENBROCK, 4, (1, 100), -5, 5)
+    x0 = np.array([-1.2, 1.0, -0.5, 0.8], np.float32)
+    fk = [o.lbfgs_one(P, x0, o.lbfgs_params(maxiters=k, history=10, linesearch=ls))[1]
+          for k in range(0, 80)]
+    for r in (1e-2, 1e-4):
+        K = next(k for k in range(1, 80) if abs(fk[k - 1] - fk[k]) <= np.float32(r) * abs(fk[k - 1]))
+        assert K > 1
+        x, f = o.lbfgs_one(P, x0, o.lbfgs_params(maxiters=500, history=10, linesearch=ls,
+                                                reltol=r))
+        xk, f_k = o.lbfgs_one(P, x0, o.lbfgs_params(maxiters=K, history=10, linesearch=ls))
+        np.testing.assert_array_equal(x, xk)
+        assert f == f_k
