@@ -247,7 +247,7 @@ def _ode_leg(run, particles, iters, extra=None, compare_full=False):
     return out
 
 
-def ode_c3(P):
+def ode_c3(P, L, clock_khz):
     import json as _json
     doc = _json.load(open(os.path.join(ROOT, "tests", "golden", "lv_obs.json")))
     obs = np.array(doc["obs"], np.float32)
@@ -262,6 +262,25 @@ def ode_c3(P):
            "particle_iters_per_s": (1 << 14) * 300 / (ms / 1e3),
            "rk4_steps_per_s": (1 << 14) * 300 * rk4_steps / (ms / 1e3),
            "theta": [float(v) for v in sol.u], "loss_f64": sol.objective}
+    # Latency floor: a particle's loss is one chain of 1000 dependent RK4
+    # steps, so an iteration cannot end before one workgroup's chain does.
+    # The same kernel and iteration graph on 64 particles (one workgroup of
+    # two waves, nothing else on the device) measures that floor, launch and
+    # global-best hand-over included; the full swarm's time per iteration is
+    # compared with it.
+    _, sw1 = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 64, 300, seed=3,
+                                   launch=L.LAUNCH_GRAPH)
+    sw1.free()
+    _, sw1 = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 64, 300, seed=3,
+                                   launch=L.LAUNCH_GRAPH)
+    floor_us = sw1.elapsed_ms() * 1e3 / 300
+    sw1.free()
+    it_us = ms * 1e3 / 300
+    out["latency_bound"] = {
+        "bound": "dependent chain", "floor_us_per_iter": floor_us, "us_per_iter": it_us,
+        "frac": floor_us / it_us,
+        "cycles_per_rk4_step_at_floor": floor_us * clock_khz / 1e3 / rk4_steps,
+        "workload": "the same kernel on 64 particles (one workgroup), 300 iterations"}
     # the reference's default solver, GPUTsit5: fixed steps (dt = 0.01) and
     # adaptive steps (abstol 1e-6, reltol 1e-3, initial dt 0.01), same shape
     for key, adaptive in (("tsit5", False), ("tsit5_adaptive", True)):
@@ -346,6 +365,32 @@ def beyond_cache(P, L, lg=22, iters=200):
             "particle_iters_per_s": n / (avg * 1e-6)}
 
 
+# Problems with n * d at or above this run as several workgroups per
+# iteration (PART items) in the packed kernel, smaller ones in one workgroup
+# each (SMALL items); the library's split.
+LARGE_WORK = 8192
+
+
+def profile_mix(P, kind):
+    """Only the packed solve of the 4096-problem served mix, or of its SMALL
+    or PART problems, with no warm-up (for counter runs, whose per-dispatch
+    figures are summed over every k_slice dispatch).  Algorithmic bytes: each
+    problem's init writes x, v, best (3d) and best cost, each iteration reads
+    x, v, best (3d) and best cost and writes x, v (2d)."""
+    reqs, _ = P.served_mix(4096, seed=2024)
+    if kind == "small":
+        reqs = [q for q in reqs if q.n * q.prob.d < LARGE_WORK]
+    elif kind == "large":
+        reqs = [q for q in reqs if q.n * q.prob.d >= LARGE_WORK]
+    t0 = time.perf_counter()
+    res, ms = P.batch_solve(reqs)
+    wall = time.perf_counter() - t0
+    ab = sum(q.n * 4 * ((3 * q.prob.d + 1) + q.maxiters * (5 * q.prob.d + 1)) for q in reqs)
+    return {"profile_mix": kind, "problems": len(reqs), "wall_s": wall,
+            "particle_iters": sum(q.n * q.maxiters for q in reqs), "algorithmic_bytes": ab,
+            "algorithmic_gbps": ab / wall / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -355,6 +400,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--extras", default="hbm,async,qlock,c0,c4,hybrid,ode",
                     help="comma list of the legs to run besides the headline")
+    ap.add_argument("--profile-mix", choices=("all", "small", "large"),
+                    help="run only the packed solve of the served mix (all, its SMALL or its "
+                         "PART problems) and print its algorithmic bytes; for counter runs")
     ap.add_argument("--sweep", action="store_true",
                     help="also time the sync step at 2^10..2^24 particles (adds ~1 min)")
     args = ap.parse_args()
@@ -370,6 +418,9 @@ def main():
     ctx = P.context()
     info = ctx.info()
     log(f"[bench] device: {info}")
+    if args.profile_mix:
+        print(json.dumps(profile_mix(P, args.profile_mix)), flush=True)
+        return
     swarms = [make_swarm(P, "rastrigin", N, D, 101, L.MODE_SYNC),
               make_swarm(P, "rosenbrock", N, D, 202, L.MODE_SYNC)]
 
@@ -479,7 +530,7 @@ def main():
         out["hybrid"] = hybrid_c2(P, L, ctx)
         log("[bench] hybrid done")
     if "ode" in legs:
-        out["ode"] = ode_c3(P)
+        out["ode"] = ode_c3(P, L, info["clock_khz"])
         log("[bench] ode done")
 
     if not args.no_cpu_baseline:

@@ -66,9 +66,9 @@ struct BatchItem {
 };
 
 bool batch_supported(int obj, int d);
-// Parts of a large problem of n particles (tiles of 256 particles, at most
-// BATCH_MAXPARTS).
-int batch_parts(int64_t n);
+// Parts of a large problem of n particles: tiles of 256 particles dealt
+// round robin, tiles_per_part of them per part, at most BATCH_MAXPARTS parts.
+int batch_parts(int64_t n, int tiles_per_part);
 // One slice: items[0..count) each run in one workgroup of BATCH_BLOCK threads.
 // SMALL items stop after the first iteration that ends past budget_ticks of
 // the device wall clock.  Finished problems publish result + done flag to

@@ -463,9 +463,10 @@ bool batch_supported(int obj, int d) {
   return d >= 1 && d <= PPS_BMAXD;
 }
 
-int batch_parts(int64_t n) {
+int batch_parts(int64_t n, int tiles_per_part) {
   const int64_t tiles = (n + PTILE - 1) / PTILE;
-  return (int)(tiles < BATCH_MAXPARTS ? tiles : BATCH_MAXPARTS);
+  const int64_t parts = (tiles + tiles_per_part - 1) / tiles_per_part;
+  return (int)(parts < BATCH_MAXPARTS ? parts : BATCH_MAXPARTS);
 }
 
 hipError_t launch_slice(BatchProb* probs, const BatchProb* fresh, const BatchItem* items,

@@ -47,7 +47,7 @@ double now_s() {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Swarms with n * d at or above this run as `batch_parts(n)` workgroups, one
+// Swarms with n * d at or above this run as `batch_parts(n, .)` workgroups, one
 // iteration per slice; smaller ones run their own loop in one workgroup.  A
 // slice lasts at least the longest single iteration of a SMALL item (about
 // n * d / 2048 dependent rounds of loads in one workgroup), and one slice is
@@ -61,6 +61,15 @@ constexpr int64_t kLargeWork = 8192;
 // 20 us, 17.0/47.2 at 40 us, 13.4/30.7 at 10 us (the slice's fixed cost
 // dominates there), with the same throughput.
 constexpr int kSliceUsWithLarge = 20;
+// Tiles per PART item of a large problem.  A PART item spends a few dependent
+// round trips (its descriptor, the part keys, the winner's position, and at
+// the end its own winner's row) around the tiles it streams; two tiles per
+// item amortise them (batched mix 5488 -> 6003 solves/s) but lengthen the
+// slice that paces every large problem (served p99 28.8 -> 44.8 ms; four
+// tiles: 5883 solves/s, p99 72.9 ms).  So a problem admitted while at least
+// kBacklog requests wait behind it (throughput) gets two-tile parts, and one
+// admitted without such a backlog (latency) one-tile parts.
+constexpr size_t kBacklog = 64;
 
 enum { R_PENDING = 0, R_RUNNING = 1, R_DONE = 2, R_CANCELLED = 3, R_FAILED = 4 };
 
@@ -266,7 +275,7 @@ void pps_server::loop() {
         r->large = r->n * r->P.d >= kLargeWork;
         if (r->large) {
           // parts read the device copy; upload it ahead of the slice
-          r->parts = batch_parts(r->n);
+          r->parts = batch_parts(r->n, pending.size() >= kBacklog ? 2 : 1);
           r->t_next = 0;
           r->w_next = r->w;
           (void)hipMemcpyAsync(d_desc + s, &B, sizeof(BatchProb), hipMemcpyHostToDevice, stream);
