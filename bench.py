@@ -4,7 +4,9 @@
 Headline workload (one step): the plain ParallelSyncPSOKernel path at the
 single-GPU benchmark shape — 10-D Rastrigin and 10-D Rosenbrock (p = [1, 100]),
 bounds [-5.12, 5.12], 2^20 particles each, 1000 iterations each, FP32 SoA state
-resident in HBM, the iteration loop replayed as a captured hipGraph.  `value`
+resident in HBM, the launch shape chosen by the library's autotuner (untimed,
+before the warm-up) and the iterations launched as the library chooses at this
+size (one launch each; a captured graph below 2^19 particles).  `value`
 is particle-iterations per second over the timed steps (init excluded, as in
 the reference's benchmarks/CPU_vs_GPU/benchmark.jl:107-121).
 
@@ -68,7 +70,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_swarm(P, obj, n, d, seed, mode, launch=None):
+def make_swarm(P, obj, n, d, seed, mode, launch=None, shape=(0, 0)):
     lb = np.full(d, -5.12, np.float32)
     ub = np.full(d, 5.12, np.float32)
     f = P.Rastrigin() if obj == "rastrigin" else P.Rosenbrock()
@@ -77,7 +79,8 @@ def make_swarm(P, obj, n, d, seed, mode, launch=None):
     from ppswarm import _lib as L
 
     sp = P.swarm_params(n, mode, seed=seed,
-                        launch=L.LAUNCH_GRAPH if launch is None else launch)
+                        launch=L.LAUNCH_AUTO if launch is None else launch,
+                        particles_per_workgroup=shape[0], grid_per_cu=shape[1])
     return P.Swarm(P.context(), prob, sp)
 
 
@@ -397,6 +400,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner before the headline (default launch shape)")
+    ap.add_argument("--launch-shape", default=None,
+                    help="PARTICLES_PER_WORKGROUP/GRID_PER_CU of the headline swarms instead of "
+                         "the autotuner's choice (e.g. 512/2)")
+    ap.add_argument("--eager", action="store_true",
+                    help="force one launch per iteration for the headline swarms (the "
+                         "library's own choice at 2^20 particles)")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--extras", default="hbm,async,qlock,c0,c4,hybrid,ode",
                     help="comma list of the legs to run besides the headline")
@@ -421,8 +432,27 @@ def main():
     if args.profile_mix:
         print(json.dumps(profile_mix(P, args.profile_mix)), flush=True)
         return
-    swarms = [make_swarm(P, "rastrigin", N, D, 101, L.MODE_SYNC),
-              make_swarm(P, "rosenbrock", N, D, 202, L.MODE_SYNC)]
+    # The library's autotuner picks particles per workgroup and workgroups per
+    # CU for each headline problem (cached for the swarms created after it);
+    # untimed, before the warm-up.
+    tuned = {}
+    shape = (0, 0)
+    if args.launch_shape:
+        ppw, g = (int(v) for v in args.launch_shape.split("/"))
+        shape = (ppw, g)
+        tuned = {"forced": {"particles_per_workgroup": ppw, "grid_per_cu": g}}
+    elif not args.no_tune:
+        for obj in ("rastrigin", "rosenbrock"):
+            f = P.Rastrigin() if obj == "rastrigin" else P.Rosenbrock()
+            prob = P.OptimizationProblem(f, np.zeros(D, np.float32),
+                                         None if obj == "rastrigin" else [1.0, 100.0],
+                                         lb=np.full(D, -5.12), ub=np.full(D, 5.12))
+            g, b, us = P.autotune_sync(prob, N, iters=50)
+            tuned[obj] = {"grid_per_cu": g, "particles_per_workgroup": b, "us_per_iter": us}
+        log(f"[bench] autotuned: {tuned}")
+    hl = L.LAUNCH_EAGER if args.eager else L.LAUNCH_AUTO
+    swarms = [make_swarm(P, "rastrigin", N, D, 101, L.MODE_SYNC, hl, shape),
+              make_swarm(P, "rosenbrock", N, D, 202, L.MODE_SYNC, hl, shape)]
 
     def step():
         ms = 0.0
@@ -453,8 +483,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plain ParallelSyncPSOKernel, 10-D Rastrigin + 10-D Rosenbrock "
                                "(p=[1,100]), bounds [-5.12,5.12], 2^20 particles each, "
-                               "1000 iterations each per step, FP32 SoA, hipGraph loop",
-                   "particles": N, "dim": D, "iterations": T, "problems_per_step": 2},
+                               "1000 iterations each per step, FP32 SoA, "
+                               + "one launch per iteration (the library's choice from 2^19 "
+                                 "particles; a captured graph below)",
+                   "particles": N, "dim": D, "iterations": T, "problems_per_step": 2,
+                   "launch": tuned or "default (256 particles per workgroup, 2 per CU)"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic_from_profiles(), "kernel": "k_sync_step",

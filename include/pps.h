@@ -116,7 +116,8 @@ enum {
 
 /* Launch strategy for PPS_MODE_SYNC. */
 enum {
-  PPS_LAUNCH_AUTO = 0,
+  PPS_LAUNCH_AUTO = 0,       /* WORKGROUP up to 1024 particles, GRAPH up to
+                                2^19, EAGER from 2^19 particles            */
   PPS_LAUNCH_EAGER = 1,      /* one launch per iteration, no graph            */
   PPS_LAUNCH_GRAPH = 2,      /* iteration loop captured in a hipGraph         */
   PPS_LAUNCH_WORKGROUP = 3   /* whole loop in one workgroup (small swarms)    */
@@ -193,6 +194,17 @@ typedef struct pps_swarm_params {
                                 reinit); bounds that are infinite in every
                                 coordinate select PPS_SAMPLER_UNBOUNDED, as
                                 the reference's init does (src/init.jl:70-74) */
+  int32_t particles_per_workgroup; /* particles (= threads) per workgroup of
+                                the multi-workgroup sync step: 256, 512 or
+                                1024 (Rosenbrock, Rastrigin and Sphere at
+                                d in {1,2,3,4,8,10,16}, 256 or 512 at d = 32,
+                                without constraints, state arrays below
+                                4 GiB), 256 otherwise; 0 =
+                                autotuned value if one is cached for
+                                (objective, d, N), else 256.  With G
+                                workgroups each thread steps ceil(tiles / G)
+                                tiles of that many particles per iteration.
+                                Results do not depend on it.               */
 } pps_swarm_params;
 
 /* Initial-position samplers (the reference's `sampler` keyword of init,
@@ -339,17 +351,25 @@ int pps_swarm_lbfgs_noalloc(pps_swarm* s, const pps_lbfgs_params* lp, float* out
                             int64_t* best_idx);
 
 /* ---- autotuning ------------------------------------------------------- */
-/* Time the multi-workgroup sync step for grid_per_cu in {1, 2, 3, 4, 6, 8} on a
- * scratch swarm of `n` particles (`iters` timed iterations each), remember the
- * fastest for (objective, d, ceil(log2 n)) for later pps_swarm_init calls in
- * this process, and append it to the file named by $PPS_TUNE_CACHE if set
- * (read back at the first lookup).  best_grid_per_cu and best_us_per_iter may
- * be NULL. */
+/* Time the multi-workgroup sync step on a scratch swarm of `n` particles
+ * (after max(iters, 100) untimed iterations, `iters` timed ones, best of
+ * three) for every particles_per_workgroup
+ * the kernels are compiled for (256, 512, 1024; 256 only for other objectives
+ * and dimensions) and grid_per_cu in {1, 2, 3, 4, 6, 8} (which sets the tiles
+ * each thread steps per iteration), remember the fastest pair for
+ * (objective, d, ceil(log2 n)) for later pps_swarm_init calls in this
+ * process, and append the line
+ *     objective d ceil(log2 n) grid_per_cu particles_per_workgroup
+ * to the file named by $PPS_TUNE_CACHE if set (read back at the first lookup;
+ * lines without the last field mean 256).  Output pointers may be NULL. */
 int pps_autotune_sync(pps_ctx* ctx, const pps_problem* prob, int64_t n,
                       int32_t iters, int32_t* best_grid_per_cu,
+                      int32_t* best_particles_per_workgroup,
                       float* best_us_per_iter);
-/* The cached choice for (objective, d, n), or 0 if none. */
-int pps_tuned_grid_per_cu(int32_t objective, int32_t d, int64_t n);
+/* The cached choice for (objective, d, n): PPS_OK with both values, or
+ * PPS_E_NOTFOUND (outputs untouched). */
+int pps_tuned_launch(int32_t objective, int32_t d, int64_t n,
+                     int32_t* grid_per_cu, int32_t* particles_per_workgroup);
 
 /* ---- host FP64 re-evaluation -------------------------------------------- */
 int pps_eval_f64(const pps_problem* prob, int64_t n, const double* x_aos,

@@ -15,6 +15,10 @@ struct StepArgs {
 // One set of launchers per (objective, dimension bucket).
 struct KernelSet {
   int maxd;
+  // Largest workgroup (particles) the multi-workgroup sync kernels (init,
+  // rekey, sync_step) are compiled for: 1024 or 512 (then also 512 and 256),
+  // or the objective's block_of() size alone.
+  int max_block;
   // Uniform init of every particle, then block keys/candidates for parity
   // `par` (G workgroups of `block` threads).
   hipError_t (*init)(const SwarmDev&, const ProbArgs&, uint32_t par, hipStream_t);

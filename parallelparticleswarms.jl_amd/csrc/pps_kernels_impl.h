@@ -678,19 +678,60 @@ constexpr int WGB = 1024;  // single-workgroup sync loop
 template <int MAXD, bool EXACT, int OBJ>
 struct Launch {
   static constexpr int BLK = block_of(OBJ);  // workgroup of the multi-workgroup kernels
+  // Also compiled for workgroups of 512 and (up to 16 coordinates; at 32 a
+  // 1024-thread workgroup's 128 registers per thread spill) 1024 particles,
+  // the autotuner's choices: the streaming objectives at their exact
+  // dimensions.
+  static constexpr bool MULTI = EXACT && (OBJ == PPS_OBJ_ROSENBROCK ||
+                                          OBJ == PPS_OBJ_RASTRIGIN || OBJ == PPS_OBJ_SPHERE);
+  static constexpr int MAX_BLOCK = !MULTI ? BLK : (MAXD <= 16 ? 1024 : 512);
+  template <int B>
+  static hipError_t init_b(const SwarmDev& S, const ProbArgs& P, uint32_t par, int grid,
+                           hipStream_t st) {
+    hipLaunchKernelGGL((k_init<MAXD, EXACT, OBJ, B>), dim3(grid), dim3(B), 0, st, S, P, par);
+    return hipGetLastError();
+  }
   static hipError_t init(const SwarmDev& S, const ProbArgs& P, uint32_t par, hipStream_t st) {
-    if (S.block == WGB && S.G == 1)
-      hipLaunchKernelGGL((k_init<MAXD, EXACT, OBJ, WGB>), dim3(1), dim3(WGB), 0, st, S, P, par);
-    else
-      hipLaunchKernelGGL((k_init<MAXD, EXACT, OBJ, BLK>), dim3(S.G), dim3(BLK), 0, st, S, P, par);
+    if (S.block == WGB && S.G == 1) return init_b<WGB>(S, P, par, 1, st);
+    if constexpr (MULTI) {
+      if (S.block == 512) return init_b<512>(S, P, par, S.G, st);
+      if constexpr (MAX_BLOCK >= 1024)
+        if (S.block == 1024) return init_b<1024>(S, P, par, S.G, st);
+    }
+    return init_b<BLK>(S, P, par, S.G, st);
+  }
+  template <int B>
+  static hipError_t rekey_b(const SwarmDev& S, const ProbArgs& P, uint32_t par, int32_t gf,
+                            int grid, hipStream_t st) {
+    hipLaunchKernelGGL((k_rekey<MAXD, EXACT, OBJ, B>), dim3(grid), dim3(B), 0, st, S, P, par, gf);
     return hipGetLastError();
   }
   static hipError_t rekey(const SwarmDev& S, const ProbArgs& P, uint32_t par, int32_t gf,
                           hipStream_t st) {
-    if (S.block == WGB && S.G == 1)
-      hipLaunchKernelGGL((k_rekey<MAXD, EXACT, OBJ, WGB>), dim3(1), dim3(WGB), 0, st, S, P, par, gf);
+    if (S.block == WGB && S.G == 1) return rekey_b<WGB>(S, P, par, gf, 1, st);
+    if constexpr (MULTI) {
+      if (S.block == 512) return rekey_b<512>(S, P, par, gf, S.G, st);
+      if constexpr (MAX_BLOCK >= 1024)
+        if (S.block == 1024) return rekey_b<1024>(S, P, par, gf, S.G, st);
+    }
+    return rekey_b<BLK>(S, P, par, gf, S.G, st);
+  }
+  // Multi-workgroup step with B-particle workgroups (64-bit row offsets only
+  // in the objective's default size).
+  template <int B>
+  static hipError_t step_b(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w,
+                           uint32_t k, bool off32, bool nt, hipStream_t st) {
+    if (off32 && !nt)
+      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, B, true, false>), dim3(S.G), dim3(B), 0,
+                         st, S, P, A, w, k);
+    else if (off32)
+      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, B, true, true>), dim3(S.G), dim3(B), 0,
+                         st, S, P, A, w, k);
+    else if constexpr (B == BLK)
+      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, B, false, true>), dim3(S.G), dim3(B), 0,
+                         st, S, P, A, w, k);
     else
-      hipLaunchKernelGGL((k_rekey<MAXD, EXACT, OBJ, BLK>), dim3(S.G), dim3(BLK), 0, st, S, P, par, gf);
+      return hipErrorInvalidValue;
     return hipGetLastError();
   }
   static hipError_t sync_step(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w,
@@ -712,19 +753,17 @@ struct Launch {
         return hipGetLastError();
       }
     }
-    if (S.block == WGB && S.G == 1)
+    if (S.block == WGB && S.G == 1) {
       hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, WGB, true, false>), dim3(1), dim3(WGB), 0,
                          st, S, P, A, w, k);
-    else if (off32 && !nt)
-      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, BLK, true, false>), dim3(S.G), dim3(BLK),
-                         0, st, S, P, A, w, k);
-    else if (off32)
-      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, BLK, true, true>), dim3(S.G), dim3(BLK), 0,
-                         st, S, P, A, w, k);
-    else
-      hipLaunchKernelGGL((k_sync_step<MAXD, EXACT, OBJ, BLK, false, true>), dim3(S.G), dim3(BLK),
-                         0, st, S, P, A, w, k);
-    return hipGetLastError();
+      return hipGetLastError();
+    }
+    if constexpr (MULTI) {
+      if (S.block == 512) return step_b<512>(S, P, A, w, k, off32, nt, st);
+      if constexpr (MAX_BLOCK >= 1024)
+        if (S.block == 1024) return step_b<1024>(S, P, A, w, k, off32, nt, st);
+    }
+    return step_b<BLK>(S, P, A, w, k, off32, nt, st);
   }
   static hipError_t sync_wg(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w0,
                             float wdamp, uint32_t t0, int32_t T, hipStream_t st) {
@@ -751,7 +790,8 @@ struct Launch {
     return hipGetLastError();
   }
   static const KernelSet* set() {
-    static const KernelSet ks = {MAXD, &init, &rekey, &sync_step, &sync_wg, &async_run, &qlock_step};
+    static const KernelSet ks = {MAXD,       MAX_BLOCK,  &init,      &rekey,
+                                 &sync_step, &sync_wg,   &async_run, &qlock_step};
     return &ks;
   }
 };

@@ -222,6 +222,7 @@ void pps_swarm_params_default(pps_swarm_params* p) {
   p->qlock_min_improvers = 2;
   p->grid_per_cu = 0;
   p->sampler = PPS_SAMPLER_UNIFORM;
+  p->particles_per_workgroup = 0;
 }
 
 int pps_ctx_create(int device, pps_ctx** out) {
@@ -285,10 +286,11 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 namespace {
 
-// Autotuned grid choices keyed by (objective, d, ceil(log2 n)).
+// Autotuned (grid_per_cu, particles_per_workgroup) keyed by
+// (objective, d, ceil(log2 n)).
 struct TuneTable {
   std::mutex mu;
-  std::map<std::tuple<int, int, int>, int> best;
+  std::map<std::tuple<int, int, int>, std::pair<int, int>> best;
   bool loaded = false;
 };
 TuneTable& tune_table() {
@@ -307,20 +309,29 @@ void tune_load_locked(TuneTable& t) {
   if (!path) return;
   FILE* f = fopen(path, "r");
   if (!f) return;
-  int obj, d, b, g;
-  while (fscanf(f, "%d %d %d %d", &obj, &d, &b, &g) == 4)
-    if (g > 0 && g <= 64) t.best[std::make_tuple(obj, d, b)] = g;
+  char line[256];
+  while (fgets(line, sizeof line, f)) {
+    int obj, d, b, g, ppw = 256;
+    const int got = sscanf(line, "%d %d %d %d %d", &obj, &d, &b, &g, &ppw);
+    if (got < 4) continue;
+    if (g > 0 && g <= 64 && (ppw == 256 || ppw == 512 || ppw == 1024))
+      t.best[std::make_tuple(obj, d, b)] = std::make_pair(g, ppw);
+  }
   fclose(f);
 }
 
 }  // namespace
 
-extern "C" int pps_tuned_grid_per_cu(int32_t obj, int32_t d, int64_t n) {
+extern "C" int pps_tuned_launch(int32_t obj, int32_t d, int64_t n, int32_t* grid_per_cu,
+                                int32_t* particles_per_workgroup) {
   TuneTable& t = tune_table();
   std::lock_guard<std::mutex> lk(t.mu);
   tune_load_locked(t);
   auto it = t.best.find(std::make_tuple((int)obj, (int)d, log2_bucket(n)));
-  return it == t.best.end() ? 0 : it->second;
+  if (it == t.best.end()) return PPS_E_NOTFOUND;
+  if (grid_per_cu) *grid_per_cu = it->second.first;
+  if (particles_per_workgroup) *particles_per_workgroup = it->second.second;
+  return PPS_OK;
 }
 
 static int swarm_prepare_launch(pps_swarm* s) {
@@ -328,7 +339,16 @@ static int swarm_prepare_launch(pps_swarm* s) {
   const int mode = s->prm.mode;
   int launch = s->prm.launch;
   if (mode == PPS_MODE_SYNC) {
-    if (launch == PPS_LAUNCH_AUTO) launch = n <= 1024 ? PPS_LAUNCH_WORKGROUP : PPS_LAUNCH_GRAPH;
+    // Up to 1024 particles one resident workgroup runs the whole loop.  Above,
+    // a captured graph of the iteration launches saves ~0.3 us of launch cost
+    // per iteration on small swarms (2^12 particles x 10-D: 4.73 vs 5.03 us),
+    // while from 2^19 particles launches issued one by one run the kernels
+    // slightly closer together than the graph's replay (2^20: 37.0 vs 37.2 us,
+    // 2^22: 154.7 vs 156.5 us; the host issues a launch in a few us, far
+    // ahead of the device).
+    if (launch == PPS_LAUNCH_AUTO)
+      launch = n <= 1024 ? PPS_LAUNCH_WORKGROUP
+                         : (n >= ((int64_t)1 << 19) ? PPS_LAUNCH_EAGER : PPS_LAUNCH_GRAPH);
     if (launch == PPS_LAUNCH_WORKGROUP && n > 1024)
       return fail(PPS_E_INVALID, "workgroup launch holds at most 1024 particles (n=%lld)",
                   (long long)n);
@@ -340,7 +360,25 @@ static int swarm_prepare_launch(pps_swarm* s) {
     s->S.G = 1;
     s->S.block = 1024;
   } else {
-    const int blk = block_of(s->P.obj);
+    int blk = block_of(s->P.obj);
+    int tuned_g = 0, tuned_b = 0;
+    if (mode == PPS_MODE_SYNC && (s->prm.grid_per_cu == 0 || s->prm.particles_per_workgroup == 0))
+      pps_tuned_launch(s->P.obj, s->P.d, n, &tuned_g, &tuned_b);
+    if (mode == PPS_MODE_SYNC) {
+      // Workgroups of 512 or 1024 particles where the kernels are compiled
+      // for them and every state array is below 4 GiB (32-bit row offsets).
+      const bool small_arrays = (uint64_t)((n + 63) / 64 * 64) * (uint64_t)s->P.d * 4u < (1ull << 32);
+      const int want = s->prm.particles_per_workgroup > 0 ? s->prm.particles_per_workgroup : tuned_b;
+      if (want > 0 && want != blk) {
+        if (want <= s->ks->max_block && small_arrays)
+          blk = want;
+        else if (s->prm.particles_per_workgroup > 0)
+          return fail(PPS_E_INVALID,
+                      "particles_per_workgroup %d is compiled for Rosenbrock, Rastrigin and "
+                      "Sphere at d in {1,2,3,4,8,10,16} (512 also at d = 32) without "
+                      "constraints, with state arrays below 4 GiB", want);
+      }
+    }
     s->S.block = blk;
     const int64_t tiles = (n + blk - 1) / blk;
     int per_cu = PPS_SYNC_GRID_PER_CU;
@@ -348,9 +386,8 @@ static int swarm_prepare_launch(pps_swarm* s) {
       per_cu = 8;
     } else if (s->prm.grid_per_cu > 0) {
       per_cu = s->prm.grid_per_cu;
-    } else {
-      const int tuned = pps_tuned_grid_per_cu(s->P.obj, s->P.d, n);
-      if (tuned > 0) per_cu = tuned;
+    } else if (tuned_g > 0) {
+      per_cu = tuned_g;
     }
     const int64_t cap = (int64_t)s->ctx->num_cus * per_cu;
     s->S.G = (int32_t)std::max<int64_t>(1, std::min<int64_t>(tiles, cap));
@@ -471,6 +508,10 @@ static int swarm_create(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_p
                 "an ODE objective (objective %d given)", prob->objective);
   if (prm->grid_per_cu < 0 || prm->grid_per_cu > 64)
     return fail(PPS_E_INVALID, "grid_per_cu %d outside 0..64", prm->grid_per_cu);
+  if (prm->particles_per_workgroup != 0 && prm->particles_per_workgroup != 256 &&
+      prm->particles_per_workgroup != 512 && prm->particles_per_workgroup != 1024)
+    return fail(PPS_E_INVALID, "particles_per_workgroup %d is not 0, 256, 512 or 1024",
+                prm->particles_per_workgroup);
   pps_swarm* s = new pps_swarm();
   s->ctx = ctx;
   s->prm = *prm;
@@ -725,51 +766,68 @@ int pps_swarm_free(pps_swarm* s) {
 }
 
 int pps_autotune_sync(pps_ctx* ctx, const pps_problem* prob, int64_t n, int32_t iters,
-                      int32_t* best_grid_per_cu, float* best_us_per_iter) {
+                      int32_t* best_grid_per_cu, int32_t* best_particles_per_workgroup,
+                      float* best_us_per_iter) {
   if (!ctx || !prob) return fail(PPS_E_INVALID, "NULL argument");
   if (n <= 1024) return fail(PPS_E_INVALID, "swarms of at most 1024 particles run in one workgroup");
   if (iters < 1) return fail(PPS_E_INVALID, "iters < 1");
-  static const int cand[] = {1, 2, 3, 4, 6, 8};
-  int best_g = 0;
+  const KernelSet* ks = find_kernels(prob->objective, prob->d, prob->n_cons > 0);
+  if (!ks) return fail(PPS_E_INVALID, "no kernels for objective %d at d=%d", prob->objective, prob->d);
+  static const int grids[] = {1, 2, 3, 4, 6, 8};
+  static const int blocks[] = {256, 512, 1024};
+  const int default_block = block_of(prob->objective);
+  int best_g = 0, best_b = 0;
   float best_us = 0.0f;
-  for (int g : cand) {
-    pps_swarm_params prm;
-    pps_swarm_params_default(&prm);
-    prm.n = n;
-    prm.launch = PPS_LAUNCH_GRAPH;
-    prm.grid_per_cu = g;
-    prm.seed = 0x7E57u;
-    pps_swarm* s = nullptr;
-    int rc = pps_swarm_init(ctx, prob, &prm, &s);
-    if (rc != PPS_OK) return rc;
-    float ms = 0.0f, best_ms = 1e30f;
-    rc = pps_swarm_run(s, iters, 0.7298f, 1.0f);  // warm-up + graph capture
-    for (int rep = 0; rep < 3 && rc == PPS_OK; ++rep) {
-      rc = pps_swarm_run(s, iters, 0.7298f, 1.0f);
-      if (rc == PPS_OK) rc = pps_swarm_elapsed_ms(s, &ms);
-      best_ms = std::min(best_ms, ms);
-    }
-    pps_swarm_free(s);
-    if (rc != PPS_OK) return rc;
-    const float us = best_ms * 1e3f / (float)iters;
-    if (best_g == 0 || us < best_us) {
-      best_g = g;
-      best_us = us;
+  for (int b : blocks) {
+    if (b != default_block && b > ks->max_block) continue;
+    if (b != default_block &&
+        (uint64_t)((n + 63) / 64 * 64) * (uint64_t)prob->d * 4u >= (1ull << 32))
+      continue;
+    for (int g : grids) {
+      pps_swarm_params prm;
+      pps_swarm_params_default(&prm);
+      prm.n = n;
+      prm.launch = PPS_LAUNCH_AUTO;  // as the swarms it is tuned for (n > 1024)
+      prm.grid_per_cu = g;
+      prm.particles_per_workgroup = b;
+      prm.seed = 0x7E57u;
+      pps_swarm* s = nullptr;
+      int rc = pps_swarm_init(ctx, prob, &prm, &s);
+      if (rc != PPS_OK) return rc;
+      float ms = 0.0f, best_ms = 1e30f;
+      // past the first iterations, in which most particles improve and write
+      // their best positions (also captures a graph)
+      rc = pps_swarm_run(s, std::max(iters, 100), 0.7298f, 1.0f);
+      for (int rep = 0; rep < 3 && rc == PPS_OK; ++rep) {
+        rc = pps_swarm_run(s, iters, 0.7298f, 1.0f);
+        if (rc == PPS_OK) rc = pps_swarm_elapsed_ms(s, &ms);
+        best_ms = std::min(best_ms, ms);
+      }
+      pps_swarm_free(s);
+      if (rc != PPS_OK) return rc;
+      const float us = best_ms * 1e3f / (float)iters;
+      if (best_g == 0 || us < best_us) {
+        best_g = g;
+        best_b = b;
+        best_us = us;
+      }
     }
   }
   {
     TuneTable& t = tune_table();
     std::lock_guard<std::mutex> lk(t.mu);
     tune_load_locked(t);
-    t.best[std::make_tuple((int)prob->objective, (int)prob->d, log2_bucket(n))] = best_g;
+    t.best[std::make_tuple((int)prob->objective, (int)prob->d, log2_bucket(n))] =
+        std::make_pair(best_g, best_b);
     if (const char* path = getenv("PPS_TUNE_CACHE")) {
       if (FILE* f = fopen(path, "a")) {
-        fprintf(f, "%d %d %d %d\n", prob->objective, prob->d, log2_bucket(n), best_g);
+        fprintf(f, "%d %d %d %d %d\n", prob->objective, prob->d, log2_bucket(n), best_g, best_b);
         fclose(f);
       }
     }
   }
   if (best_grid_per_cu) *best_grid_per_cu = best_g;
+  if (best_particles_per_workgroup) *best_particles_per_workgroup = best_b;
   if (best_us_per_iter) *best_us_per_iter = best_us;
   return PPS_OK;
 }

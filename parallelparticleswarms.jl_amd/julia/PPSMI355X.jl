@@ -62,6 +62,7 @@ struct CSwarmParams        # pps_swarm_params
     qlock_min_improvers::Int32
     grid_per_cu::Int32
     sampler::Int32
+    particles_per_workgroup::Int32   # 0 = autotuned or 256
 end
 
 struct CLbfgsParams        # pps_lbfgs_params
@@ -260,7 +261,7 @@ function init(prob::OptimizationProblem,
         sampler = ParallelParticleSwarms.GPUUniformSampler(), kwargs...)
     c, arrays = cproblem(prob)
     prm = CSwarmParams(alg.num_particles, mode_of(alg), PPS_GBEST_PBEST, Int32(0), Int32(0),
-        UInt64(seed), 1.4962f0, 1.4962f0, Int32(2), Int32(0), sampler_code(sampler))
+        UInt64(seed), 1.4962f0, 1.4962f0, Int32(2), Int32(0), sampler_code(sampler), Int32(0))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     GC.@preserve arrays begin
         if host_sampled(prob, sampler)
@@ -363,7 +364,7 @@ function parameter_estim_lv(data::AbstractMatrix, lb, ub; backend = MI355XBacken
         Ptr{Float32}(C_NULL), pointer(obs), Int32(size(data, 2)), Ptr{Float32}(C_NULL),
         Int32(0), pointer(z))
     prm = CSwarmParams(num_particles, PPS_MODE_SYNC, PPS_GBEST_PBEST, Int32(1), Int32(0),
-        UInt64(seed), 1.4962f0, 1.4962f0, Int32(2), Int32(0), PPS_SAMPLER_UNIFORM)
+        UInt64(seed), 1.4962f0, 1.4962f0, Int32(2), Int32(0), PPS_SAMPLER_UNIFORM, Int32(0))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     GC.@preserve l u par obs z begin
         check(ccall((:pps_swarm_init, libpps), Cint,

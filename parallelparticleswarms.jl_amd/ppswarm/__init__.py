@@ -10,7 +10,7 @@ from .api import (  # noqa: F401
     ArmijoBacktracking, Himmelblau, HybridPSO, HybridPSOCache, LatinHypercubeSample, MI355XBackend,
     Objective, OptimizationProblem, OptimizationSolution, ParallelPSOKernel, QuadraticConstraints,
     ParallelSyncPSOKernel, PSOCache, Quadratic, Rastrigin, Rosen2, Rosenbrock, SimpleLBFGS,
-    Sphere, StrongWolfeLineSearch, Swarm, autotune_sync, context, eval_f64, init, lotka_volterra_sse, fitzhugh_nagumo_sse, ode_sse, reinit_, seed_, solve, solve_,
+    Sphere, StrongWolfeLineSearch, Swarm, autotune_sync, tuned_launch, context, eval_f64, init, lotka_volterra_sse, fitzhugh_nagumo_sse, ode_sse, reinit_, seed_, solve, solve_,
     swarm_params)
 from .serve import Request, Result, Server, batch_solve, solve_alone  # noqa: F401
 from .workloads import served_mix  # noqa: F401

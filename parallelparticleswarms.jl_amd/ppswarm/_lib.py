@@ -62,7 +62,7 @@ class pps_swarm_params(C.Structure):
         ("n", C.c_int64), ("mode", C.c_int32), ("gbest_from", C.c_int32), ("clamp", C.c_int32),
         ("launch", C.c_int32), ("seed", C.c_uint64), ("c1", C.c_float), ("c2", C.c_float),
         ("qlock_min_improvers", C.c_int32), ("grid_per_cu", C.c_int32),
-        ("sampler", C.c_int32),
+        ("sampler", C.c_int32), ("particles_per_workgroup", C.c_int32),
     ]
 
 
@@ -132,8 +132,9 @@ def lib():
     _sig(L, "pps_eval_f64", [C.POINTER(pps_problem), C.c_int64, dp, dp])
     _sig(L, "pps_lbfgs_params_default", [C.POINTER(pps_lbfgs_params)], None)
     _sig(L, "pps_autotune_sync", [vp, C.POINTER(pps_problem), C.c_int64, C.c_int32,
-                                  C.POINTER(C.c_int32), fp])
-    _sig(L, "pps_tuned_grid_per_cu", [C.c_int32, C.c_int32, C.c_int64])
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32), fp])
+    _sig(L, "pps_tuned_launch", [C.c_int32, C.c_int32, C.c_int64, C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32)])
     _sig(L, "pps_swarm_lbfgs", [vp, C.POINTER(pps_lbfgs_params), fp, fp, fp, fp, i64p])
     _sig(L, "pps_swarm_lbfgs_noalloc", [vp, C.POINTER(pps_lbfgs_params), fp, fp, fp, fp, i64p])
     _sig(L, "pps_server_create", [vp, C.POINTER(pps_server_config), C.POINTER(vp)])

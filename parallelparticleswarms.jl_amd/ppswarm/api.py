@@ -592,12 +592,14 @@ def swarm_params(n: int, mode: int = L.MODE_SYNC, *, seed: int = 0,
                  gbest_from: int = L.GBEST_PBEST, clamp: int = L.CLAMP_COMPONENT,
                  launch: int = L.LAUNCH_AUTO, qlock_min_improvers: int = 2,
                  c1: float = 1.4962, c2: float = 1.4962,
-                 grid_per_cu: int = 0, sampler: int = L.SAMPLER_UNIFORM) -> L.pps_swarm_params:
+                 grid_per_cu: int = 0, sampler: int = L.SAMPLER_UNIFORM,
+                 particles_per_workgroup: int = 0) -> L.pps_swarm_params:
     p = L.pps_swarm_params()
     L.lib().pps_swarm_params_default(C.byref(p))
     p.n, p.mode, p.seed = int(n), int(mode), int(seed) & 0xFFFFFFFFFFFFFFFF
     p.grid_per_cu = int(grid_per_cu)
     p.sampler = int(sampler)
+    p.particles_per_workgroup = int(particles_per_workgroup)
     p.gbest_from, p.clamp, p.launch = int(gbest_from), int(clamp), int(launch)
     p.qlock_min_improvers = int(qlock_min_improvers)
     p.c1, p.c2 = float(c1), float(c2)
@@ -606,14 +608,29 @@ def swarm_params(n: int, mode: int = L.MODE_SYNC, *, seed: int = 0,
 
 def autotune_sync(prob: OptimizationProblem, n: int, iters: int = 20,
                   backend: MI355XBackend = MI355XBackend()):
-    """Time the sync step's grid sizes for this (objective, d, n) and remember
-    the fastest for later swarms; returns (grid_per_cu, us_per_iteration)."""
+    """Time the sync step's workgroup sizes (particles per workgroup) and grid
+    sizes (tiles per thread) for this (objective, d, n) and remember the
+    fastest pair for later swarms; returns (grid_per_cu,
+    particles_per_workgroup, us_per_iteration)."""
     pr, keep = prob.c()
     g = C.c_int32()
+    b = C.c_int32()
     us = C.c_float()
     L.check(L.lib().pps_autotune_sync(context(backend).h, C.byref(pr), int(n), int(iters),
-                                      C.byref(g), C.byref(us)))
-    return int(g.value), float(us.value)
+                                      C.byref(g), C.byref(b), C.byref(us)))
+    return int(g.value), int(b.value), float(us.value)
+
+
+def tuned_launch(objective: Objective, d: int, n: int):
+    """The cached (grid_per_cu, particles_per_workgroup) for (objective, d,
+    n), or None."""
+    g = C.c_int32()
+    b = C.c_int32()
+    rc = L.lib().pps_tuned_launch(int(objective.code), int(d), int(n), C.byref(g), C.byref(b))
+    if rc == L.E_NOTFOUND:
+        return None
+    L.check(rc)
+    return int(g.value), int(b.value)
 
 
 # ---------------------------------------------------------------------------

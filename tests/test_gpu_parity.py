@@ -192,36 +192,80 @@ def test_shifted_objective_parity():
     assert_same_gbest(g, o)
 
 
-@pytest.mark.parametrize("grid_per_cu", [1, 3, 8])
-def test_grid_size_does_not_change_results(grid_per_cu):
-    # the autotuner may pick any grid: results must not depend on it
-    note(f"grid_per_cu={grid_per_cu}")
+TUNED = [(O.RASTRIGIN, 10, 300000, ppw, g) for ppw in (256, 512, 1024) for g in (1, 3, 8)] + [
+    (O.ROSENBROCK, 32, 50000, 512, 2), (O.ROSENBROCK, 32, 50000, 256, 6),
+    (O.SPHERE, 3, 70000, 1024, 4), (O.ROSENBROCK, 16, 40000, 1024, 2),
+    (O.RASTRIGIN, 2, 200000, 512, 8)]
+
+
+@pytest.mark.parametrize("obj,d,n,ppw,grid_per_cu", TUNED)
+def test_tuned_launch_does_not_change_results(obj, d, n, ppw, grid_per_cu):
+    # every choice the autotuner can make (particles per workgroup, and the
+    # grid, which sets the tiles each thread steps) gives the same swarm
+    note(f"tuned launch obj={obj} d={d} n={n} ppw={ppw} grid={grid_per_cu}")
     import ppswarm as P
 
-    d, n = 10, 300000
     lb, ub = np.full(d, -5.12, np.float32), np.full(d, 5.12, np.float32)
-    prob = P.OptimizationProblem(P.Rastrigin(), np.zeros(d, np.float32), lb=lb, ub=ub)
+    f = {O.RASTRIGIN: P.Rastrigin, O.ROSENBROCK: P.Rosenbrock, O.SPHERE: P.Sphere}[obj]()
+    par = [1.0, 100.0] if obj == O.ROSENBROCK else None
+    prob = P.OptimizationProblem(f, np.zeros(d, np.float32), par, lb=lb, ub=ub)
     g = P.Swarm(P.context(), prob, P.swarm_params(n, L.MODE_SYNC, seed=9, launch=L.LAUNCH_GRAPH,
-                                                  grid_per_cu=grid_per_cu))
-    o = O.Swarm(O.Problem(O.RASTRIGIN, d, lb, ub), n, 9)
-    g.run(6)
-    o.run_sync(6)
+                                                  grid_per_cu=grid_per_cu,
+                                                  particles_per_workgroup=ppw))
+    o = O.Swarm(O.Problem(obj, d, lb, ub, np.asarray(par or [], np.float32)), n, 9)
+    assert_same_state(g, o, "init")
+    g.run(6, 0.7298, 0.99)
+    o.run_sync(6, 0.7298, 0.99)
     assert_same_state(g, o)
     assert_same_gbest(g, o)
+    g.reinit()
+    o.reinit()
+    g.run(3)
+    o.run_sync(3)
+    assert_same_state(g, o, "after reinit")
+    assert_same_gbest(g, o, "after reinit")
 
 
-def test_autotune_records_a_choice():
+def test_workgroup_sizes_outside_the_compiled_set_are_refused():
+    note("particles_per_workgroup refusals")
+    import ppswarm as P
+
+    def prob(f, d, par=None):
+        return P.OptimizationProblem(f, np.zeros(d, np.float32), par, lb=np.full(d, -1.0),
+                                     ub=np.full(d, 1.0))
+    for pr, ppw in ((prob(P.Rastrigin(), 10), 384),        # not a compiled size
+                    (prob(P.Rosenbrock(), 32, [1, 100]), 1024),  # d = 32: up to 512
+                    (prob(P.Rastrigin(), 5), 512),          # runtime-dimension kernels
+                    (prob(P.Beale(), 2), 512)):             # two-dimensional test functions
+        with pytest.raises(P.PPSError):
+            P.Swarm(P.context(), pr, P.swarm_params(5000, L.MODE_SYNC, seed=1,
+                                                    launch=L.LAUNCH_GRAPH,
+                                                    particles_per_workgroup=ppw))
+
+
+def test_autotune_records_a_choice(tmp_path, monkeypatch):
     note("autotune")
     import ppswarm as P
-    from ppswarm import _lib as LL
 
     d, n = 8, 1 << 16
     prob = P.OptimizationProblem(P.Sphere(), np.zeros(d, np.float32), lb=np.full(d, -1.0),
                                  ub=np.full(d, 1.0))
-    g, us = P.autotune_sync(prob, n, iters=5)
-    assert g in (1, 2, 3, 4, 6, 8) and us > 0
-    assert LL.lib().pps_tuned_grid_per_cu(LL.OBJ_SPHERE, d, n) == g
-    assert LL.lib().pps_tuned_grid_per_cu(LL.OBJ_SPHERE, d, 4 * n) == 0
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("PPS_TUNE_CACHE", str(cache))
+    g, b, us = P.autotune_sync(prob, n, iters=5)
+    assert g in (1, 2, 3, 4, 6, 8) and b in (256, 512, 1024) and us > 0
+    assert P.tuned_launch(P.Sphere(), d, n) == (g, b)
+    assert P.tuned_launch(P.Sphere(), d, 4 * n) is None
+    # the choice is appended to the cache file: objective d ceil(log2 n) grid block
+    line = cache.read_text().split()
+    assert [int(v) for v in line] == [L.OBJ_SPHERE, d, 16, g, b]
+    # a later swarm of that (objective, d, size bucket) runs with it, same results
+    sw = P.Swarm(P.context(), prob, P.swarm_params(n, L.MODE_SYNC, seed=4, launch=L.LAUNCH_GRAPH))
+    o = O.Swarm(O.Problem(O.SPHERE, d, np.full(d, -1.0, np.float32), np.full(d, 1.0, np.float32)),
+                n, 4)
+    sw.run(4)
+    o.run_sync(4)
+    assert_same_state(sw, o)
 
 
 @pytest.mark.parametrize("n,launch", [(5000, L.LAUNCH_GRAPH), (1000, L.LAUNCH_WORKGROUP)])
