@@ -66,6 +66,15 @@ T = 1000
 BYTES_PER_PARTICLE_ITER = 4 * (5 * D + 1)  # read x,v,best (3d) + best_cost, write x,v (2d)
 
 
+# Set by --eager: every multi-workgroup sync swarm of every leg launches its
+# iterations one by one (the same kernels; used for profiler runs).
+EAGER = False
+
+
+def _launch(L, default):
+    return L.LAUNCH_EAGER if EAGER else default
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -195,7 +204,8 @@ def hybrid_c2(P, L, ctx):
     d, n = 16, 1 << 16
     prob = P.OptimizationProblem(P.Rastrigin(), np.zeros(d, np.float32), lb=np.full(d, -5.12),
                                  ub=np.full(d, 5.12))
-    sw = P.Swarm(ctx, prob, P.swarm_params(n, L.MODE_SYNC, seed=16, launch=L.LAUNCH_GRAPH))
+    sw = P.Swarm(ctx, prob, P.swarm_params(n, L.MODE_SYNC, seed=16,
+                                           launch=_launch(L, L.LAUNCH_GRAPH)))
     sw.run(5)  # warm-up (graph capture)
     sw.reinit()
     t0 = time.perf_counter()
@@ -252,11 +262,12 @@ def _ode_leg(run, particles, iters, extra=None, compare_full=False):
 
 def ode_c3(P, L, clock_khz):
     import json as _json
+    lk = {"launch": _launch(L, L.LAUNCH_AUTO)}
     doc = _json.load(open(os.path.join(ROOT, "tests", "golden", "lv_obs.json")))
     obs = np.array(doc["obs"], np.float32)
-    P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 1 << 14, 5, seed=1)  # warm-up
+    P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 1 << 14, 5, seed=1, **lk)  # warm-up
     t0 = time.perf_counter()
-    sol, sw = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 1 << 14, 300, seed=3)
+    sol, sw = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 1 << 14, 300, seed=3, **lk)
     wall = time.perf_counter() - t0
     ms = sw.elapsed_ms()
     sw.free()
@@ -272,10 +283,10 @@ def ode_c3(P, L, clock_khz):
     # global-best hand-over included; the full swarm's time per iteration is
     # compared with it.
     _, sw1 = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 64, 300, seed=3,
-                                   launch=L.LAUNCH_GRAPH)
+                                   launch=_launch(L, L.LAUNCH_GRAPH))
     sw1.free()
     _, sw1 = P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 64, 300, seed=3,
-                                   launch=L.LAUNCH_GRAPH)
+                                   launch=_launch(L, L.LAUNCH_GRAPH))
     floor_us = sw1.elapsed_ms() * 1e3 / 300
     sw1.free()
     it_us = ms * 1e3 / 300
@@ -288,7 +299,7 @@ def ode_c3(P, L, clock_khz):
     # adaptive steps (abstol 1e-6, reltol 1e-3, initial dt 0.01), same shape
     for key, adaptive in (("tsit5", False), ("tsit5_adaptive", True)):
         out[key] = _ode_leg(
-            lambda: P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 1 << 14, 300, seed=3,
+            lambda: P.parameter_estim_ode(obs, np.zeros(4), np.full(4, 5.0), 1 << 14, 300, seed=3, **lk,
                                           ode_alg="tsit5", adaptive=adaptive), 1 << 14, 300,
             compare_full=adaptive)
     # the reference's own Lotka-Volterra benchmark shape (10000 particles, 100
@@ -297,10 +308,10 @@ def ode_c3(P, L, clock_khz):
     doc = _json.load(open(os.path.join(ROOT, "tests", "golden", "lv_obs_benchmark.json")))
     obs_b = np.array(doc["obs"], np.float32)
     theta_true = np.asarray(doc["theta"])
-    P.parameter_estim_ode(obs_b, np.zeros(4), np.full(4, 10.0), 10000, 5, seed=1, ode_alg="tsit5",
+    P.parameter_estim_ode(obs_b, np.zeros(4), np.full(4, 10.0), 10000, 5, seed=1, **lk, ode_alg="tsit5",
                           dt=0.1, saveat=0.01)  # warm-up
     out["reference_benchmark_shape"] = _ode_leg(
-        lambda: P.parameter_estim_ode(obs_b, np.zeros(4), np.full(4, 10.0), 10000, 100, seed=1,
+        lambda: P.parameter_estim_ode(obs_b, np.zeros(4), np.full(4, 10.0), 10000, 100, seed=1, **lk,
                                       ode_alg="tsit5", dt=0.1, saveat=0.01), 10000, 100,
         lambda sol: {"max_rel_err_theta": float(np.max(np.abs(np.asarray(sol.u) / theta_true - 1))),
                      "workload": "10000 particles x 100 iterations, Tsit5 dt 0.1, "
@@ -310,9 +321,9 @@ def ode_c3(P, L, clock_khz):
     doc = _json.load(open(os.path.join(ROOT, "tests", "golden", "fhn_obs_benchmark.json")))
     obs_f = np.array(doc["obs"], np.float32)
     kw = dict(ode_alg="tsit5", dt=0.1, saveat=0.01, model="fitzhugh_nagumo")
-    P.parameter_estim_ode(obs_f, np.zeros(4), np.full(4, 5.0), 10000, 5, seed=0, **kw)  # warm-up
+    P.parameter_estim_ode(obs_f, np.zeros(4), np.full(4, 5.0), 10000, 5, seed=0, **lk, **kw)  # warm-up
     out["fitzhugh_nagumo_reference_benchmark_shape"] = _ode_leg(
-        lambda: P.parameter_estim_ode(obs_f, np.zeros(4), np.full(4, 5.0), 10000, 100, seed=0,
+        lambda: P.parameter_estim_ode(obs_f, np.zeros(4), np.full(4, 5.0), 10000, 100, seed=0, **lk,
                                       **kw), 10000, 100,
         lambda sol: {"workload": "10000 particles x 100 iterations, Tsit5 dt 0.1, "
                                  "301 samples every 0.01"})
@@ -406,8 +417,9 @@ def main():
                     help="PARTICLES_PER_WORKGROUP/GRID_PER_CU of the headline swarms instead of "
                          "the autotuner's choice (e.g. 512/2)")
     ap.add_argument("--eager", action="store_true",
-                    help="force one launch per iteration for the headline swarms (the "
-                         "library's own choice at 2^20 particles)")
+                    help="force one launch per iteration for every multi-workgroup sync swarm "
+                         "of every leg (the library's own choice for the headline's 2^20 "
+                         "particles; the same kernels everywhere)")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--extras", default="hbm,async,qlock,c0,c4,hybrid,ode",
                     help="comma list of the legs to run besides the headline")
@@ -417,6 +429,8 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="also time the sync step at 2^10..2^24 particles (adds ~1 min)")
     args = ap.parse_args()
+    global EAGER
+    EAGER = args.eager
     legs = set() if args.no_extras else {x.strip() for x in args.extras.split(",") if x.strip()}
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): "
