@@ -199,9 +199,11 @@ typedef struct pps_swarm_params {
                                 1024 (Rosenbrock, Rastrigin and Sphere at
                                 d in {1,2,3,4,8,10,16}, 256 or 512 at d = 32,
                                 without constraints, state arrays below
-                                4 GiB), 256 otherwise; 0 =
+                                4 GiB), 256 otherwise (64 for the ODE
+                                objectives, their only size); 0 =
                                 autotuned value if one is cached for
-                                (objective, d, N), else 256.  With G
+                                (objective, d, N), else the objective's own
+                                size.  With G
                                 workgroups each thread steps ceil(tiles / G)
                                 tiles of that many particles per iteration.
                                 Results do not depend on it.               */
@@ -361,7 +363,9 @@ int pps_swarm_lbfgs_noalloc(pps_swarm* s, const pps_lbfgs_params* lp, float* out
  * process, and append the line
  *     objective d ceil(log2 n) grid_per_cu particles_per_workgroup
  * to the file named by $PPS_TUNE_CACHE if set (read back at the first lookup;
- * lines without the last field mean 256).  Output pointers may be NULL. */
+ * lines without the last field mean 256).  The ODE objectives keep their
+ * 64-particle workgroups; only the grid is tuned for them.  Output pointers
+ * may be NULL. */
 int pps_autotune_sync(pps_ctx* ctx, const pps_problem* prob, int64_t n,
                       int32_t iters, int32_t* best_grid_per_cu,
                       int32_t* best_particles_per_workgroup,

@@ -314,7 +314,7 @@ void tune_load_locked(TuneTable& t) {
     int obj, d, b, g, ppw = 256;
     const int got = sscanf(line, "%d %d %d %d %d", &obj, &d, &b, &g, &ppw);
     if (got < 4) continue;
-    if (g > 0 && g <= 64 && (ppw == 256 || ppw == 512 || ppw == 1024))
+    if (g > 0 && g <= 64 && (ppw == 64 || ppw == 256 || ppw == 512 || ppw == 1024))
       t.best[std::make_tuple(obj, d, b)] = std::make_pair(g, ppw);
   }
   fclose(f);
@@ -508,10 +508,15 @@ static int swarm_create(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_p
                 "an ODE objective (objective %d given)", prob->objective);
   if (prm->grid_per_cu < 0 || prm->grid_per_cu > 64)
     return fail(PPS_E_INVALID, "grid_per_cu %d outside 0..64", prm->grid_per_cu);
-  if (prm->particles_per_workgroup != 0 && prm->particles_per_workgroup != 256 &&
-      prm->particles_per_workgroup != 512 && prm->particles_per_workgroup != 1024)
-    return fail(PPS_E_INVALID, "particles_per_workgroup %d is not 0, 256, 512 or 1024",
-                prm->particles_per_workgroup);
+  // 0, the objective's own workgroup size (64 particles for the ODE
+  // objectives, 256 otherwise) or one of the wider ones
+  const int own_block = prob ? block_of(prob->objective) : 256;
+  if (prm->particles_per_workgroup != 0 && prm->particles_per_workgroup != own_block &&
+      prm->particles_per_workgroup != 256 && prm->particles_per_workgroup != 512 &&
+      prm->particles_per_workgroup != 1024)
+    return fail(PPS_E_INVALID,
+                "particles_per_workgroup %d is not 0, %d (this objective's own), 256, 512 or 1024",
+                prm->particles_per_workgroup, own_block);
   pps_swarm* s = new pps_swarm();
   s->ctx = ctx;
   s->prm = *prm;
@@ -774,15 +779,16 @@ int pps_autotune_sync(pps_ctx* ctx, const pps_problem* prob, int64_t n, int32_t 
   const KernelSet* ks = find_kernels(prob->objective, prob->d, prob->n_cons > 0);
   if (!ks) return fail(PPS_E_INVALID, "no kernels for objective %d at d=%d", prob->objective, prob->d);
   static const int grids[] = {1, 2, 3, 4, 6, 8};
-  static const int blocks[] = {256, 512, 1024};
+  // the objective's own workgroup size first (64 particles for the ODE
+  // objectives, 256 otherwise), then the wider ones its kernels are compiled for
   const int default_block = block_of(prob->objective);
+  const int blocks[] = {default_block, 512, 1024};
+  const bool small_arrays = (uint64_t)((n + 63) / 64 * 64) * (uint64_t)prob->d * 4u < (1ull << 32);
   int best_g = 0, best_b = 0;
   float best_us = 0.0f;
-  for (int b : blocks) {
-    if (b != default_block && b > ks->max_block) continue;
-    if (b != default_block &&
-        (uint64_t)((n + 63) / 64 * 64) * (uint64_t)prob->d * 4u >= (1ull << 32))
-      continue;
+  for (int bi = 0; bi < 3; ++bi) {
+    const int b = blocks[bi];
+    if (bi > 0 && (b <= default_block || b > ks->max_block || !small_arrays)) continue;
     for (int g : grids) {
       pps_swarm_params prm;
       pps_swarm_params_default(&prm);
@@ -813,6 +819,7 @@ int pps_autotune_sync(pps_ctx* ctx, const pps_problem* prob, int64_t n, int32_t 
       }
     }
   }
+  if (best_g == 0) return fail(PPS_E_STATE, "no launch shape was timed");
   {
     TuneTable& t = tune_table();
     std::lock_guard<std::mutex> lk(t.mu);

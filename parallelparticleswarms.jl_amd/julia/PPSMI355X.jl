@@ -62,7 +62,7 @@ struct CSwarmParams        # pps_swarm_params
     qlock_min_improvers::Int32
     grid_per_cu::Int32
     sampler::Int32
-    particles_per_workgroup::Int32   # 0 = autotuned or 256
+    particles_per_workgroup::Int32   # 0 = autotuned or the objective's own (256; 64 for ODE objectives)
 end
 
 struct CLbfgsParams        # pps_lbfgs_params

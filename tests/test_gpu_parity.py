@@ -234,6 +234,7 @@ def test_workgroup_sizes_outside_the_compiled_set_are_refused():
         return P.OptimizationProblem(f, np.zeros(d, np.float32), par, lb=np.full(d, -1.0),
                                      ub=np.full(d, 1.0))
     for pr, ppw in ((prob(P.Rastrigin(), 10), 384),        # not a compiled size
+                    (prob(P.Rastrigin(), 10), 64),         # the ODE objectives' size
                     (prob(P.Rosenbrock(), 32, [1, 100]), 1024),  # d = 32: up to 512
                     (prob(P.Rastrigin(), 5), 512),          # runtime-dimension kernels
                     (prob(P.Beale(), 2), 512)):             # two-dimensional test functions
@@ -266,6 +267,22 @@ def test_autotune_records_a_choice(tmp_path, monkeypatch):
     sw.run(4)
     o.run_sync(4)
     assert_same_state(sw, o)
+
+
+def test_autotune_ode_objective_keeps_its_workgroups():
+    # the ODE objectives run 64-particle workgroups (two lanes per particle);
+    # only their grid is tuned
+    note("autotune ode")
+    import json
+    import os
+    import ppswarm as P
+
+    doc = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "lv_obs.json")))
+    f = P.lotka_volterra_sse(np.array(doc["obs"], np.float32))
+    prob = P.OptimizationProblem(f, np.zeros(4, np.float32), lb=np.zeros(4), ub=np.full(4, 5.0))
+    g, b, us = P.autotune_sync(prob, 1 << 14, iters=3)
+    assert b == 64 and g in (1, 2, 3, 4, 6, 8) and us > 0
+    assert P.tuned_launch(f, 4, 1 << 14) == (g, b)
 
 
 @pytest.mark.parametrize("n,launch", [(5000, L.LAUNCH_GRAPH), (1000, L.LAUNCH_WORKGROUP)])
