@@ -241,6 +241,9 @@ __device__ uint64_t part_step_sweep(const Swarm& S, const ProbLds* L, int64_t fi
     const int64_t i = tile * PTILE + threadIdx.x;
     if (i >= S.n) continue;
     const uint32_t i4 = (uint32_t)i * 4u, row4 = (uint32_t)S.np * 4u;
+    // the best cost goes out with the first round of loads rather than after
+    // the last (one dependent round trip less per tile)
+    float pc = S.PC[i];
     Accum<OBJ> acc;
     acc.start(d);
     for (int g = 0; g < d; g += PCOORD) {
@@ -280,7 +283,6 @@ __device__ uint64_t part_step_sweep(const Swarm& S, const ProbLds* L, int64_t fi
         }
       }
     }
-    float pc = S.PC[i];
     if (acc.res < pc) {
       for (int j = 0; j < d; ++j) at32(S.PB, i4 + (uint32_t)j * row4) = at32(S.X, i4 + (uint32_t)j * row4);
       pc = acc.res;
@@ -367,14 +369,13 @@ __device__ bool run_small(BatchProb* B, uint64_t deadline, uint64_t* s_red, Prob
   return done >= steps;
 }
 
-// Global best of a large problem after iteration `t`: minimum of its part keys
-// of parity t & 1; the winning part is the one owning the winner's tile
-// (tiles of PTILE particles dealt round robin to the k parts).
-__device__ __forceinline__ uint64_t part_gbest(const BatchXchg* XC, int k, uint32_t t, int d,
-                                               float* dst, uint64_t* s_red) {
-  const uint32_t par = t & 1u;
-  uint64_t key = KEY_NONE;
-  for (int q = threadIdx.x; q < k; q += BATCH_BLOCK) key = umin64(key, XC->key[par][q]);
+// Global best of a large problem from this thread's share `key` of its part
+// keys of parity `par`: the block minimum, and the winner's position (the
+// winning part is the one owning the winner's tile; tiles of PTILE particles
+// are dealt round robin to the k parts) copied to dst.
+__device__ __forceinline__ uint64_t part_gbest_from(const BatchXchg* XC, uint64_t key, int k,
+                                                    uint32_t par, int d, float* dst,
+                                                    uint64_t* s_red) {
   key = block_min<BATCH_BLOCK>(key, s_red);
   const int64_t wi = (int64_t)(uint32_t)key;
   const int wpart = (int)((wi / PTILE) % k);
@@ -382,18 +383,33 @@ __device__ __forceinline__ uint64_t part_gbest(const BatchXchg* XC, int k, uint3
   __syncthreads();
   return key;
 }
+// This thread's share of the part keys of parity `par`.
+__device__ __forceinline__ uint64_t part_keys(const BatchXchg* XC, int k, uint32_t par) {
+  uint64_t key = KEY_NONE;
+  for (int q = threadIdx.x; q < k; q += BATCH_BLOCK) key = umin64(key, XC->key[par][q]);
+  return key;
+}
+// Global best of a large problem after iteration `t`: minimum of its part keys
+// of parity t & 1.
+__device__ __forceinline__ uint64_t part_gbest(const BatchXchg* XC, int k, uint32_t t, int d,
+                                               float* dst, uint64_t* s_red) {
+  return part_gbest_from(XC, part_keys(XC, k, t & 1u), k, t & 1u, d, dst, s_red);
+}
 
 // PART item: iteration t (t = 0: init) of part q of k.
 template <int OBJ>
 __device__ void run_part(BatchProb* B, const BatchItem& it, BatchXchg* XC, uint64_t* s_red,
                          ProbLds* L) {
+  // the part keys of the previous iteration are requested with the
+  // descriptor (both need only the work item)
+  const uint64_t keys = it.t == 0 ? KEY_NONE : part_keys(XC, it.nparts, (it.t - 1u) & 1u);
   const Swarm S = load_swarm(B, L);
   __syncthreads();
   uint64_t mykey;
   if (it.t == 0) {
     mykey = init_sweep<OBJ, 1>(S, L, it.part, it.nparts);
   } else {
-    part_gbest(XC, it.nparts, it.t - 1u, S.d, L->gb, s_red);
+    part_gbest_from(XC, keys, it.nparts, (it.t - 1u) & 1u, S.d, L->gb, s_red);
     mykey = part_step_sweep<OBJ>(S, L, it.part, it.nparts, it.t, it.w);
   }
   const uint64_t bk = block_min<BATCH_BLOCK>(mykey, s_red);
