@@ -354,7 +354,7 @@ int pps_swarm_lbfgs_noalloc(pps_swarm* s, const pps_lbfgs_params* lp, float* out
 
 /* ---- autotuning ------------------------------------------------------- */
 /* Time the multi-workgroup sync step on a scratch swarm of `n` particles
- * (after max(iters, 100) untimed iterations, `iters` timed ones, best of
+ * (after max(iters, 1000) untimed iterations, `iters` timed ones, best of
  * three) for every particles_per_workgroup
  * the kernels are compiled for (256, 512, 1024; 256 only for other objectives
  * and dimensions) and grid_per_cu in {1, 2, 3, 4, 6, 8} (which sets the tiles

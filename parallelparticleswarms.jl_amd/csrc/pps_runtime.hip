@@ -801,9 +801,13 @@ int pps_autotune_sync(pps_ctx* ctx, const pps_problem* prob, int64_t n, int32_t 
       int rc = pps_swarm_init(ctx, prob, &prm, &s);
       if (rc != PPS_OK) return rc;
       float ms = 0.0f, best_ms = 1e30f;
-      // past the first iterations, in which most particles improve and write
-      // their best positions (also captures a graph)
-      rc = pps_swarm_run(s, std::max(iters, 100), 0.7298f, 1.0f);
+      // Timed past the first thousand iterations, where a solve spends most of
+      // its time: early on most particles improve and write their best
+      // positions, and the fastest shape there is not always the fastest
+      // later (2^20 x 10-D: 4 workgroups per CU lead 2 per CU by 3 us at
+      // iteration 100 and trail or tie them from iteration 1000).  Also
+      // captures a graph where the launch mode uses one.
+      rc = pps_swarm_run(s, std::max(iters, 1000), 0.7298f, 1.0f);
       for (int rep = 0; rep < 3 && rc == PPS_OK; ++rep) {
         rc = pps_swarm_run(s, iters, 0.7298f, 1.0f);
         if (rc == PPS_OK) rc = pps_swarm_elapsed_ms(s, &ms);
