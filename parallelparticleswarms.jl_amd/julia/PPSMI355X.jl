@@ -255,10 +255,23 @@ function read_gbest(h, d)
     return pos, cost[], idx[]
 end
 
+# The kernels apply the reference's default penalty schedules θ_default,
+# γ_default and h = sqrt (src/utils.jl:320-372, src/algorithms.jl:192-203); they
+# only matter for constrained problems, where custom ones are refused.
+function check_schedules(prob, alg)
+    prob.f.cons === nothing && return nothing
+    (alg.θ === ParallelParticleSwarms.θ_default && alg.γ === ParallelParticleSwarms.γ_default &&
+     alg.h === sqrt) || throw(ArgumentError(
+        "the MI355X backend applies the default penalty schedules (θ_default, γ_default, " *
+        "h = sqrt); custom θ, γ or h with constraints are not supported"))
+    return nothing
+end
+
 function init(prob::OptimizationProblem,
         alg::Union{ParallelSyncPSOKernel{<:MI355XBackend}, ParallelPSOKernel{<:MI355XBackend}},
         args...; seed::Integer = rand(UInt64),
         sampler = ParallelParticleSwarms.GPUUniformSampler(), kwargs...)
+    check_schedules(prob, alg)
     c, arrays = cproblem(prob)
     prm = CSwarmParams(alg.num_particles, mode_of(alg), PPS_GBEST_PBEST, Int32(0), Int32(0),
         UInt64(seed), 1.4962f0, 1.4962f0, Int32(2), Int32(0), sampler_code(sampler), Int32(0))
