@@ -304,7 +304,10 @@ __device__ __forceinline__ bool wolfe_search(float x, float f, float g, float p,
   return true;
 }
 
-template <int G, int OBJ>
+// H = history capacity (m <= H): the pairs live in registers, so a kernel
+// sized for the usual m <= 10 holds 18 fewer per lane than one sized for 16
+// (71 against 90 VGPRs for 16-D Rastrigin: 7 waves per SIMD instead of 5).
+template <int G, int H, int OBJ>
 __global__ __launch_bounds__(256) void k_lbfgs(SwarmDev S, ProbArgs P, LbfgsArgs A, float* out_x,
                                                float* out_f) {
   const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -317,18 +320,18 @@ __global__ __launch_bounds__(256) void k_lbfgs(SwarmDev S, ProbArgs P, LbfgsArgs
   float x = projc(act ? S.pb[(int64_t)l * S.npad + i] : 0.0f, lo, hi);
   float f, g;
   lane_fg<G, OBJ>(x, l, d, P, f, g);
-  float Sh[LB_MAXH], Yh[LB_MAXH], RHO[LB_MAXH];
+  float Sh[H], Yh[H], RHO[H];
 #pragma unroll
-  for (int h = 0; h < LB_MAXH; ++h) Sh[h] = Yh[h] = RHO[h] = 0.0f;
+  for (int h = 0; h < H; ++h) Sh[h] = Yh[h] = RHO[h] = 0.0f;
   int k = 0;
   const int m = A.history;
   for (int it = 0; it < A.maxiters; ++it) {
     const float pg = projc(x - g, lo, hi) - x;
     const float pgn = gmaxabs<G>(pg);
     if (!(pgn > A.abstol)) break;
-    float q = g, a[LB_MAXH];
+    float q = g, a[H];
 #pragma unroll
-    for (int h = 0; h < LB_MAXH; ++h) {
+    for (int h = 0; h < H; ++h) {
       a[h] = 0.0f;
       if (h < k) {
         a[h] = RHO[h] * gsum<G>(Sh[h] * q);
@@ -339,7 +342,7 @@ __global__ __launch_bounds__(256) void k_lbfgs(SwarmDev S, ProbArgs P, LbfgsArgs
     if (k > 0) gamma = gsum<G>(Sh[0] * Yh[0]) / gsum<G>(Yh[0] * Yh[0]);
     float r = gamma * q;
 #pragma unroll
-    for (int h = LB_MAXH - 1; h >= 0; --h) {
+    for (int h = H - 1; h >= 0; --h) {
       if (h < k) {
         const float b = RHO[h] * gsum<G>(Yh[h] * r);
         r = r + (a[h] - b) * Sh[h];
@@ -372,7 +375,7 @@ __global__ __launch_bounds__(256) void k_lbfgs(SwarmDev S, ProbArgs P, LbfgsArgs
     if (sy > 1e-10f * yy && sy > 0.0f) {
       const int top = k < m ? k : m - 1;
 #pragma unroll
-      for (int h = LB_MAXH - 1; h >= 1; --h) {
+      for (int h = H - 1; h >= 1; --h) {
         if (h <= top) {
           Sh[h] = Sh[h - 1];
           Yh[h] = Yh[h - 1];
@@ -401,7 +404,12 @@ static hipError_t launch_lbfgs_g(const SwarmDev& S, const ProbArgs& P, const Lbf
                                  float* ox, float* of, hipStream_t st) {
   const int64_t threads = S.n * G;
   const int64_t blocks = (threads + 255) / 256;
-  hipLaunchKernelGGL((k_lbfgs<G, OBJ>), dim3((unsigned)blocks), dim3(256), 0, st, S, P, A, ox, of);
+  if (A.history <= 10)
+    hipLaunchKernelGGL((k_lbfgs<G, 10, OBJ>), dim3((unsigned)blocks), dim3(256), 0, st, S, P, A,
+                       ox, of);
+  else
+    hipLaunchKernelGGL((k_lbfgs<G, LB_MAXH, OBJ>), dim3((unsigned)blocks), dim3(256), 0, st, S, P,
+                       A, ox, of);
   return hipGetLastError();
 }
 
