@@ -182,7 +182,9 @@ typedef struct pps_swarm_params {
   int32_t clamp;             /* PPS_CLAMP_*                                  */
   int32_t launch;            /* PPS_LAUNCH_* (sync only)                     */
   uint64_t seed;             /* Philox key of this swarm's random stream     */
-  float c1, c2;              /* 1.4962f0 in the reference                    */
+  float c1, c2;              /* 1.4962f0 in the reference; 0, not finite or
+                                at least 2^-102 in magnitude (others are
+                                refused)                                     */
   int32_t qlock_min_improvers; /* QLOCK: block publishes only when at least
                                   this many particles beat the global best;
                                   the reference uses 2 (src/kernels.jl:97)   */

@@ -164,6 +164,7 @@ __device__ uint64_t step_sweep(const Swarm& S, const ProbLds* L, int64_t first, 
   uint64_t mykey = KEY_NONE;
   const int d = S.d;
   const uint32_t row4 = (uint32_t)S.np * 4u;
+  const float c1s = scaled_coef(S.c1), c2s = scaled_coef(S.c2);
   for (int64_t tile = first; tile * TILE < S.n; tile += stride) {
     const int64_t i0 = tile * TILE + threadIdx.x;
     Accum<OBJ> acc[PPT];
@@ -194,11 +195,11 @@ __device__ uint64_t step_sweep(const Swarm& S, const ProbLds* L, int64_t first, 
         for (int h = 0; h < 2; ++h) {
           const int j = 2 * k + h;
           if (j < d) {
-            const float r1 = u01(h == 0 ? r.x : r.z);
-            const float r2 = u01(h == 0 ? r.y : r.w);
+            const float m1 = draw_m(h == 0 ? r.x : r.z);
+            const float m2 = draw_m(h == 0 ? r.y : r.w);
             const float a = w * v[q][h];
-            const float b = (S.c1 * r1) * (pbv[q][h] - x[q][h]);
-            const float c = (S.c2 * r2) * (L->gb[j] - x[q][h]);
+            const float b = (c1s * m1) * (pbv[q][h] - x[q][h]);
+            const float c = (c2s * m2) * (L->gb[j] - x[q][h]);
             const float vn = (a + b) + c;
             float xn = x[q][h] + vn;
             xn = jl_min(jl_max(xn, L->lb[j]), L->ub[j]);
@@ -237,6 +238,7 @@ __device__ uint64_t part_step_sweep(const Swarm& S, const ProbLds* L, int64_t fi
                                     int64_t stride, uint32_t t, float w) {
   uint64_t mykey = KEY_NONE;
   const int d = S.d;
+  const float c1s = scaled_coef(S.c1), c2s = scaled_coef(S.c2);
   for (int64_t tile = first; tile * PTILE < S.n; tile += stride) {
     const int64_t i = tile * PTILE + threadIdx.x;
     if (i >= S.n) continue;
@@ -267,11 +269,11 @@ __device__ uint64_t part_step_sweep(const Swarm& S, const ProbLds* L, int64_t fi
         for (int e = 0; e < 2; ++e) {
           const int h = h2 + e, j = j0 + e;
           if (j < d) {
-            const float r1 = u01(e == 0 ? r.x : r.z);
-            const float r2 = u01(e == 0 ? r.y : r.w);
+            const float m1 = draw_m(e == 0 ? r.x : r.z);
+            const float m2 = draw_m(e == 0 ? r.y : r.w);
             const float a = w * v[h];
-            const float b = (S.c1 * r1) * (pbv[h] - x[h]);
-            const float c = (S.c2 * r2) * (L->gb[j] - x[h]);
+            const float b = (c1s * m1) * (pbv[h] - x[h]);
+            const float c = (c2s * m2) * (L->gb[j] - x[h]);
             const float vn = (a + b) + c;
             float xn = x[h] + vn;
             xn = jl_min(jl_max(xn, L->lb[j]), L->ub[j]);

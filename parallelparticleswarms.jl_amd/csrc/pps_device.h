@@ -49,6 +49,14 @@ __device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
 __device__ __forceinline__ float u01(uint32_t u) {
   return (float)(u >> 8) * 0x1.0p-24f;
 }
+// The step's attraction terms c * u01(u) as (c * 2^-24) * m, m = the top 24
+// bits as a float: m * 2^-24 is exact, and so is c * 2^-24 for c = 0, c not
+// finite or |c| >= 2^-102 (the library refuses other nonzero c), so both forms
+// round the same real number c m 2^-24 once and are equal bit for bit, with
+// one multiplication per draw less.  scaled_coef(c) is computed once per
+// thread, draw_m per draw.
+__device__ __forceinline__ float draw_m(uint32_t u) { return (float)(u >> 8); }
+__device__ __forceinline__ float scaled_coef(float c) { return c * 0x1.0p-24f; }
 
 // ---------------------------------------------------------------------------
 // Julia's Base.max / Base.min for IEEE floats: NaN-propagating, and
@@ -812,6 +820,7 @@ __device__ __forceinline__ void velocity_position(
     float (&x)[MAXD], float (&v)[MAXD], const float (&pb)[MAXD],
     const float* gb, float w, float c1, float c2, uint32_t i, uint32_t t,
     uint32_t epoch, uint32_t k0, uint32_t k1, const int d, const ProbArgs& P) {
+  const float c1s = scaled_coef(c1), c2s = scaled_coef(c2);
 #pragma unroll
   for (int k = 0; 2 * k < MAXD; ++k) {
     if (2 * k < d) {
@@ -820,11 +829,11 @@ __device__ __forceinline__ void velocity_position(
       for (int h = 0; h < 2; ++h) {
         const int j = 2 * k + h;
         if (j < MAXD && j < d) {
-          const float r1 = u01(h == 0 ? r.x : r.z);
-          const float r2 = u01(h == 0 ? r.y : r.w);
+          const float m1 = draw_m(h == 0 ? r.x : r.z);
+          const float m2 = draw_m(h == 0 ? r.y : r.w);
           const float a = w * v[j];
-          const float b = (c1 * r1) * (pb[j] - x[j]);
-          const float c = (c2 * r2) * (gb[j] - x[j]);
+          const float b = (c1s * m1) * (pb[j] - x[j]);  // (c1 * r1) * (pb - x)
+          const float c = (c2s * m2) * (gb[j] - x[j]);
           v[j] = (a + b) + c;
           x[j] = x[j] + v[j];
         }
@@ -836,8 +845,9 @@ __device__ __forceinline__ void velocity_position(
 
 // The same step split in two: the draws of particle i at iteration t (they
 // depend on neither the state nor the global best, so a kernel can compute
-// them while its loads are in flight), then the update from those draws.
-// Together they compute exactly what velocity_position computes.
+// them while its loads are in flight; kept as draw_m values), then the update
+// from those draws.  Together they compute exactly what velocity_position
+// computes.
 template <int MAXD>
 __device__ __forceinline__ void step_draws(float (&r1)[MAXD], float (&r2)[MAXD], uint32_t i,
                                            uint32_t t, uint32_t epoch, uint32_t k0, uint32_t k1,
@@ -846,11 +856,11 @@ __device__ __forceinline__ void step_draws(float (&r1)[MAXD], float (&r2)[MAXD],
   for (int k = 0; 2 * k < MAXD; ++k) {
     if (2 * k < d) {
       const u4 r = philox4x32_10(u4{i, t, (uint32_t)k | (PPS_TAG_STEP << 16), epoch}, k0, k1);
-      r1[2 * k] = u01(r.x);
-      r2[2 * k] = u01(r.y);
+      r1[2 * k] = draw_m(r.x);
+      r2[2 * k] = draw_m(r.y);
       if (2 * k + 1 < MAXD) {
-        r1[2 * k + 1] = u01(r.z);
-        r2[2 * k + 1] = u01(r.w);
+        r1[2 * k + 1] = draw_m(r.z);
+        r2[2 * k + 1] = draw_m(r.w);
       }
     }
   }
@@ -860,12 +870,13 @@ __device__ __forceinline__ void velocity_position_drawn(
     float (&x)[MAXD], float (&v)[MAXD], const float (&pb)[MAXD], const float (&gb)[MAXD],
     float w, float c1, float c2, const float (&r1)[MAXD], const float (&r2)[MAXD], const int d,
     const ProbArgs& P) {
+  const float c1s = scaled_coef(c1), c2s = scaled_coef(c2);
 #pragma unroll
   for (int j = 0; j < MAXD; ++j) {
     if (j < d) {
       const float a = w * v[j];
-      const float b = (c1 * r1[j]) * (pb[j] - x[j]);
-      const float c = (c2 * r2[j]) * (gb[j] - x[j]);
+      const float b = (c1s * r1[j]) * (pb[j] - x[j]);
+      const float c = (c2s * r2[j]) * (gb[j] - x[j]);
       v[j] = (a + b) + c;
       x[j] = x[j] + v[j];
     }

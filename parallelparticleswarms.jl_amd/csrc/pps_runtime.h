@@ -25,5 +25,8 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 // Validate a pps_problem and convert it to kernel arguments (obs pointer left
 // unset: the caller uploads observations).
 int fill_prob_args(const pps_problem* prob, ProbArgs* P);
+// The step's c1 and c2: 0, not finite, or at least 2^-102 in magnitude (the
+// kernels fold the draws' 2^-24 scale into them, exactly for these values).
+int check_step_coefs(float c1, float c2);
 
 }  // namespace pps

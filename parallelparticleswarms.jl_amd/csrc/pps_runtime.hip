@@ -28,6 +28,15 @@ namespace pps {
 
 thread_local std::string g_err;
 
+int check_step_coefs(float c1, float c2) {
+  for (float c : {c1, c2})
+    if (c != 0.0f && std::isfinite(c) && std::fabs(c) < 0x1.0p-102f)
+      return fail(PPS_E_INVALID,
+                  "c1 = %g, c2 = %g: nonzero coefficients below 2^-102 in magnitude are not "
+                  "supported", (double)c1, (double)c2);
+  return PPS_OK;
+}
+
 int fail(int code, const char* fmt, ...) {
   char buf[1024];
   va_list ap;
@@ -498,6 +507,7 @@ static int swarm_create(pps_ctx* ctx, const pps_problem* prob, const pps_swarm_p
     return fail(PPS_E_INVALID, "num_particles %lld outside 1..2^31-1", (long long)prm->n);
   if (prm->mode < PPS_MODE_SYNC || prm->mode > PPS_MODE_QLOCK)
     return fail(PPS_E_INVALID, "unknown mode %d", prm->mode);
+  if (check_step_coefs(prm->c1, prm->c2) != PPS_OK) return PPS_E_INVALID;
   if (prm->gbest_from != PPS_GBEST_PBEST && prm->gbest_from != PPS_GBEST_CURRENT)
     return fail(PPS_E_INVALID, "unknown gbest_from %d", prm->gbest_from);
   if (prm->clamp != PPS_CLAMP_COMPONENT && prm->clamp != PPS_CLAMP_LEXICOGRAPHIC)

@@ -490,6 +490,7 @@ int pps_server_submit(pps_server* sv, const pps_problem* prob, const pps_swarm_p
     return fail(PPS_E_INVALID, "served problems use sync PSO with best-position global best");
   if (prm->n < 1 || prm->n > (1 << 24)) return fail(PPS_E_INVALID, "num_particles outside 1..2^24");
   if (maxiters < 0) return fail(PPS_E_INVALID, "maxiters < 0");
+  if (check_step_coefs(prm->c1, prm->c2) != PPS_OK) return PPS_E_INVALID;
   Req* r = new Req();
   int rc = fill_prob_args(prob, &r->P);
   if (rc != PPS_OK) {

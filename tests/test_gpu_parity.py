@@ -321,3 +321,77 @@ def test_argmin_at_every_lane(n, launch):
     bc[n // 2] = -0.0
     g.set_state(x, v, b, bc, 3)
     assert g.gbest().index == n // 2
+
+
+@pytest.mark.parametrize("c1,c2", [(0.0, 1.4962), (2.0 ** -102, -2.5), (3.0, 1e-20),
+                                   (-1.25, 7.0)])
+@pytest.mark.parametrize("launch", [L.LAUNCH_WORKGROUP, L.LAUNCH_GRAPH])
+def test_step_coefficients(c1, c2, launch):
+    # the kernels fold the draws' 2^-24 scale into c1 and c2; bit for bit the
+    # oracle's c * u01(u) for every coefficient the library accepts
+    note(f"coefficients c1={c1} c2={c2} launch={launch}")
+    import ppswarm as P
+
+    d, n = 5, 900 if launch == L.LAUNCH_WORKGROUP else 3000
+    lb, ub = np.full(d, -5.12, np.float32), np.full(d, 5.12, np.float32)
+    prob = P.OptimizationProblem(P.Rastrigin(), np.zeros(d, np.float32), lb=lb, ub=ub)
+    g = P.Swarm(P.context(), prob, P.swarm_params(n, L.MODE_SYNC, seed=41, launch=launch,
+                                                  c1=c1, c2=c2))
+    o = O.Swarm(O.Problem(O.RASTRIGIN, d, lb, ub), n, 41)
+    for chunk in (1, 6):
+        g.run(chunk)
+        o.run_sync(chunk, c1=c1, c2=c2)
+        assert_same_state(g, o, f"after {chunk}")
+        assert_same_gbest(g, o, f"after {chunk}")
+
+
+def test_tiny_step_coefficients_are_refused():
+    note("coefficient refusals")
+    import ctypes as C
+    import ppswarm as P
+
+    d = 3
+    prob = P.OptimizationProblem(P.Rastrigin(), np.zeros(d, np.float32), lb=np.full(d, -1.0),
+                                 ub=np.full(d, 1.0))
+    for c1, c2 in ((1e-35, 1.0), (1.0, -3e-32)):
+        with pytest.raises(P.PPSError):
+            P.Swarm(P.context(), prob, P.swarm_params(2000, L.MODE_SYNC, seed=1, c1=c1, c2=c2))
+    sv = P.Server()
+    try:
+        pr, keep = prob.c()
+        sp = P.swarm_params(2000, L.MODE_SYNC, seed=1, c1=1e-35)
+        t = C.c_uint64()
+        rc = L.lib().pps_server_submit(sv.h, C.byref(pr), C.byref(sp), 10, 0.7298, 1.0,
+                                       C.byref(t))
+        assert rc == L.E_INVALID
+    finally:
+        sv.close()
+
+
+def test_served_step_coefficients_equal_the_plain_path():
+    # a served problem with other coefficients ends where pps_swarm_init +
+    # pps_swarm_run end with them
+    note("served coefficients")
+    import ctypes as C
+    import ppswarm as P
+
+    d, n, T = 6, 4000, 40
+    prob = P.OptimizationProblem(P.Rosenbrock(), np.zeros(d, np.float32), [1.0, 100.0],
+                                 lb=np.full(d, -2.0), ub=np.full(d, 2.0))
+    sp = P.swarm_params(n, L.MODE_SYNC, seed=9, c1=0.75, c2=2.0 ** -102)
+    plain = P.Swarm(P.context(), prob, sp)
+    plain.run(T)
+    ref = plain.gbest()
+    plain.free()
+    sv = P.Server()
+    try:
+        pr, keep = prob.c()
+        t = C.c_uint64()
+        L.check(L.lib().pps_server_submit(sv.h, C.byref(pr), C.byref(sp), T, 0.7298, 1.0,
+                                          C.byref(t)))
+        res = sv.wait(int(t.value))
+    finally:
+        sv.close()
+    assert res.index == ref.index
+    assert np.float32(res.cost_f32) == np.float32(ref.cost)
+    np.testing.assert_array_equal(np.asarray(res.u, np.float32), ref.position)

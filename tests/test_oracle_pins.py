@@ -326,3 +326,31 @@ def test_lv_fixture_loss_at_true_parameters(oracle):
     assert abs(sse32 - sse64) < 1e-3 * sse64
     # and the noise has the stated scale
     assert abs(np.std(obs - traj) - doc["sigma"]) < 0.005
+
+
+def _bits(a):
+    a = np.asarray(a, np.float32)
+    return np.where(np.isnan(a), np.uint32(0x7FC00000), a.view(np.uint32))
+
+
+def test_folded_draw_scale_equals_the_scaled_draw():
+    # The kernels compute the attraction coefficient c * u01(u) as
+    # (c * 2^-24) * m with m = the draw's top 24 bits as a float; the oracle
+    # computes c * (m * 2^-24).  Both round the same real number once when
+    # c * 2^-24 is exact: c = 0, c not finite, or |c| >= 2^-102.
+    rng = np.random.default_rng(5)
+    m = np.concatenate([rng.integers(0, 1 << 24, 20000), [0, 1, 2, (1 << 24) - 1]])
+    m = m.astype(np.float32)
+    s = np.float32(2.0 ** -24)
+    allowed = [1.4962, -1.4962, 2.0 ** -102, -(2.0 ** -102), 1.3 * 2.0 ** -100, 1e-20, 7.0,
+               1e30, 3.4e38, 0.0, -0.0, np.inf, -np.inf, np.nan]
+    for c in allowed:
+        c = np.float32(c)
+        with np.errstate(over="ignore", invalid="ignore"):  # c = 3.4e38 overflows, inf * 0
+            folded = np.float32(c * s) * m
+            scaled = c * (m * s)
+        np.testing.assert_array_equal(_bits(folded), _bits(scaled), err_msg=f"c = {c}")
+    # below the limit c * 2^-24 is subnormal and loses bits: the forms differ,
+    # which is why the library refuses such coefficients
+    c = np.float32(1.7 * 2.0 ** -110)
+    assert np.any(_bits(np.float32(c * s) * m) != _bits(c * (m * s)))
