@@ -97,6 +97,24 @@ __device__ __forceinline__ float& at32(float* base, uint32_t off) {
   return *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off);
 }
 
+// Copies coordinates j < d of one particle from the rows of src to those of
+// dst (byte offsets i4 + j * row4) in chunks of CH coordinates, each chunk's
+// loads issued before its stores: ceil(d / CH) memory round trips instead of
+// d (a load after a store that may alias it waits for the store).
+template <int CH>
+__device__ __forceinline__ void copy_particle_rows(float* dst, float* src, uint32_t i4,
+                                                   uint32_t row4, int d) {
+  for (int j0 = 0; j0 < d; j0 += CH) {
+    float t[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (j0 + c < d) t[c] = at32(src, i4 + (uint32_t)(j0 + c) * row4);
+#pragma unroll
+    for (int c = 0; c < CH; ++c)
+      if (j0 + c < d) at32(dst, i4 + (uint32_t)(j0 + c) * row4) = t[c];
+  }
+}
+
 __device__ __forceinline__ Swarm load_swarm(const BatchProb* B, ProbLds* L) {
   Swarm S;
   S.d = B->P.d;
@@ -168,8 +186,14 @@ __device__ uint64_t step_sweep(const Swarm& S, const ProbLds* L, int64_t first, 
   for (int64_t tile = first; tile * TILE < S.n; tile += stride) {
     const int64_t i0 = tile * TILE + threadIdx.x;
     Accum<OBJ> acc[PPT];
+    float pcq[PPT];
 #pragma unroll
-    for (int q = 0; q < PPT; ++q) acc[q].start(d);
+    for (int q = 0; q < PPT; ++q) {
+      acc[q].start(d);
+      // the best costs go out with the first coordinate pair's loads
+      const int64_t i = i0 + (int64_t)q * BATCH_BLOCK;
+      pcq[q] = i < S.n ? S.PC[i] : 0.0f;
+    }
     for (int k = 0; 2 * k < d; ++k) {
       float x[PPT][2], v[PPT][2], pbv[PPT][2];
 #pragma unroll
@@ -215,11 +239,9 @@ __device__ uint64_t step_sweep(const Swarm& S, const ProbLds* L, int64_t first, 
     for (int q = 0; q < PPT; ++q) {
       const int64_t i = i0 + (int64_t)q * BATCH_BLOCK;
       if (i >= S.n) continue;
-      float pc = S.PC[i];
+      float pc = pcq[q];
       if (acc[q].res < pc) {
-        for (int j = 0; j < d; ++j)
-          at32(S.PB, (uint32_t)i * 4u + (uint32_t)j * row4) =
-              at32(S.X, (uint32_t)i * 4u + (uint32_t)j * row4);
+        copy_particle_rows<4>(S.PB, S.X, (uint32_t)i * 4u, row4, d);
         pc = acc[q].res;
         S.PC[i] = pc;
       }
@@ -286,7 +308,7 @@ __device__ uint64_t part_step_sweep(const Swarm& S, const ProbLds* L, int64_t fi
       }
     }
     if (acc.res < pc) {
-      for (int j = 0; j < d; ++j) at32(S.PB, i4 + (uint32_t)j * row4) = at32(S.X, i4 + (uint32_t)j * row4);
+      copy_particle_rows<8>(S.PB, S.X, i4, row4, d);
       pc = acc.res;
       S.PC[i] = pc;
     }
@@ -295,14 +317,13 @@ __device__ uint64_t part_step_sweep(const Swarm& S, const ProbLds* L, int64_t fi
   return mykey;
 }
 
-// The owning thread of the workgroup minimum copies that particle's best
-// position (which it wrote itself, or which is unchanged) to dst[0..d).
-__device__ __forceinline__ void copy_winner(const Swarm& S, uint64_t mykey, uint64_t bk,
-                                            float* dst) {
-  if (bk != KEY_NONE && mykey == bk) {
-    const uint32_t wi = (uint32_t)bk;
-    for (int j = 0; j < S.d; ++j) dst[j] = at32(S.PB, wi * 4u + (uint32_t)j * ((uint32_t)S.np * 4u));
-  }
+// The best position of the workgroup minimum `bk` (known to every thread) to
+// dst[0..d), one coordinate per thread.  Called after the block reduction,
+// whose barriers order this workgroup's best-position writes before the reads.
+__device__ __forceinline__ void copy_winner(const Swarm& S, uint64_t bk, float* dst) {
+  const int j = (int)threadIdx.x;
+  if (bk != KEY_NONE && j < S.d)
+    dst[j] = at32(S.PB, (uint32_t)bk * 4u + (uint32_t)j * ((uint32_t)S.np * 4u));
 }
 
 // SMALL item: run iterations until the problem ends or the deadline passes.
@@ -320,7 +341,7 @@ __device__ bool run_small(BatchProb* B, uint64_t deadline, uint64_t* s_red, Prob
   if (!was_initialized) {
     const uint64_t mykey = init_sweep<OBJ, PPT>(S, L, 0, 1);
     const uint64_t bk = block_min<BATCH_BLOCK>(mykey, s_red);
-    copy_winner(S, mykey, bk, L->gb);
+    copy_winner(S, bk, L->gb);
     __syncthreads();
     if ((int)threadIdx.x < S.d) B->gb[threadIdx.x] = L->gb[threadIdx.x];
     if (threadIdx.x == 0) {
@@ -348,7 +369,7 @@ __device__ bool run_small(BatchProb* B, uint64_t deadline, uint64_t* s_red, Prob
     const uint64_t mykey = S.d > 4 ? part_step_sweep<OBJ>(S, L, 0, 1, t, w)
                                    : step_sweep<OBJ>(S, L, 0, 1, t, w);
     bk = block_min<BATCH_BLOCK>(mykey, s_red);
-    copy_winner(S, mykey, bk, L->gb);
+    copy_winner(S, bk, L->gb);
     if (threadIdx.x == 0) {
       // stop when another iteration as long as this one would end past the
       // deadline (the first iteration always runs)
@@ -416,11 +437,30 @@ __device__ void run_part(BatchProb* B, const BatchItem& it, BatchXchg* XC, uint6
   }
   const uint64_t bk = block_min<BATCH_BLOCK>(mykey, s_red);
   const uint32_t par = it.t & 1u;
-  copy_winner(S, mykey, bk, XC->cand[par][it.part]);
+  copy_winner(S, bk, XC->cand[par][it.part]);
   if (threadIdx.x == 0) XC->key[par][it.part] = bk;
 }
 
 }  // namespace
+
+// Result of a finished problem into host-visible memory (one thread): every
+// coordinate read from LDS before the stores, the done flag last with a
+// system-scope release.
+__device__ __forceinline__ void publish_result(BatchOut* o, float cost, uint32_t idx,
+                                               uint32_t iter, const float* gb, int d) {
+  float g[PPS_BMAXD];
+#pragma unroll
+  for (int j = 0; j < PPS_BMAXD; ++j) g[j] = j < d ? gb[j] : 0.0f;
+  o->gb_cost = cost;
+  o->gb_idx = idx;
+  o->iter = iter;
+  o->remaining = 0;
+#pragma unroll
+  for (int j = 0; j < PPS_BMAXD; ++j)
+    if (j < d) o->gb[j] = g[j];
+  __threadfence_system();
+  __hip_atomic_store(&o->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 __global__ __launch_bounds__(BATCH_BLOCK) void k_slice(BatchProb* probs, const BatchProb* fresh,
                                                        const BatchItem* items,
@@ -437,24 +477,29 @@ __global__ __launch_bounds__(BATCH_BLOCK) void k_slice(BatchProb* probs, const B
       // newly admitted: pull the descriptor from host-visible memory
       const uint32_t* src = reinterpret_cast<const uint32_t*>(fresh + it.slot);
       uint32_t* dst = reinterpret_cast<uint32_t*>(B);
-      for (int k = threadIdx.x; k < (int)(sizeof(BatchProb) / 4); k += BATCH_BLOCK)
-        dst[k] = src[k];
+      constexpr int W = (int)(sizeof(BatchProb) / 4);
+      constexpr int R = (W + BATCH_BLOCK - 1) / BATCH_BLOCK;
+      uint32_t t[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {  // every load before the first store
+        const int k = (int)threadIdx.x + r * BATCH_BLOCK;
+        if (k < W) t[r] = src[k];
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int k = (int)threadIdx.x + r * BATCH_BLOCK;
+        if (k < W) dst[k] = t[r];
+      }
       __syncthreads();
     }
     const bool finished = B->P.obj == PPS_OBJ_RASTRIGIN
                               ? run_small<PPS_OBJ_RASTRIGIN>(B, deadline, s_red, &s_lds, &s_stop)
                               : run_small<PPS_OBJ_ROSENBROCK>(B, deadline, s_red, &s_lds, &s_stop);
     __syncthreads();
-    if (threadIdx.x == 0 && finished) {
-      BatchOut* o = out + it.slot;
-      o->gb_cost = B->gb_cost;
-      o->gb_idx = B->gb_idx;
-      o->iter = B->iter;
-      o->remaining = 0;
-      for (int j = 0; j < B->P.d; ++j) o->gb[j] = B->gb[j];
-      __threadfence_system();
-      __hip_atomic_store(&o->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+    // s_lds.gb holds the problem's final global best position (run_small
+    // leaves it there whenever it returns true)
+    if (threadIdx.x == 0 && finished)
+      publish_result(out + it.slot, B->gb_cost, B->gb_idx, B->iter, s_lds.gb, B->P.d);
   } else if (it.kind == BATCH_PART) {
     if (B->P.obj == PPS_OBJ_RASTRIGIN)
       run_part<PPS_OBJ_RASTRIGIN>(B, it, xchg + it.slot, s_red, &s_lds);
@@ -463,16 +508,8 @@ __global__ __launch_bounds__(BATCH_BLOCK) void k_slice(BatchProb* probs, const B
   } else {  // BATCH_FINAL: it.t = last iteration
     const int d = B->P.d;
     const uint64_t key = part_gbest(xchg + it.slot, it.nparts, it.t, d, s_lds.gb, s_red);
-    if (threadIdx.x == 0) {
-      BatchOut* o = out + it.slot;
-      o->gb_cost = key_cost(key);
-      o->gb_idx = (uint32_t)key;
-      o->iter = it.t;
-      o->remaining = 0;
-      for (int j = 0; j < d; ++j) o->gb[j] = s_lds.gb[j];
-      __threadfence_system();
-      __hip_atomic_store(&o->done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+    if (threadIdx.x == 0)
+      publish_result(out + it.slot, key_cost(key), (uint32_t)key, it.t, s_lds.gb, d);
   }
 }
 

@@ -91,17 +91,19 @@ __device__ __forceinline__ uint64_t fetch_gbest(const SwarmDev& S, uint32_t par,
   return k;
 }
 
-// Block argmin over `mykey`; the owning thread copies the winner's position
-// (already written by this same thread) into cand[par][b].
+// Block argmin over `mykey`; the winner's position (its rows of src) is
+// copied into cand[par][b], one coordinate per thread: the reduction's
+// barriers order this workgroup's writes of src before the reads, and the d
+// loads are one round trip instead of d dependent ones.
 template <int BLOCK>
 __device__ __forceinline__ void publish_block(const SwarmDev& S, uint32_t par, int d,
                                               uint64_t mykey, const float* src,
                                               uint64_t* s_red) {
   const uint64_t bk = block_min_u64<BLOCK>(mykey, s_red);
   const int b = blockIdx.x;
-  if (bk != KEY_NONE && mykey == bk) {
+  if (bk != KEY_NONE) {
     const int64_t wi = (int64_t)(uint32_t)bk;
-    for (int j = 0; j < d; ++j)
+    for (int j = threadIdx.x; j < d; j += BLOCK)
       S.cand[((int64_t)par * S.G + b) * d + j] = src[(int64_t)j * S.npad + wi];
   }
   if (threadIdx.x == 0) S.bkey[(int64_t)par * S.G + b] = bk;
