@@ -56,11 +56,12 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // particles, a slice follows the budget.
 constexpr int64_t kLargeWork = 8192;
 // Device-clock budget of SMALL items while large problems are resident, so a
-// slice (and with it one iteration of every large problem) stays short.  On
-// the 4096-request mix at half the saturating rate: p50/p99 13.5/30.8 ms at
-// 20 us, 17.0/47.2 at 40 us, 13.4/30.7 at 10 us (the slice's fixed cost
-// dominates there), with the same throughput.
-constexpr int kSliceUsWithLarge = 20;
+// slice (and with it one iteration of every large problem) stays short.  The
+// slice lasts as long as its longest item: a SMALL item's budget, or a large
+// problem's 32-coordinate PART tile (~14 us).  On the 4096-request mix at half
+// the saturating rate, served p50/p99: 10.2/23.2 ms at 20 us, 9.6/21.5 at
+// 15 us, 10.7/21.7 at 10 us, 11.3/21.7 at 5 us, with the same throughput.
+constexpr int kSliceUsWithLarge = 15;
 // Tiles per PART item of a large problem.  A PART item spends a few dependent
 // round trips (its descriptor, the part keys, the winner's position, and at
 // the end its own winner's row) around the tiles it streams; two tiles per
