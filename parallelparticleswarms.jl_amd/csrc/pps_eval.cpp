@@ -113,7 +113,7 @@ double ode_sse_tsit5_f64(const double* th, const pps_problem* P, bool adaptive) 
         const double h = hit ? t_next - t : dt;
         const long max_steps = P->n_params > 6 && P->params[6] > 0 ? (long)P->params[6] : 10000;
         if (adaptive && (++steps > max_steps || !(t + h > t))) return INFINITY;
-        double un[2], err[2];
+        double un[2] = {0.0, 0.0}, err[2] = {0.0, 0.0};
         step(h, un, err);
         if (!adaptive) {
           u[0] = un[0], u[1] = un[1];
@@ -162,7 +162,7 @@ double ode_sse_tsit5_saveat_f64(const double* th, const pps_problem* P, bool ada
     const bool last = !(t + dt < t_end);
     const double h = last ? t_end - t : dt;
     if (++steps > max_steps || !(t + h > t)) return INFINITY;
-    double un[2], err[2];
+    double un[2] = {0.0, 0.0}, err[2] = {0.0, 0.0};
     for (int i = 1; i < 7; ++i) {
       double y[2];
       for (int c = 0; c < 2; ++c) {

@@ -293,6 +293,8 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 #define PPS_SYNC_GRID_PER_CU 2
 #endif
 
+}  // extern "C"
+
 namespace {
 
 // Autotuned (grid_per_cu, particles_per_workgroup) keyed by
@@ -330,6 +332,8 @@ void tune_load_locked(TuneTable& t) {
 }
 
 }  // namespace
+
+extern "C" {
 
 extern "C" int pps_tuned_launch(int32_t obj, int32_t d, int64_t n, int32_t* grid_per_cu,
                                 int32_t* particles_per_workgroup) {
