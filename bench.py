@@ -155,7 +155,7 @@ def c4_mix(P, L, ctx, count=4096, plain_sample=256, served_qps=None):
     wall = time.perf_counter() - t0
     # plain per-problem launch loop on a sample of the same mix
     t0 = time.perf_counter()
-    plain = [P.solve_alone(q, L.LAUNCH_EAGER) for q in reqs[:plain_sample]]
+    plain = [P.solve_alone(q) for q in reqs[:plain_sample]]
     pwall = time.perf_counter() - t0
     mismatch = sum(1 for a, b in zip(res, plain)
                    if not (np.array_equal(a.u, b.u) and a.cost_f32 == b.cost_f32))
@@ -337,7 +337,8 @@ def ode_c3(P, L, clock_khz):
 
 def n_sweep(P, L):
     """Sync step at 2^10..2^24 particles (10-D, both objectives): average launch
-    time over a timed graph replay after 1000 warm-up iterations (the
+    time over a timed run (a graph replay below 2^19 particles, launches one by
+    one from there, the library's choice) after 1000 warm-up iterations (the
     headline's timed range also starts after 1000 iterations).  The
     state is 124 B per particle, so it outgrows the 256 MB Infinity Cache
     from 2^22 particles on; below that the achieved rate includes cache hits."""
@@ -349,7 +350,12 @@ def n_sweep(P, L):
         for obj, seed in (("rastrigin", 11), ("rosenbrock", 12)):
             s = make_swarm(P, obj, n, D, seed, L.MODE_SYNC)
             s.run(1000)  # past the early phase where most particles still improve
-            s.run(iters)  # captures the graph of this length
+            # below 2^19 particles the library captures a graph when a run
+            # repeats the last one's parameters: the second run of this length
+            # captures it and uploads it with its first launch, the third is
+            # timed (from 2^19 every run launches its iterations one by one)
+            s.run(iters)
+            s.run(iters)
             s.run(iters)
             us.append(s.elapsed_ms() * 1e3 / iters)
             s.free()
@@ -366,7 +372,8 @@ def n_sweep(P, L):
 def beyond_cache(P, L, lg=22, iters=200):
     """The sync step with the state past the 256 MB Infinity Cache (2^22
     particles x 10-D: 520 MB of state), both objectives: average launch time
-    over `iters` graph-replayed iterations after 1000 untimed ones."""
+    over `iters` iterations after 1000 untimed ones (launched one by one, the
+    library's choice at this size)."""
     n = 1 << lg
     us = []
     for obj, seed in (("rastrigin", 21), ("rosenbrock", 22)):

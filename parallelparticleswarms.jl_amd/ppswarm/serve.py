@@ -128,8 +128,10 @@ def batch_solve(reqs: list[Request], ctx: Optional[api.Context] = None):
     return [_result(out[i]) for i in range(n)], float(ms.value)
 
 
-def solve_alone(req: Request, launch: int = L.LAUNCH_EAGER, ctx=None) -> Result:
-    """The plain per-problem path: init, `maxiters` launches, read the best."""
+def solve_alone(req: Request, launch: int = L.LAUNCH_AUTO, ctx=None) -> Result:
+    """The plain per-problem path: init, `maxiters` iterations with the
+    library's launch choice (one workgroup up to 1024 particles, else one
+    launch per iteration for a one-shot run), read the best."""
     ctx = ctx or api.context()
     sw = api.Swarm(ctx, req.prob, api.swarm_params(req.n, L.MODE_SYNC, seed=req.seed,
                                                    launch=launch))
