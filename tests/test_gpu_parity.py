@@ -84,6 +84,26 @@ def test_graph_replay_after_reinit(obj, d, n):
     assert_same_gbest(g, o, "after the last reinit")
 
 
+def test_automatic_launch_captures_on_repeat():
+    # the automatic launch mode runs a first run's iterations one by one and
+    # captures a graph only when a run repeats the previous run's parameters;
+    # every run (one by one, capture, replay, a new length, after a reinit)
+    # must equal the oracle
+    note("automatic launch: eager first run, capture on repeat, replay")
+    g, o = make_pair(O.RASTRIGIN, 10, 5000, seed=77, launch=L.LAUNCH_AUTO)
+    for rnd, T in enumerate((9, 9, 9, 5, 5, 9)):
+        g.run(T)
+        o.run_sync(T)
+        assert_same_state(g, o, f"run {rnd} (T={T})")
+        assert_same_gbest(g, o, f"run {rnd} (T={T})")
+        if rnd == 3:
+            g.reinit()
+            o.reinit()
+    g.run(5)
+    o.run_sync(5)
+    assert_same_state(g, o, "last run")
+
+
 def test_sync_current_mode_multiblock():
     note("sync multiblock gbest_from=CURRENT")
     g, o = make_pair(O.ROSENBROCK, 4, 4000, seed=99, lb=-1.0, ub=10.0,
