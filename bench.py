@@ -137,8 +137,23 @@ def cpu_baseline(sample_iters=200):
             s.run_sync(500, gbest_from=O.GBEST_CURRENT)
             times.append(time.perf_counter() - t0)
         c0[key] = float(np.median(times[1:]) * 1e3)
+    # the reference's CPU_vs_GPU benchmark on the CPU backend (the restatement's
+    # sync run, global best from the argmin's current position), up to 2^15
+    # particles: a timed run after an untimed one on the same swarm
+    ref = []
+    prob = O.Problem(O.ROSENBROCK, 10, np.full(10, -1.0, np.float32),
+                     np.full(10, 10.0, np.float32), np.asarray((1.0, 100.0), np.float32))
+    for lg in REF_SWEEP_LOG2N:
+        if lg > 15:
+            break
+        s = O.Swarm(prob, 1 << lg, seed=lg, nthreads=threads)
+        s.run_sync(REF_SWEEP_ITERS, gbest_from=O.GBEST_CURRENT)
+        t0 = time.perf_counter()
+        s.run_sync(REF_SWEEP_ITERS, gbest_from=O.GBEST_CURRENT)
+        ref.append({"particles": 1 << lg, "wall_ms": (time.perf_counter() - t0) * 1e3})
     return {"value": total_pi / total_s, "unit": "particle-iters/s", "cores": threads,
             "kind": "port",
+            "reference_sweep": ref,
             "sample": f"sync PSO, 10-D Rastrigin + 10-D Rosenbrock, 2^20 particles each, "
                       f"{sample_iters} timed iterations each (after 1 warm-up), "
                       f"{total_s:.1f} s of CPU time",
@@ -338,7 +353,7 @@ def ode_c3(P, L, clock_khz):
 def n_sweep(P, L):
     """Sync step at 2^10..2^24 particles (10-D, both objectives): average launch
     time over a timed run (a graph replay below 2^19 particles, launches one by
-    one from there, the library's choice) after 1000 warm-up iterations (the
+    one from there) after 1000 warm-up iterations (the
     headline's timed range also starts after 1000 iterations).  The
     state is 124 B per particle, so it outgrows the 256 MB Infinity Cache
     from 2^22 particles on; below that the achieved rate includes cache hits."""
@@ -348,14 +363,12 @@ def n_sweep(P, L):
         iters = int(max(30, min(1000, 4e9 / (n * BYTES_PER_PARTICLE_ITER))))
         us = []
         for obj, seed in (("rastrigin", 11), ("rosenbrock", 12)):
-            s = make_swarm(P, obj, n, D, seed, L.MODE_SYNC)
+            # a captured graph below 2^19 particles (as the library's automatic
+            # mode replays for repeated runs), launches one by one from 2^19
+            launch = L.LAUNCH_GRAPH if 1024 < n < (1 << 19) else L.LAUNCH_AUTO
+            s = make_swarm(P, obj, n, D, seed, L.MODE_SYNC, launch)
             s.run(1000)  # past the early phase where most particles still improve
-            # below 2^19 particles the library captures a graph when a run
-            # repeats the last one's parameters: the second run of this length
-            # captures it and uploads it with its first launch, the third is
-            # timed (from 2^19 every run launches its iterations one by one)
-            s.run(iters)
-            s.run(iters)
+            s.run(iters)  # captures the graph of this length (its first launch uploads it)
             s.run(iters)
             us.append(s.elapsed_ms() * 1e3 / iters)
             s.free()
@@ -367,6 +380,50 @@ def n_sweep(P, L):
                      "particle_iters_per_s": n / (avg * 1e-6)})
         log(f"[bench] sweep 2^{lg}: {avg:.2f} us/launch, {rate / 1e12:.2f} TB/s")
     return rows
+
+
+# The reference's own benchmark (benchmarks/CPU_vs_GPU/benchmark.jl:10-23,
+# 86-197, 200-222): 10-D Rosenbrock p = [1, 100], bounds [-1, 10], N = 2^3,
+# 2^5, ..., 2^19 particles, 500 iterations; each algorithm runs once untimed and
+# then once timed on the same swarm (vectorized_solve! twice on the same
+# particles); HybridPSO = async PSO (global_update = false) + SimpleLBFGS with
+# local_maxiters = 30 through solve, timed by its second solve's stats.time.
+REF_SWEEP_LOG2N = tuple(range(3, 20, 2))
+REF_SWEEP_ITERS = 500
+
+
+def reference_problem_rosenbrock10(P):
+    return P.OptimizationProblem(P.Rosenbrock(), np.zeros(10, np.float32), [1.0, 100.0],
+                                 lb=np.full(10, -1.0), ub=np.full(10, 10.0))
+
+
+def reference_sweep(P, L, ctx):
+    prob = reference_problem_rosenbrock10(P)
+    rows = []
+    for lg in REF_SWEEP_LOG2N:
+        n = 1 << lg
+        row = {"particles": n}
+        for name, mode in (("sync", L.MODE_SYNC), ("async", L.MODE_ASYNC),
+                           ("queue_lock", L.MODE_QLOCK)):
+            sw = P.Swarm(ctx, prob, P.swarm_params(n, mode, seed=lg))
+            sw.run(REF_SWEEP_ITERS)
+            t0 = time.perf_counter()
+            sw.run(REF_SWEEP_ITERS)
+            wall = time.perf_counter() - t0
+            row[name] = {"wall_ms": wall * 1e3, "device_ms": sw.elapsed_ms(),
+                         "best_f32": sw.gbest().cost}
+            sw.free()
+        alg = P.HybridPSO(pso=P.ParallelPSOKernel(n, global_update=False),
+                          local_opt=P.SimpleLBFGS())
+        P.solve(prob, alg, REF_SWEEP_ITERS, seed=lg, local_maxiters=30)
+        sol = P.solve(prob, alg, REF_SWEEP_ITERS, seed=lg, local_maxiters=30)
+        row["hybrid"] = {"wall_ms": sol.stats.time * 1e3, "objective": sol.objective}
+        rows.append(row)
+        log(f"[bench] reference sweep 2^{lg}: sync {row['sync']['wall_ms']:.3f} ms")
+    return {"workload": "the reference's CPU_vs_GPU benchmark: 10-D Rosenbrock, bounds [-1, 10], "
+                        "500 iterations, a timed run after an untimed one on the same swarm; "
+                        "HybridPSO = async PSO + L-BFGS (30 iterations, Strong-Wolfe)",
+            "rows": rows}
 
 
 def beyond_cache(P, L, lg=22, iters=200):
@@ -433,7 +490,7 @@ def main():
                          "of every leg (the library's own choice for the headline's 2^20 "
                          "particles; the same kernels everywhere)")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--extras", default="hbm,async,qlock,c0,c4,hybrid,ode",
+    ap.add_argument("--extras", default="hbm,async,qlock,c0,refsweep,c4,hybrid,ode",
                     help="comma list of the legs to run besides the headline")
     ap.add_argument("--profile-mix", choices=("all", "small", "large"),
                     help="run only the packed solve of the served mix (all, its SMALL or its "
@@ -581,6 +638,9 @@ def main():
     if args.sweep:
         out["sweep"] = n_sweep(P, L)
         log("[bench] sweep done")
+    if "refsweep" in legs:
+        out["reference_sweep"] = reference_sweep(P, L, ctx)
+        log("[bench] reference sweep done")
 
     if "c4" in legs:
         out["batched"], out["served"] = c4_mix(P, L, ctx)
@@ -596,6 +656,15 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         out["cpu_baseline"] = cpu_baseline()
         log("[bench] cpu baseline done")
+        if "reference_sweep" in out:
+            # CPU over GPU sync time per size, as the reference's benchmark
+            # reports mean(cpu_times ./ gpu_sync_times)
+            gpu = {r["particles"]: r["sync"]["wall_ms"] for r in out["reference_sweep"]["rows"]}
+            ratios = [c["wall_ms"] / gpu[c["particles"]]
+                      for c in out["cpu_baseline"]["reference_sweep"] if c["particles"] in gpu]
+            out["reference_sweep"]["cpu_over_gpu_sync"] = ratios
+            out["reference_sweep"]["mean_cpu_over_gpu_sync"] = (float(np.mean(ratios))
+                                                                if ratios else None)
     print(json.dumps(out), flush=True)
 
 

@@ -117,9 +117,9 @@ enum {
 /* Launch strategy for PPS_MODE_SYNC. */
 enum {
   PPS_LAUNCH_AUTO = 0,       /* WORKGROUP up to 1024 particles, GRAPH up to
-                                2^19 (captured when a run repeats the last
-                                run's parameters; a first run is EAGER),
-                                EAGER from 2^19 particles                  */
+                                2^19 (captured once 8 runs in a row have used
+                                the same parameters; EAGER before), EAGER from
+                                2^19 particles                             */
   PPS_LAUNCH_EAGER = 1,      /* one launch per iteration, no graph            */
   PPS_LAUNCH_GRAPH = 2,      /* iteration loop captured in a hipGraph         */
   PPS_LAUNCH_WORKGROUP = 3   /* whole loop in one workgroup (small swarms)    */

@@ -171,10 +171,11 @@ struct pps_swarm {
   float g_w = 0.0f, g_wdamp = 0.0f;
   StepArgs g_args{};
   // Parameters of the last run that the automatic launch mode ran without a
-  // graph (a graph is captured when a run repeats them).
+  // graph, and how many runs in a row used them.
   int32_t seen_T = -1;
   float seen_w = 0.0f, seen_wdamp = 0.0f;
   StepArgs seen_args{};
+  int32_t seen_runs = 0;
   int32_t sampler = PPS_SAMPLER_UNIFORM;  // sampler of init / reinit
   // Local-search outputs (allocated on first use): x (d x npad), f, argmin.
   float* lx = nullptr;
@@ -645,21 +646,28 @@ int pps_swarm_run(pps_swarm* s, int32_t maxiters, float w, float wdamp) {
   const int32_t T = maxiters;
   bool graph = T > 0 && s->prm.mode == PPS_MODE_SYNC && s->launch == PPS_LAUNCH_GRAPH;
   if (graph && s->prm.launch == PPS_LAUNCH_AUTO) {
-    // The automatic mode captures the iteration loop only for a run that
-    // repeats the previous run's parameters (the solve! / reinit! / solve!
-    // flow): a capture costs more than it saves within one run, so a one-shot
-    // solve launches its iterations one by one (same kernels, same results).
+    // The automatic mode captures the iteration loop once kCaptureAfter runs
+    // in a row have used the same parameters (the solve! / reinit! / solve!
+    // flow), and launches one by one before (same kernels, same results): a
+    // capture costs ~4 us per launch while a replay saves ~0.3 us per launch,
+    // so it pays only over many runs.
+    constexpr int32_t kCaptureAfter = 8;
     const auto same = [&](int32_t T_, float w_, float wd_, const StepArgs& a) {
       return T_ == T && w_ == w && wd_ == wdamp && a.c1 == A.c1 && a.c2 == A.c2 &&
              a.gbest_from == A.gbest_from;
     };
     const bool cached = s->gexec && same(s->g_T, s->g_w, s->g_wdamp, s->g_args);
-    if (!cached && !same(s->seen_T, s->seen_w, s->seen_wdamp, s->seen_args)) {
-      graph = false;
-      s->seen_T = T;
-      s->seen_w = w;
-      s->seen_wdamp = wdamp;
-      s->seen_args = A;
+    if (!cached) {
+      if (same(s->seen_T, s->seen_w, s->seen_wdamp, s->seen_args)) {
+        ++s->seen_runs;
+      } else {
+        s->seen_T = T;
+        s->seen_w = w;
+        s->seen_wdamp = wdamp;
+        s->seen_args = A;
+        s->seen_runs = 1;
+      }
+      if (s->seen_runs <= kCaptureAfter) graph = false;
     }
   }
   if (graph) {
@@ -842,10 +850,9 @@ int pps_autotune_sync(pps_ctx* ctx, const pps_problem* prob, int64_t n, int32_t 
       // its time: early on most particles improve and write their best
       // positions, and the fastest shape there is not always the fastest
       // later (2^20 x 10-D: 4 workgroups per CU lead 2 per CU by 3 us at
-      // iteration 100 and trail or tie them from iteration 1000).  Where the
-      // launch mode uses a graph (below 2^19 particles) the first timed run
-      // launches one by one and the next two capture and replay one, as
-      // repeated solves do; the best of the three is kept.
+      // iteration 100 and trail or tie them from iteration 1000).  These
+      // runs launch one by one (the automatic mode captures a graph only
+      // after 8 runs with the same parameters); the best of three is kept.
       rc = pps_swarm_run(s, std::max(iters, 1000), 0.7298f, 1.0f);
       for (int rep = 0; rep < 3 && rc == PPS_OK; ++rep) {
         rc = pps_swarm_run(s, iters, 0.7298f, 1.0f);

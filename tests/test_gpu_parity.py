@@ -85,18 +85,18 @@ def test_graph_replay_after_reinit(obj, d, n):
 
 
 def test_automatic_launch_captures_on_repeat():
-    # the automatic launch mode runs a first run's iterations one by one and
-    # captures a graph only when a run repeats the previous run's parameters;
-    # every run (one by one, capture, replay, a new length, after a reinit)
-    # must equal the oracle
-    note("automatic launch: eager first run, capture on repeat, replay")
+    # the automatic launch mode launches iterations one by one and captures a
+    # graph once 8 runs in a row have used the same parameters; every run (one
+    # by one, the capture, replays, a new length, after a reinit) must equal
+    # the oracle
+    note("automatic launch: one by one, capture after 8 repeats, replay")
     g, o = make_pair(O.RASTRIGIN, 10, 5000, seed=77, launch=L.LAUNCH_AUTO)
-    for rnd, T in enumerate((9, 9, 9, 5, 5, 9)):
+    for rnd, T in enumerate([4] * 11 + [3, 3] + [4] * 10):
         g.run(T)
         o.run_sync(T)
         assert_same_state(g, o, f"run {rnd} (T={T})")
         assert_same_gbest(g, o, f"run {rnd} (T={T})")
-        if rnd == 3:
+        if rnd == 12:
             g.reinit()
             o.reinit()
     g.run(5)
