@@ -40,6 +40,10 @@ struct KernelSet {
   // (workgroups of 1024 particles).
   hipError_t (*qlock_step)(const SwarmDev&, const ProbArgs&, const StepArgs&, float w,
                            uint32_t k, hipStream_t);
+  // Queue-lock iterations tbase[0]+1 .. tbase[0]+T of a swarm of at most one
+  // such workgroup, in one launch.
+  hipError_t (*qlock_wg)(const SwarmDev&, const ProbArgs&, const StepArgs&, float w0,
+                         float wdamp, int32_t T, hipStream_t);
 };
 
 // nullptr when (obj, d) has no compiled kernels (constrained: no kernels that

@@ -681,6 +681,88 @@ __global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArg
   }
 }
 
+// Queue-lock iterations t = tbase[0] + 1 .. tbase[0] + T of a swarm that fits
+// one workgroup (n <= BLOCK): the reference's queue-lock kernel launched T
+// times with its one block, as one launch.  With one block nothing else
+// publishes, so the live global best is the block's own and lives in LDS
+// between iterations; each iteration is k_qlock's: every particle steps
+// against it, the particles whose best cost beats it are counted, and when at
+// least min_improvers did, the first of the lowest of them is published.  The
+// swarm stays in registers; state and record are written back at the end.
+template <int MAXD, bool EXACT, int OBJ, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_qlock_wg(SwarmDev S, ProbArgs P, StepArgs A, float w,
+                                                    float wdamp, int32_t T) {
+  __shared__ uint64_t s_red[BLOCK / 64];
+  __shared__ int s_cnt[BLOCK / 64];
+  __shared__ float s_gb[MAXD];
+  __shared__ float s_gcost;
+  __shared__ uint32_t s_gidx;
+  const uint32_t t0 = S.tbase[0];
+  const uint32_t epoch = S.tbase[1];
+  const int d = EXACT ? MAXD : P.d;
+  GbRec* R = S.gbrec;
+  if ((int)threadIdx.x < d) s_gb[threadIdx.x] = R->pos[threadIdx.x];
+  if (threadIdx.x == 0) {
+    s_gcost = R->cost;
+    s_gidx = R->idx;
+  }
+  const int64_t i = threadIdx.x;
+  const bool act = i < S.n;
+  float x[MAXD], v[MAXD], pb[MAXD], pc = 0.0f;
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) x[j] = v[j] = pb[j] = 0.0f;
+  if (act) {
+    load_vec<MAXD>(x, S.x, S.npad, i, d);
+    load_vec<MAXD>(v, S.v, S.npad, i, d);
+    load_vec<MAXD>(pb, S.pb, S.npad, i, d);
+    pc = S.pc[i];
+  }
+  __syncthreads();
+  for (int32_t k = 1; k <= T; ++k) {
+    const uint32_t t = t0 + (uint32_t)k;
+    float gb[MAXD];
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
+    const float gcost = s_gcost;
+    if (act) {
+      velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, epoch, S.key0,
+                                   S.key1, d, P);
+      const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2), pc);
+      if (c < pc) {
+#pragma unroll
+        for (int j = 0; j < MAXD; ++j) pb[j] = x[j];
+        pc = c;
+      }
+    }
+    const bool imp = act && (pc < gcost);
+    const int count = block_sum_i32<BLOCK>(imp ? 1 : 0, s_cnt);
+    const uint64_t mykey = imp ? make_key(pc, (uint32_t)i) : KEY_NONE;
+    const uint64_t bk = block_min_u64<BLOCK>(mykey, s_red);
+    // block_min_u64 ends with a barrier: every read of s_gb and s_gcost of
+    // this iteration is done before the winner overwrites them
+    if (count >= A.min_improvers && imp && mykey == bk) {
+#pragma unroll
+      for (int j = 0; j < MAXD; ++j)
+        if (j < d) s_gb[j] = pb[j];
+      s_gcost = pc;
+      s_gidx = (uint32_t)i;
+    }
+    __syncthreads();
+    w = w * wdamp;
+  }
+  if (act) {
+    store_vec<MAXD>(x, S.x, S.npad, i, d);
+    store_vec<MAXD>(v, S.v, S.npad, i, d);
+    store_vec<MAXD>(pb, S.pb, S.npad, i, d);
+    S.pc[i] = pc;
+  }
+  if ((int)threadIdx.x < d) R->pos[threadIdx.x] = s_gb[threadIdx.x];
+  if (threadIdx.x == 0) {
+    R->cost = s_gcost;
+    R->idx = s_gidx;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Launchers.
 constexpr int QBLK = 1024; // queue-lock block (the reference's workgroup size)
@@ -800,9 +882,16 @@ struct Launch {
                        P, A, w, k);
     return hipGetLastError();
   }
+  static hipError_t qlock_wg(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w0,
+                             float wdamp, int32_t T, hipStream_t st) {
+    if (!(S.n <= QBLK)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_qlock_wg<MAXD, EXACT, OBJ, QBLK>), dim3(1), dim3(QBLK), 0, st, S, P, A,
+                       w0, wdamp, T);
+    return hipGetLastError();
+  }
   static const KernelSet* set() {
-    static const KernelSet ks = {MAXD,       MAX_BLOCK,  &init,      &rekey,
-                                 &sync_step, &sync_wg,   &async_run, &qlock_step};
+    static const KernelSet ks = {MAXD,       MAX_BLOCK,  &init,      &rekey,    &sync_step,
+                                 &sync_wg,   &async_run, &qlock_step, &qlock_wg};
     return &ks;
   }
 };

@@ -699,6 +699,12 @@ int pps_swarm_run(pps_swarm* s, int32_t maxiters, float w, float wdamp) {
         HIPCHK(s->ks->async_run(s->S, s->P, A, w, wdamp, t0, T, st));
         break;
       case PPS_MODE_QLOCK: {
+        // One workgroup's worth of particles (one queue-lock block): the T
+        // iterations in one launch, the live global best in LDS.
+        if (s->prm.n <= 1024) {
+          HIPCHK(s->ks->qlock_wg(s->S, s->P, A, w, wdamp, T, st));
+          break;
+        }
         float wk = w;
         for (int32_t k = 1; k <= T; ++k) {
           HIPCHK(s->ks->qlock_step(s->S, s->P, A, wk, (uint32_t)k, st));
