@@ -139,6 +139,34 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t k) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t k) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)k, CTRL, ROW_MASK, 0xF, false);
+}
+
+// Lowest lane among those with `sel` whose `o` is the smallest over them
+// (wave-uniform; -1 when no lane has `sel`).  With o = cost_order(cost) and
+// lane = particle index this is the lane of the smallest make_key: the same
+// DPP minimum as wave_min_u64 on 32 bits, then the first lane of the ballot of
+// the lanes holding it.  All 64 lanes must be active.
+__device__ __forceinline__ int wave_argmin_first(uint32_t o, bool sel) {
+  uint32_t m = sel ? o : 0xFFFFFFFFu;
+  m = min(m, dpp_u32<0xB1, 0xF>(m));
+  m = min(m, dpp_u32<0x4E, 0xF>(m));
+  m = min(m, dpp_u32<0x124, 0xF>(m));
+  m = min(m, dpp_u32<0x128, 0xF>(m));
+  m = min(m, dpp_u32<0x142, 0xA>(m));
+  m = min(m, dpp_u32<0x143, 0xC>(m));
+  m = (uint32_t)__builtin_amdgcn_readlane((int)m, 63);
+  const uint64_t eq = __ballot(sel && o == m);
+  return eq ? (int)__builtin_ctzll(eq) : -1;
+}
+
+// Lane `l` (wave-uniform) of a float.
+__device__ __forceinline__ float read_lane(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
 // ---------------------------------------------------------------------------
 // cos(2 pi x), the same function on every device and in every build:
 //   r = x - rint(x)          exact, r in [-1/2, 1/2]

@@ -15,6 +15,7 @@
 //               [update_particle_states! sync, src/kernels.jl:127-194 and the
 //               per-iteration minimum of src/lowerlevel_solve.jl:23-33]
 //   k_sync_wg   the whole sync loop in one workgroup, swarm in registers
+//   k_sync_wave the same for a swarm of one wavefront (no LDS, no barrier)
 //   k_async     all iterations per particle in registers
 //               [update_particles_states_async!, src/kernels.jl:196-225]
 //   k_qlock     one iteration against the live global best, lock-protected
@@ -41,6 +42,20 @@ __device__ __forceinline__ uint64_t block_min_u64(uint64_t k, uint64_t* s_red) {
   uint64_t r = s_red[0];
 #pragma unroll
   for (int i = 1; i < BLOCK / 64; ++i) r = umin64(r, s_red[i]);
+  __syncthreads();
+  return r;
+}
+
+// Minimum over a workgroup of nw wavefronts (nw = blockDim.x / 64, uniform);
+// with one wavefront no barrier is taken.  Every thread receives the result.
+__device__ __forceinline__ uint64_t block_min_u64_n(uint64_t k, uint64_t* s_red, int nw) {
+  k = wave_min_u64(k);
+  if (nw == 1) return k;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) s_red[wid] = k;
+  __syncthreads();
+  uint64_t r = s_red[0];
+  for (int i = 1; i < nw; ++i) r = umin64(r, s_red[i]);
   __syncthreads();
   return r;
 }
@@ -449,13 +464,15 @@ __global__ __launch_bounds__(128) void k_sync_step_pair(SwarmDev S, ProbArgs P, 
   if (tid == 0) S.bkey[(int64_t)(t & 1u) * G + blockIdx.x] = bk;
 }
 
-// Whole sync loop in one workgroup of BLOCK threads, one particle per thread
-// held in registers.  The reduction slots alternate by iteration parity, so a
-// slot is rewritten only after the barrier of the following iteration.  For
-// MAXD <= 8 every thread also writes its candidate position to a parity
-// buffer in LDS before the reduction and every thread reads the winner's row
-// after it: one barrier per iteration.  Wider problems hand the winner's
-// position over through s_gb (two barriers).
+// Whole sync loop in one workgroup of nw = ceil(n / 64) wavefronts (at most
+// BLOCK threads), one particle per thread held in registers.  The reduction
+// slots alternate by iteration parity, so a slot is rewritten only after the
+// barrier of the following iteration.  For MAXD <= 8 every thread also writes
+// its candidate position to a parity buffer in LDS before the reduction and
+// every thread reads the winner's row after it: one barrier per iteration.
+// Wider problems hand the winner's position over through s_gb (two barriers).
+// Swarms of one wavefront run k_sync_wave instead (a one-wavefront branch in
+// this loop, even untaken, slowed 16-wavefront swarms by up to a third).
 template <int MAXD, bool EXACT, int OBJ, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepArgs A,
                                                    float w, float wdamp, uint32_t t0,
@@ -466,10 +483,14 @@ __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepA
   __shared__ float s_gb[MAXD];
   __shared__ float s_pos[POS_LDS ? 2 : 1][POS_LDS ? MAXD : 1][POS_LDS ? BLOCK : 1];
   const int d = EXACT ? MAXD : P.d;
-  fetch_gbest<BLOCK>(S, t0 & 1u, d, s_gb, s_red[0]);
+  const int nw = (int)(blockDim.x >> 6);
   const int64_t i = threadIdx.x;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const bool act = i < S.n;
+  // Global best handed over in slot (t0 & 1) of the one workgroup.
+  float gb[MAXD];
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? S.cand[(int64_t)(t0 & 1u) * d + j] : 0.0f;
   float x[MAXD], v[MAXD], pb[MAXD], pc = 0.0f;
 #pragma unroll
   for (int j = 0; j < MAXD; ++j) x[j] = v[j] = pb[j] = 0.0f;
@@ -480,9 +501,13 @@ __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepA
     pc = S.pc[i];
   }
   const bool cur = A.gbest_from == PPS_GBEST_CURRENT;
-  float gb[MAXD];
-#pragma unroll
-  for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
+  // Slots of wavefronts not launched hold KEY_NONE, so every iteration reads
+  // all NW slots in one unrolled round of LDS loads.
+  if (nw > 1) {
+    if ((int)threadIdx.x < 2 * NW && (int)threadIdx.x % NW >= nw)
+      s_red[threadIdx.x / NW][threadIdx.x % NW] = KEY_NONE;
+    __syncthreads();
+  }
   for (int32_t k = 1; k <= T; ++k) {
     const int b = k & 1;
     uint64_t mykey = KEY_NONE;
@@ -533,8 +558,64 @@ __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepA
   // Hand-over state for whatever runs next (parity of t0 + T): the last
   // winner's position (held by every thread in gb) and its key.  The
   // reduction uses the slot set the last iteration did not.
-  const uint64_t bk = block_min_u64<BLOCK>(act ? make_key(pc, (uint32_t)i) : KEY_NONE,
-                                           s_red[(T + 1) & 1]);
+  const uint64_t bk = block_min_u64_n(act ? make_key(pc, (uint32_t)i) : KEY_NONE,
+                                      s_red[(T + 1) & 1], nw);
+  const uint32_t par = (t0 + (uint32_t)T) & 1u;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j)
+      if (j < d) S.cand[(int64_t)par * d + j] = gb[j];
+    S.bkey[par] = bk;
+  }
+}
+
+// Whole sync loop of a swarm of at most 64 particles in one wavefront, one
+// particle per lane held in registers: each iteration's winner (smallest
+// best cost, then lowest index) comes from a DPP minimum and a ballot, and
+// its position from the winner's lane, so nothing goes through LDS and no
+// barrier is taken.  Hand-over state as k_sync_wg.
+template <int MAXD, bool EXACT, int OBJ>
+__global__ __launch_bounds__(64) void k_sync_wave(SwarmDev S, ProbArgs P, StepArgs A, float w,
+                                                  float wdamp, uint32_t t0, int32_t T) {
+  const int d = EXACT ? MAXD : P.d;
+  const int64_t i = threadIdx.x;
+  const bool act = i < S.n;
+  float gb[MAXD];
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? S.cand[(int64_t)(t0 & 1u) * d + j] : 0.0f;
+  float x[MAXD], v[MAXD], pb[MAXD], pc = 0.0f;
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) x[j] = v[j] = pb[j] = 0.0f;
+  if (act) {
+    load_vec<MAXD>(x, S.x, S.npad, i, d);
+    load_vec<MAXD>(v, S.v, S.npad, i, d);
+    load_vec<MAXD>(pb, S.pb, S.npad, i, d);
+    pc = S.pc[i];
+  }
+  const bool cur = A.gbest_from == PPS_GBEST_CURRENT;
+  for (int32_t k = 1; k <= T; ++k) {
+    if (act) {
+      velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
+                                   S.epoch, S.key0, S.key1, d, P);
+      const float c = cost_at<MAXD, EXACT, OBJ>(x, d, P, (float)(i + 2), pc);
+      if (c < pc) {
+#pragma unroll
+        for (int j = 0; j < MAXD; ++j) pb[j] = x[j];
+        pc = c;
+      }
+    }
+    const int wi = wave_argmin_first(cost_order(pc), act);
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? read_lane(cur ? x[j] : pb[j], wi) : 0.0f;
+    w = w * wdamp;
+  }
+  if (act) {
+    store_vec<MAXD>(x, S.x, S.npad, i, d);
+    store_vec<MAXD>(v, S.v, S.npad, i, d);
+    store_vec<MAXD>(pb, S.pb, S.npad, i, d);
+    S.pc[i] = pc;
+  }
+  const uint64_t bk = wave_min_u64(act ? make_key(pc, (uint32_t)i) : KEY_NONE);
   const uint32_t par = (t0 + (uint32_t)T) & 1u;
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -683,29 +764,43 @@ __global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArg
 
 // Queue-lock iterations t = tbase[0] + 1 .. tbase[0] + T of a swarm that fits
 // one workgroup (n <= BLOCK): the reference's queue-lock kernel launched T
-// times with its one block, as one launch.  With one block nothing else
-// publishes, so the live global best is the block's own and lives in LDS
-// between iterations; each iteration is k_qlock's: every particle steps
-// against it, the particles whose best cost beats it are counted, and when at
-// least min_improvers did, the first of the lowest of them is published.  The
-// swarm stays in registers; state and record are written back at the end.
+// times with its one block, as one launch of nw = ceil(n / 64) wavefronts.
+// With one block nothing else publishes, so the live global best is the
+// block's own and stays on chip between iterations (in every thread's
+// registers below 16 coordinates, else in LDS, re-read each iteration: the
+// 128-register budget of a 1024-thread workgroup); each iteration is
+// k_qlock's: every particle steps against it, the particles whose best cost
+// beats it are counted, and when at least min_improvers did, the first of the
+// lowest of them is published.  One wavefront counts with a ballot and takes
+// the winner's row from its lane (no barrier); more exchange per-wavefront
+// counts and keys through LDS (two barriers: the second orders the winner's
+// LDS writes before the reads and every read of this iteration's slots before
+// the next iteration's writes).  The swarm stays in registers; state and
+// record are written back at the end.
 template <int MAXD, bool EXACT, int OBJ, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_qlock_wg(SwarmDev S, ProbArgs P, StepArgs A, float w,
                                                     float wdamp, int32_t T) {
+  constexpr bool GB_REG = MAXD < 16;
   __shared__ uint64_t s_red[BLOCK / 64];
   __shared__ int s_cnt[BLOCK / 64];
   __shared__ float s_gb[MAXD];
   __shared__ float s_gcost;
-  __shared__ uint32_t s_gidx;
   const uint32_t t0 = S.tbase[0];
   const uint32_t epoch = S.tbase[1];
   const int d = EXACT ? MAXD : P.d;
+  const int nw = (int)(blockDim.x >> 6);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   GbRec* R = S.gbrec;
-  if ((int)threadIdx.x < d) s_gb[threadIdx.x] = R->pos[threadIdx.x];
-  if (threadIdx.x == 0) {
-    s_gcost = R->cost;
-    s_gidx = R->idx;
+  float gb[MAXD];
+  if constexpr (GB_REG) {
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? R->pos[j] : 0.0f;
+  } else {
+    for (int j = threadIdx.x; j < d; j += blockDim.x) s_gb[j] = R->pos[j];
+    __syncthreads();
   }
+  float gcost = R->cost;
+  uint32_t gidx = R->idx;
   const int64_t i = threadIdx.x;
   const bool act = i < S.n;
   float x[MAXD], v[MAXD], pb[MAXD], pc = 0.0f;
@@ -717,13 +812,21 @@ __global__ __launch_bounds__(BLOCK) void k_qlock_wg(SwarmDev S, ProbArgs P, Step
     load_vec<MAXD>(pb, S.pb, S.npad, i, d);
     pc = S.pc[i];
   }
-  __syncthreads();
+  // Slots of wavefronts not launched hold KEY_NONE and 0, so every iteration
+  // reads all slots in one unrolled round of LDS loads.
+  if (nw > 1) {
+    if ((int)threadIdx.x < BLOCK / 64 && (int)threadIdx.x >= nw) {
+      s_red[threadIdx.x] = KEY_NONE;
+      s_cnt[threadIdx.x] = 0;
+    }
+    __syncthreads();
+  }
   for (int32_t k = 1; k <= T; ++k) {
     const uint32_t t = t0 + (uint32_t)k;
-    float gb[MAXD];
+    if constexpr (!GB_REG) {
 #pragma unroll
-    for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
-    const float gcost = s_gcost;
+      for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
+    }
     if (act) {
       velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t, epoch, S.key0,
                                    S.key1, d, P);
@@ -734,21 +837,58 @@ __global__ __launch_bounds__(BLOCK) void k_qlock_wg(SwarmDev S, ProbArgs P, Step
         pc = c;
       }
     }
+    w = w * wdamp;
     const bool imp = act && (pc < gcost);
-    const int count = block_sum_i32<BLOCK>(imp ? 1 : 0, s_cnt);
+    const uint64_t bal = __ballot(imp);
+    // one wavefront: count with a ballot, take the winner from its lane
+    if (nw == 1) {
+      const int count = (int)__popcll(bal);
+      if (count > 0 && count >= A.min_improvers) {
+        const int wi = wave_argmin_first(cost_order(pc), imp);
+        if constexpr (GB_REG) {
+#pragma unroll
+          for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? read_lane(pb[j], wi) : 0.0f;
+        } else if (lane == wi) {
+          // this wavefront's next reads of s_gb follow in program order
+#pragma unroll
+          for (int j = 0; j < MAXD; ++j)
+            if (j < d) s_gb[j] = pb[j];
+        }
+        gcost = read_lane(pc, wi);
+        gidx = (uint32_t)wi;
+      }
+      continue;
+    }
     const uint64_t mykey = imp ? make_key(pc, (uint32_t)i) : KEY_NONE;
-    const uint64_t bk = block_min_u64<BLOCK>(mykey, s_red);
-    // block_min_u64 ends with a barrier: every read of s_gb and s_gcost of
-    // this iteration is done before the winner overwrites them
-    if (count >= A.min_improvers && imp && mykey == bk) {
+    const uint64_t wk = wave_min_u64(mykey);
+    if (lane == 0) {
+      s_red[wid] = wk;
+      s_cnt[wid] = (int)__popcll(bal);
+    }
+    __syncthreads();
+    uint64_t r = s_red[0];
+    int count = s_cnt[0];
+#pragma unroll
+    for (int q = 1; q < BLOCK / 64; ++q) {
+      r = umin64(r, s_red[q]);
+      count += s_cnt[q];
+    }
+    const bool pub = count > 0 && count >= A.min_improvers;
+    if (pub && imp && mykey == r) {
 #pragma unroll
       for (int j = 0; j < MAXD; ++j)
         if (j < d) s_gb[j] = pb[j];
       s_gcost = pc;
-      s_gidx = (uint32_t)i;
     }
     __syncthreads();
-    w = w * wdamp;
+    if (pub) {
+      if constexpr (GB_REG) {
+#pragma unroll
+        for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
+      }
+      gcost = s_gcost;
+      gidx = (uint32_t)r;
+    }
   }
   if (act) {
     store_vec<MAXD>(x, S.x, S.npad, i, d);
@@ -756,10 +896,12 @@ __global__ __launch_bounds__(BLOCK) void k_qlock_wg(SwarmDev S, ProbArgs P, Step
     store_vec<MAXD>(pb, S.pb, S.npad, i, d);
     S.pc[i] = pc;
   }
-  if ((int)threadIdx.x < d) R->pos[threadIdx.x] = s_gb[threadIdx.x];
   if (threadIdx.x == 0) {
-    R->cost = s_gcost;
-    R->idx = s_gidx;
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j)
+      if (j < d) R->pos[j] = GB_REG ? gb[j] : s_gb[j];
+    R->cost = gcost;
+    R->idx = gidx;
   }
 }
 
@@ -767,6 +909,8 @@ __global__ __launch_bounds__(BLOCK) void k_qlock_wg(SwarmDev S, ProbArgs P, Step
 // Launchers.
 constexpr int QBLK = 1024; // queue-lock block (the reference's workgroup size)
 constexpr int WGB = 1024;  // single-workgroup sync loop
+// Wavefronts launched for a one-workgroup swarm of n particles.
+inline unsigned waves_of(int64_t n) { return (unsigned)((n + 63) / 64); }
 
 template <int MAXD, bool EXACT, int OBJ>
 struct Launch {
@@ -861,8 +1005,12 @@ struct Launch {
   static hipError_t sync_wg(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w0,
                             float wdamp, uint32_t t0, int32_t T, hipStream_t st) {
     if (!(S.G == 1 && S.block == WGB && S.n <= WGB)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_sync_wg<MAXD, EXACT, OBJ, WGB>), dim3(1), dim3(WGB), 0, st, S, P, A, w0,
-                       wdamp, t0, T);
+    if (S.n <= 64)
+      hipLaunchKernelGGL((k_sync_wave<MAXD, EXACT, OBJ>), dim3(1), dim3(64), 0, st, S, P, A, w0,
+                         wdamp, t0, T);
+    else
+      hipLaunchKernelGGL((k_sync_wg<MAXD, EXACT, OBJ, WGB>), dim3(1), dim3(waves_of(S.n) * 64),
+                         0, st, S, P, A, w0, wdamp, t0, T);
     return hipGetLastError();
   }
   static hipError_t async_run(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w0,
@@ -885,8 +1033,8 @@ struct Launch {
   static hipError_t qlock_wg(const SwarmDev& S, const ProbArgs& P, const StepArgs& A, float w0,
                              float wdamp, int32_t T, hipStream_t st) {
     if (!(S.n <= QBLK)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_qlock_wg<MAXD, EXACT, OBJ, QBLK>), dim3(1), dim3(QBLK), 0, st, S, P, A,
-                       w0, wdamp, T);
+    hipLaunchKernelGGL((k_qlock_wg<MAXD, EXACT, OBJ, QBLK>), dim3(1), dim3(waves_of(S.n) * 64), 0,
+                       st, S, P, A, w0, wdamp, T);
     return hipGetLastError();
   }
   static const KernelSet* set() {

@@ -168,6 +168,77 @@ def test_qlock_single_block_matches_oracle(min_improvers):
         assert_same_gbest(g, o)
 
 
+ONE_WG = ([(L.MODE_SYNC, O.ROSENBROCK, 10, n) for n in (1, 8, 37, 64, 65, 130, 1024)]
+          + [(L.MODE_SYNC, O.RASTRIGIN, 3, 50), (L.MODE_SYNC, O.SPHERE, 32, 64),
+             (L.MODE_SYNC, O.RASTRIGIN, 45, 40), (L.MODE_SYNC, O.SPHERE, 8, 300)]
+          + [(L.MODE_QLOCK, O.ROSENBROCK, 10, n) for n in (1, 8, 64, 65, 200)]
+          + [(L.MODE_QLOCK, O.SPHERE, 32, 60), (L.MODE_QLOCK, O.RASTRIGIN, 45, 33),
+             (L.MODE_QLOCK, O.RASTRIGIN, 45, 300), (L.MODE_QLOCK, O.SPHERE, 16, 700)])
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("mode,obj,d,n", ONE_WG)
+def test_one_workgroup_sizes_match_oracle(mode, obj, d, n, variant):
+    # a swarm of at most 1024 particles runs all iterations in one workgroup
+    # of ceil(n / 64) wavefronts; one wavefront takes each iteration's winner
+    # from its lane, more exchange keys (and counts) through LDS.  variant:
+    # the global best's source (sync) or the publication threshold (queue-lock)
+    qlock = mode == L.MODE_QLOCK
+    gf = (L.GBEST_PBEST, L.GBEST_CURRENT)[variant]
+    mi = (2, 1)[variant]
+    note(f"one workgroup mode={mode} obj={obj} d={d} n={n} variant={variant}")
+    g, o = make_pair(obj, d, n, seed=31 + n + d, mode=mode, gbest_from=0 if qlock else gf,
+                     min_improvers=mi,
+                     launch=L.LAUNCH_AUTO if qlock else L.LAUNCH_WORKGROUP)
+    done = 0
+    for chunk in (1, 7, 52):
+        g.run(chunk, 0.7298, 0.995)
+        if qlock:
+            o.run_qlock(chunk, 0.7298, 0.995, min_improvers=mi)
+        else:
+            o.run_sync(chunk, 0.7298, 0.995, gbest_from=gf)
+        done += chunk
+        assert_same_state(g, o, f"after {done} iterations")
+        assert_same_gbest(g, o, f"after {done} iterations")
+
+
+@pytest.mark.parametrize("mode", [L.MODE_SYNC, L.MODE_QLOCK])
+@pytest.mark.parametrize("n", [37, 64, 200])
+def test_one_workgroup_in_loop_argmin_ties(mode, n):
+    # the winner an iteration hands to the next: ties go to the lowest index
+    # (also +0.0 against -0.0); Sphere costs are >= 0, so with these best
+    # costs no particle improves and the tie decides both the next
+    # iteration's attractor and the final record
+    qlock = mode == L.MODE_QLOCK
+    d = 3
+    note(f"one workgroup ties mode={mode} n={n}")
+    g, o = make_pair(O.SPHERE, d, n, seed=11, mode=mode, lb=-1.0, ub=1.0,
+                     launch=L.LAUNCH_AUTO if qlock else L.LAUNCH_WORKGROUP)
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, (d, n)).astype(np.float32)
+    v = rng.normal(0, 0.3, (d, n)).astype(np.float32)
+    b = rng.uniform(-1, 1, (d, n)).astype(np.float32)
+    for lo, hi, clo, chi, rest in [(5, 33, -1.0, -1.0, -0.5), (0, n - 1, -1.0, -1.0, -0.5),
+                                   (n // 2, n - 1, 0.0, -0.0, 1e-30),
+                                   (n - 2, n - 1, -0.0, 0.0, 1e-30)]:
+        bc = np.full(n, rest, np.float32)
+        bc[lo] = clo
+        bc[hi] = chi
+        g.set_state(x, v, b, bc, 3)
+        o.set_state(x, v, b, bc, 3)
+        gb = rng.uniform(-1, 1, d).astype(np.float32)
+        g.set_gbest(gb, 1.0)
+        o.set_gbest(gb, 1.0)
+        g.run(2, 0.5, 1.0)
+        if qlock:
+            o.run_qlock(2, 0.5, 1.0)
+        else:
+            o.run_sync(2, 0.5, 1.0)
+        assert_same_state(g, o, f"ties at {lo}, {hi}")
+        assert_same_gbest(g, o, f"ties at {lo}, {hi}")
+        assert g.gbest().index == lo
+
+
 def test_qlock_multiblock_invariants():
     # several workgroups race on the live global best, as in the reference:
     # check what every schedule guarantees
