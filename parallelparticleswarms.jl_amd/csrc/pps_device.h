@@ -149,8 +149,8 @@ __device__ __forceinline__ uint32_t dpp_u32(uint32_t k) {
 // lane = particle index this is the lane of the smallest make_key: the same
 // DPP minimum as wave_min_u64 on 32 bits, then the first lane of the ballot of
 // the lanes holding it.  All 64 lanes must be active.
-__device__ __forceinline__ int wave_argmin_first(uint32_t o, bool sel) {
-  uint32_t m = sel ? o : 0xFFFFFFFFu;
+__device__ __forceinline__ int wave_argmin_first(uint32_t o, bool sel, uint32_t& m) {
+  m = sel ? o : 0xFFFFFFFFu;
   m = min(m, dpp_u32<0xB1, 0xF>(m));
   m = min(m, dpp_u32<0x4E, 0xF>(m));
   m = min(m, dpp_u32<0x124, 0xF>(m));
@@ -160,6 +160,44 @@ __device__ __forceinline__ int wave_argmin_first(uint32_t o, bool sel) {
   m = (uint32_t)__builtin_amdgcn_readlane((int)m, 63);
   const uint64_t eq = __ballot(sel && o == m);
   return eq ? (int)__builtin_ctzll(eq) : -1;
+}
+__device__ __forceinline__ int wave_argmin_first(uint32_t o, bool sel) {
+  uint32_t m;
+  return wave_argmin_first(o, sel, m);
+}
+
+// Smallest of 16 keys key = order << 32 | index held in slots[0..15] (one per
+// wavefront of a workgroup, ~0 for wavefronts not launched), ties to the
+// lowest slot: every lane reads slot lane % 16, the DPP minimum of the orders
+// over each 16-lane row (all rows hold the same slots, so every lane ends
+// with it), then the first lane holding it.  Slot q's keys carry the lowest
+// indices of wavefront q, so the lowest slot among equal orders holds the
+// lowest index.  Wave-uniform; all 64 lanes must be active.
+__device__ __forceinline__ uint64_t slots16_min(const uint64_t* slots) {
+  const uint64_t k = slots[threadIdx.x & 15];
+  const uint32_t o = (uint32_t)(k >> 32);
+  uint32_t m = o;
+  m = min(m, dpp_u32<0xB1, 0xF>(m));
+  m = min(m, dpp_u32<0x4E, 0xF>(m));
+  m = min(m, dpp_u32<0x124, 0xF>(m));
+  m = min(m, dpp_u32<0x128, 0xF>(m));
+  m = (uint32_t)__builtin_amdgcn_readfirstlane((int)m);
+  const uint64_t eq = __ballot(o == m) & 0xFFFFull;
+  const int q = (int)__builtin_ctzll(eq);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)k, q);
+  return ((uint64_t)m << 32) | lo;
+}
+
+// Sum of 16 counts in slots[0..15], the same row-wise exchange: pairs, quads,
+// then quads 4 and 8 lanes apart (disjoint sets, each count added once).
+// Wave-uniform; all 64 lanes must be active.
+__device__ __forceinline__ int slots16_sum(const int* slots) {
+  int c = slots[threadIdx.x & 15];
+  c += __builtin_amdgcn_update_dpp(0, c, 0xB1, 0xF, 0xF, false);
+  c += __builtin_amdgcn_update_dpp(0, c, 0x4E, 0xF, 0xF, false);
+  c += __builtin_amdgcn_update_dpp(0, c, 0x124, 0xF, 0xF, false);
+  c += __builtin_amdgcn_update_dpp(0, c, 0x128, 0xF, 0xF, false);
+  return __builtin_amdgcn_readfirstlane(c);
 }
 
 // Lane `l` (wave-uniform) of a float.

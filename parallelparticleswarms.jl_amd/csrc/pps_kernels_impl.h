@@ -478,6 +478,7 @@ __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepA
                                                    float w, float wdamp, uint32_t t0,
                                                    int32_t T) {
   constexpr int NW = BLOCK / 64;
+  static_assert(NW == 16, "the slot reduction reads 16 slots");
   constexpr bool POS_LDS = MAXD <= 8;
   __shared__ uint64_t s_red[2][NW];
   __shared__ float s_gb[MAXD];
@@ -510,7 +511,6 @@ __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepA
   }
   for (int32_t k = 1; k <= T; ++k) {
     const int b = k & 1;
-    uint64_t mykey = KEY_NONE;
     if (act) {
       velocity_position<MAXD, OBJ>(x, v, pb, gb, w, A.c1, A.c2, (uint32_t)i, t0 + (uint32_t)k,
                                    S.epoch, S.key0, S.key1, d, P);
@@ -520,25 +520,26 @@ __global__ __launch_bounds__(BLOCK) void k_sync_wg(SwarmDev S, ProbArgs P, StepA
         for (int j = 0; j < MAXD; ++j) pb[j] = x[j];
         pc = c;
       }
-      mykey = make_key(pc, (uint32_t)i);
       if constexpr (POS_LDS) {
 #pragma unroll
         for (int j = 0; j < MAXD; ++j)
           if (j < d) s_pos[b][j][i] = cur ? x[j] : pb[j];
       }
     }
-    const uint64_t wk = wave_min_u64(mykey);
-    if (lane == 0) s_red[b][wid] = wk;
+    // The wavefront's winner as a key (32-bit DPP minimum of the cost order
+    // and a ballot), then the workgroup's from the 16 slots.
+    uint32_t om;
+    const int wl = wave_argmin_first(cost_order(pc), act, om);
+    if (lane == 0)
+      s_red[b][wid] = wl < 0 ? KEY_NONE : ((uint64_t)om << 32) | (uint32_t)(wid * 64 + wl);
     __syncthreads();
-    uint64_t r = s_red[b][0];
-#pragma unroll
-    for (int q = 1; q < NW; ++q) r = umin64(r, s_red[b][q]);
+    const uint64_t r = slots16_min(s_red[b]);
+    const uint32_t wi = (uint32_t)r;
     if constexpr (POS_LDS) {
-      const uint32_t wi = (uint32_t)r;
 #pragma unroll
       for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_pos[b][j][wi] : 0.0f;
     } else {
-      if (act && mykey == r) {
+      if (act && (uint32_t)i == wi) {
 #pragma unroll
         for (int j = 0; j < MAXD; ++j)
           if (j < d) s_gb[j] = cur ? x[j] : pb[j];
@@ -781,6 +782,7 @@ template <int MAXD, bool EXACT, int OBJ, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_qlock_wg(SwarmDev S, ProbArgs P, StepArgs A, float w,
                                                     float wdamp, int32_t T) {
   constexpr bool GB_REG = MAXD < 16;
+  static_assert(BLOCK == 1024, "the slot reductions read 16 slots");
   __shared__ uint64_t s_red[BLOCK / 64];
   __shared__ int s_cnt[BLOCK / 64];
   __shared__ float s_gb[MAXD];
@@ -859,22 +861,19 @@ __global__ __launch_bounds__(BLOCK) void k_qlock_wg(SwarmDev S, ProbArgs P, Step
       }
       continue;
     }
-    const uint64_t mykey = imp ? make_key(pc, (uint32_t)i) : KEY_NONE;
-    const uint64_t wk = wave_min_u64(mykey);
+    // each wavefront's count and first lowest improver, then the
+    // workgroup's from the 16 slots
+    uint32_t om;
+    const int wl = wave_argmin_first(cost_order(pc), imp, om);
     if (lane == 0) {
-      s_red[wid] = wk;
+      s_red[wid] = wl < 0 ? KEY_NONE : ((uint64_t)om << 32) | (uint32_t)(wid * 64 + wl);
       s_cnt[wid] = (int)__popcll(bal);
     }
     __syncthreads();
-    uint64_t r = s_red[0];
-    int count = s_cnt[0];
-#pragma unroll
-    for (int q = 1; q < BLOCK / 64; ++q) {
-      r = umin64(r, s_red[q]);
-      count += s_cnt[q];
-    }
+    const uint64_t r = slots16_min(s_red);
+    const int count = slots16_sum(s_cnt);
     const bool pub = count > 0 && count >= A.min_improvers;
-    if (pub && imp && mykey == r) {
+    if (pub && (uint32_t)i == (uint32_t)r) {
 #pragma unroll
       for (int j = 0; j < MAXD; ++j)
         if (j < d) s_gb[j] = pb[j];
