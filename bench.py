@@ -6,7 +6,7 @@ single-GPU benchmark shape — 10-D Rastrigin and 10-D Rosenbrock (p = [1, 100])
 bounds [-5.12, 5.12], 2^20 particles each, 1000 iterations each, FP32 SoA state
 resident in HBM, the launch shape chosen by the library's autotuner (untimed,
 before the warm-up) and the iterations launched as the library chooses at this
-size (one launch each; a captured graph below 2^19 particles).  `value`
+size (one launch each; replayed chunk graphs below 2^17 particles).  `value`
 is particle-iterations per second over the timed steps (init excluded, as in
 the reference's benchmarks/CPU_vs_GPU/benchmark.jl:107-121).
 
@@ -352,8 +352,9 @@ def ode_c3(P, L, clock_khz):
 
 def n_sweep(P, L):
     """Sync step at 2^10..2^24 particles (10-D, both objectives): average launch
-    time over a timed run (a graph replay below 2^19 particles, launches one by
-    one from there) after 1000 warm-up iterations (the
+    time over a timed run (launched as the library's automatic mode chooses:
+    replayed chunk graphs below 2^17 particles, one by one from there) after
+    1000 warm-up iterations (the
     headline's timed range also starts after 1000 iterations).  The
     state is 124 B per particle, so it outgrows the 256 MB Infinity Cache
     from 2^22 particles on; below that the achieved rate includes cache hits."""
@@ -363,12 +364,9 @@ def n_sweep(P, L):
         iters = int(max(30, min(1000, 4e9 / (n * BYTES_PER_PARTICLE_ITER))))
         us = []
         for obj, seed in (("rastrigin", 11), ("rosenbrock", 12)):
-            # a captured graph below 2^19 particles (as the library's automatic
-            # mode replays for repeated runs), launches one by one from 2^19
-            launch = L.LAUNCH_GRAPH if 1024 < n < (1 << 19) else L.LAUNCH_AUTO
-            s = make_swarm(P, obj, n, D, seed, L.MODE_SYNC, launch)
-            s.run(1000)  # past the early phase where most particles still improve
-            s.run(iters)  # captures the graph of this length (its first launch uploads it)
+            s = make_swarm(P, obj, n, D, seed, L.MODE_SYNC, L.LAUNCH_AUTO)
+            s.run(1000)  # past the early phase (and the chunk graph's capture)
+            s.run(iters)
             s.run(iters)
             us.append(s.elapsed_ms() * 1e3 / iters)
             s.free()
@@ -530,18 +528,22 @@ def main():
             prob = P.OptimizationProblem(f, np.zeros(D, np.float32),
                                          None if obj == "rastrigin" else [1.0, 100.0],
                                          lb=np.full(D, -5.12), ub=np.full(D, 5.12))
-            g, b, us = P.autotune_sync(prob, N, iters=50)
+            g, b, us = P.autotune_sync(prob, N, iters=200)
             tuned[obj] = {"grid_per_cu": g, "particles_per_workgroup": b, "us_per_iter": us}
         log(f"[bench] autotuned: {tuned}")
     hl = L.LAUNCH_EAGER if args.eager else L.LAUNCH_AUTO
     swarms = [make_swarm(P, "rastrigin", N, D, 101, L.MODE_SYNC, hl, shape),
               make_swarm(P, "rosenbrock", N, D, 202, L.MODE_SYNC, hl, shape)]
 
-    def step():
+    per_swarm_ms = [0.0, 0.0]
+
+    def step(timed=False):
         ms = 0.0
-        for s in swarms:
+        for k, s in enumerate(swarms):
             s.run(T)
             ms += s.elapsed_ms()
+            if timed:
+                per_swarm_ms[k] += s.elapsed_ms()
         return ms
 
     for i in range(args.warmup):
@@ -550,7 +552,7 @@ def main():
     dev_ms = 0.0
     t0 = time.perf_counter()
     for i in range(args.steps):
-        dev_ms += step()
+        dev_ms += step(timed=True)
         log(f"[bench] step {i + 1}/{args.steps} done")
     wall = time.perf_counter() - t0
     pi_per_step = 2 * N * T
@@ -567,15 +569,18 @@ def main():
         "config": {"workload": "plain ParallelSyncPSOKernel, 10-D Rastrigin + 10-D Rosenbrock "
                                "(p=[1,100]), bounds [-5.12,5.12], 2^20 particles each, "
                                "1000 iterations each per step, FP32 SoA, "
-                               + "one launch per iteration (the library's choice from 2^19 "
-                                 "particles; a captured graph below)",
+                               + "one launch per iteration (the library's choice from 2^17 "
+                                 "particles at d > 4; replayed chunk graphs below)",
                    "particles": N, "dim": D, "iterations": T, "problems_per_step": 2,
                    "launch": tuned or "default (256 particles per workgroup, 2 per CU)"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic_from_profiles(), "kernel": "k_sync_step",
                      "bytes_per_launch": bytes_per_launch,
-                     "avg_launch_us": avg_launch_s * 1e6},
+                     "avg_launch_us": avg_launch_s * 1e6,
+                     "avg_launch_us_by_problem": {
+                         name: per_swarm_ms[k] * 1e3 / (T * args.steps)
+                         for k, name in enumerate(("rastrigin", "rosenbrock"))}},
     }
     for s in swarms:
         s.free()

@@ -116,12 +116,14 @@ enum {
 
 /* Launch strategy for PPS_MODE_SYNC. */
 enum {
-  PPS_LAUNCH_AUTO = 0,       /* WORKGROUP up to 1024 particles, GRAPH up to
-                                2^19 (captured once 8 runs in a row have used
-                                the same parameters; EAGER before), EAGER from
-                                2^19 particles                             */
+  PPS_LAUNCH_AUTO = 0,       /* WORKGROUP up to 1024 particles, GRAPH below
+                                2^17 (below 2^19 for d <= 4; a swarm's first
+                                capture waits for a run of 128 iterations,
+                                EAGER before), EAGER above                 */
   PPS_LAUNCH_EAGER = 1,      /* one launch per iteration, no graph            */
-  PPS_LAUNCH_GRAPH = 2,      /* iteration loop captured in a hipGraph         */
+  PPS_LAUNCH_GRAPH = 2,      /* chunks of 32 iterations replayed from a graph
+                                captured once per swarm (weights from a
+                                per-run table), the rest one by one        */
   PPS_LAUNCH_WORKGROUP = 3   /* whole loop in one workgroup (small swarms)    */
 };
 

@@ -10,6 +10,9 @@ struct StepArgs {
   float c1, c2;
   int32_t gbest_from;  // PPS_GBEST_*
   int32_t min_improvers;
+  // Nonzero in the launches of a replayed chunk graph: launch k takes its
+  // inertia weight from S.wtab[tbase[2] + k - 1] instead of its w argument.
+  int32_t w_table;
 };
 
 // One set of launchers per (objective, dimension bucket).
@@ -54,7 +57,11 @@ const KernelSet* find_kernels(int obj, int d, bool constrained = false);
 hipError_t launch_finalize(const SwarmDev&, int d, uint32_t par, hipStream_t);
 // p[0] = v0, p[1] = v1 on the stream (a one-lane kernel, so the write can sit
 // in a graph).
-hipError_t launch_set_u32x2(uint32_t* p, uint32_t v0, uint32_t v1, hipStream_t);
+// Stream position of a run: tbase = {t0, epoch, 0}.
+hipError_t launch_set_tbase(uint32_t* tbase, uint32_t t0, uint32_t epoch, hipStream_t);
+// End of a chunk of K launches: tbase[0] += K (iteration base), tbase[2] += K
+// (the weight table's offset).
+hipError_t launch_advance_tbase(uint32_t* tbase, uint32_t K, hipStream_t);
 // Write gbrec and a single-candidate parity `par` from host values.
 hipError_t launch_set_gbest(const SwarmDev&, int d, uint32_t par, const float* pos_dev,
                             float cost, hipStream_t);

@@ -57,15 +57,28 @@ hipError_t launch_finalize(const SwarmDev& S, int d, uint32_t par, hipStream_t s
   return hipGetLastError();
 }
 
-__global__ void k_set_u32x2(uint32_t* p, uint32_t v0, uint32_t v1) {
+__global__ void k_set_tbase(uint32_t* p, uint32_t t0, uint32_t epoch) {
   if (threadIdx.x == 0) {
-    p[0] = v0;
-    p[1] = v1;
+    p[0] = t0;
+    p[1] = epoch;
+    p[2] = 0u;
   }
 }
 
-hipError_t launch_set_u32x2(uint32_t* p, uint32_t v0, uint32_t v1, hipStream_t st) {
-  hipLaunchKernelGGL(k_set_u32x2, dim3(1), dim3(64), 0, st, p, v0, v1);
+hipError_t launch_set_tbase(uint32_t* tbase, uint32_t t0, uint32_t epoch, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_tbase, dim3(1), dim3(64), 0, st, tbase, t0, epoch);
+  return hipGetLastError();
+}
+
+__global__ void k_advance_tbase(uint32_t* p, uint32_t K) {
+  if (threadIdx.x == 0) {
+    p[0] += K;
+    p[2] += K;
+  }
+}
+
+hipError_t launch_advance_tbase(uint32_t* tbase, uint32_t K, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance_tbase, dim3(1), dim3(64), 0, st, tbase, K);
   return hipGetLastError();
 }
 

@@ -253,6 +253,13 @@ __device__ __forceinline__ void st_state(float* p, float v) {
 // random draws are computed while both are in flight.  At the end the block's
 // winner publishes its position from registers instead of reading back what
 // it stored.
+// Inertia weight of launch k: the w argument, or in a replayed chunk graph
+// the run's table entry for this launch's iteration.
+__device__ __forceinline__ float step_w(const SwarmDev& S, const StepArgs& A, float w,
+                                        uint32_t k) {
+  return A.w_table ? S.wtab[S.tbase[2] + k - 1u] : w;
+}
+
 template <int MAXD, bool EXACT, int OBJ, int BLOCK, bool OFF32, bool NT>
 __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, ProbArgs P,
                                                                    StepArgs A, float w,
@@ -265,6 +272,7 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
   __shared__ float s_gb[MAXD];
   const uint32_t t = S.tbase[0] + k;
   const uint32_t epoch = S.tbase[1];
+  w = step_w(S, A, w, k);
   const int d = EXACT ? MAXD : P.d;
   const int G = S.G;
   const int tid = threadIdx.x;
@@ -396,6 +404,7 @@ __global__ __launch_bounds__(128) void k_sync_step_pair(SwarmDev S, ProbArgs P, 
   __shared__ float s_gb[MAXD];
   const uint32_t t = S.tbase[0] + k;
   const uint32_t epoch = S.tbase[1];
+  w = step_w(S, A, w, k);
   const int d = MAXD;
   const int G = S.G;
   const int tid = threadIdx.x;
@@ -695,6 +704,7 @@ __global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArg
                                                  uint32_t k) {
   const uint32_t t = S.tbase[0] + k;
   const uint32_t epoch = S.tbase[1];
+  w = step_w(S, A, w, k);
   __shared__ uint64_t s_red[BLOCK / 64];
   __shared__ int s_cnt[BLOCK / 64];
   __shared__ float s_gb[MAXD];

@@ -165,17 +165,14 @@ struct pps_swarm {
   int launch = PPS_LAUNCH_EAGER;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.0f;
-  // Captured iteration loop, replayed while (T, w, wdamp, step args) match.
-  hipGraphExec_t gexec = nullptr;
-  int32_t g_T = -1;
-  float g_w = 0.0f, g_wdamp = 0.0f;
-  StepArgs g_args{};
-  // Parameters of the last run that the automatic launch mode ran without a
-  // graph, and how many runs in a row used them.
-  int32_t seen_T = -1;
-  float seen_w = 0.0f, seen_wdamp = 0.0f;
-  StepArgs seen_args{};
-  int32_t seen_runs = 0;
+  // Captured chunk of kChunk sync launches (weights from the table), replayed
+  // while the step arguments match; the run's weight table (device, grown on
+  // demand) and its host copy.
+  hipGraphExec_t cexec = nullptr;
+  StepArgs c_args{};
+  float* wtab = nullptr;
+  int32_t wtab_cap = 0;
+  std::vector<float> wtab_h;
   int32_t sampler = PPS_SAMPLER_UNIFORM;  // sampler of init / reinit
   // Local-search outputs (allocated on first use): x (d x npad), f, argmin.
   float* lx = nullptr;
@@ -359,15 +356,19 @@ static int swarm_prepare_launch(pps_swarm* s) {
   int launch = s->prm.launch;
   if (mode == PPS_MODE_SYNC) {
     // Up to 1024 particles one resident workgroup runs the whole loop.  Above,
-    // a captured graph of the iteration launches saves ~0.3 us of launch cost
-    // per iteration on small swarms (2^12 particles x 10-D: 4.73 vs 5.03 us),
-    // while from 2^19 particles launches issued one by one run the kernels
-    // slightly closer together than the graph's replay (2^20: 37.0 vs 37.2 us,
-    // 2^22: 154.7 vs 156.5 us; the host issues a launch in a few us, far
-    // ahead of the device).
-    if (launch == PPS_LAUNCH_AUTO)
+    // replayed chunk graphs save the host's ~1 us of issue per launch while a
+    // launch is short (500 iterations, us per iteration one by one against
+    // replayed: 10-D 2^14 5.75 / 5.27, 2^16 6.33 / 5.96; 3-D 2^16 4.80 /
+    // 4.25, 2^18 6.10 / 5.75), and from 2^17 particles at d > 4 launches
+    // issued one by one run the kernels as close together or closer (10-D
+    // 2^17 7.27 / 7.47, 196608 8.54 / 8.75; 32-D 2^17 28.1 / 28.4; 2^20
+    // 10-D 37.0 / 37.2), the host issuing far ahead of the device.
+    if (launch == PPS_LAUNCH_AUTO) {
+      const bool short_launch =
+          n < ((int64_t)1 << 17) || (n < ((int64_t)1 << 19) && s->P.d <= 4);
       launch = n <= 1024 ? PPS_LAUNCH_WORKGROUP
-                         : (n >= ((int64_t)1 << 19) ? PPS_LAUNCH_EAGER : PPS_LAUNCH_GRAPH);
+                         : (short_launch ? PPS_LAUNCH_GRAPH : PPS_LAUNCH_EAGER);
+    }
     if (launch == PPS_LAUNCH_WORKGROUP && n > 1024)
       return fail(PPS_E_INVALID, "workgroup launch holds at most 1024 particles (n=%lld)",
                   (long long)n);
@@ -597,42 +598,84 @@ int pps_swarm_reinit_positions(pps_swarm* s, const float* positions) {
 // after its start event, so the run's device time excludes the capture.  The
 // kernels read their stream position (iteration base and epoch) from device
 // memory, so a graph stays valid across runs and reinitialisations.
-static int prepare_sync_graph(pps_swarm* s, int32_t T, float w, float wdamp,
-                              const StepArgs& A) {
+// Sync launches per captured chunk.  A chunk graph holds kChunk launches of
+// the step kernel, whose weights come from the run's table, and the advance of
+// the stream position; any run replays it T / kChunk times and launches the
+// remaining iterations one by one.  Capturing costs ~4 us per launch once per
+// swarm (and step arguments); a replayed launch is ~1 us shorter than one
+// issued by the host, and each replay costs ~7.7 us at its boundary plus
+// ~0.07 us per launch for the table read (10-D, 2048-65536 particles, us per
+// iteration: all 512 launches in one graph 4.47-5.09; chunks of 16 4.96-5.59,
+// 64 4.66-5.30, 128 4.60-5.25; one by one 4.7-6.0).  32 keeps the capture of
+// a one-shot solve cheap.
+constexpr int32_t kChunk = 32;
+
+static int prepare_chunk_graph(pps_swarm* s, const StepArgs& A) {
   hipStream_t st = s->ctx->stream;
-  const bool hit = s->gexec && s->g_T == T && s->g_w == w && s->g_wdamp == wdamp &&
-                   s->g_args.c1 == A.c1 && s->g_args.c2 == A.c2 &&
-                   s->g_args.gbest_from == A.gbest_from;
-  if (!hit) {
-    if (s->gexec) {
-      hipGraphExecDestroy(s->gexec);
-      s->gexec = nullptr;
-    }
-    hipGraph_t g = nullptr;
-    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    float wk = w;
-    hipError_t le = hipSuccess;
-    for (int32_t k = 1; k <= T && le == hipSuccess; ++k) {
-      le = s->ks->sync_step(s->S, s->P, A, wk, (uint32_t)k, st);
-      wk = wk * wdamp;
-    }
-    hipError_t ce = hipStreamEndCapture(st, &g);
-    if (le != hipSuccess) {
-      if (g) hipGraphDestroy(g);
-      return hip_fail(le, "sync_step (capture)", __FILE__, __LINE__);
-    }
-    if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture", __FILE__, __LINE__);
-    hipError_t ie = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
-    if (ie != hipSuccess) {
-      s->gexec = nullptr;
-      return hip_fail(ie, "hipGraphInstantiate", __FILE__, __LINE__);
-    }
-    s->g_T = T;
-    s->g_w = w;
-    s->g_wdamp = wdamp;
-    s->g_args = A;
+  if (s->cexec && s->c_args.c1 == A.c1 && s->c_args.c2 == A.c2 &&
+      s->c_args.gbest_from == A.gbest_from)
+    return PPS_OK;
+  if (s->cexec) {
+    hipGraphExecDestroy(s->cexec);
+    s->cexec = nullptr;
   }
+  StepArgs At = A;
+  At.w_table = 1;
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  hipError_t le = hipSuccess;
+  for (int32_t k = 1; k <= kChunk && le == hipSuccess; ++k)
+    le = s->ks->sync_step(s->S, s->P, At, 0.0f, (uint32_t)k, st);
+  if (le == hipSuccess) le = launch_advance_tbase((uint32_t*)s->S.tbase, (uint32_t)kChunk, st);
+  hipError_t ce = hipStreamEndCapture(st, &g);
+  if (le != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return hip_fail(le, "sync_step (capture)", __FILE__, __LINE__);
+  }
+  if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture", __FILE__, __LINE__);
+  hipError_t ie = hipGraphInstantiate(&s->cexec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ie != hipSuccess) {
+    s->cexec = nullptr;
+    return hip_fail(ie, "hipGraphInstantiate", __FILE__, __LINE__);
+  }
+  s->c_args = A;
+  return PPS_OK;
+}
+
+// The run's weights w_1 = w, w_{k+1} = w_k * wdamp (FP32, in order, as the
+// launches issued one by one get them) into the device table.  A larger
+// table moves, and the captured chunk (which holds its address) is dropped.
+static int upload_weights(pps_swarm* s, int32_t T, float w, float wdamp) {
+  s->wtab_h.resize((size_t)T);
+  float wk = w;
+  for (int32_t k = 0; k < T; ++k) {
+    s->wtab_h[(size_t)k] = wk;
+    wk = wk * wdamp;
+  }
+  if (T > s->wtab_cap) {
+    const int32_t cap = std::max(T, std::max(1024, 2 * s->wtab_cap));
+    if (s->wtab) {
+      HIPCHK(hipStreamSynchronize(s->ctx->stream));
+      HIPCHK(hipFree(s->wtab));
+      s->wtab = nullptr;
+      s->wtab_cap = 0;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(float) * (size_t)cap);
+    if (e != hipSuccess)
+      return fail(PPS_E_NOMEM, "hipMalloc of the %d-entry weight table failed: %s", cap,
+                  hipGetErrorString(e));
+    s->wtab = (float*)p;
+    s->wtab_cap = cap;
+    s->S.wtab = s->wtab;
+    if (s->cexec) {
+      hipGraphExecDestroy(s->cexec);
+      s->cexec = nullptr;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(s->wtab, s->wtab_h.data(), sizeof(float) * (size_t)T,
+                        hipMemcpyHostToDevice, s->ctx->stream));
   return PPS_OK;
 }
 
@@ -644,40 +687,21 @@ int pps_swarm_run(pps_swarm* s, int32_t maxiters, float w, float wdamp) {
   StepArgs A{s->prm.c1, s->prm.c2, s->prm.gbest_from, s->prm.qlock_min_improvers};
   const uint32_t t0 = s->iter;
   const int32_t T = maxiters;
-  bool graph = T > 0 && s->prm.mode == PPS_MODE_SYNC && s->launch == PPS_LAUNCH_GRAPH;
-  if (graph && s->prm.launch == PPS_LAUNCH_AUTO) {
-    // The automatic mode captures the iteration loop once kCaptureAfter runs
-    // in a row have used the same parameters (the solve! / reinit! / solve!
-    // flow), and launches one by one before (same kernels, same results): a
-    // capture costs ~4 us per launch while a replay saves ~0.3 us per launch,
-    // so it pays only over many runs.
-    constexpr int32_t kCaptureAfter = 8;
-    const auto same = [&](int32_t T_, float w_, float wd_, const StepArgs& a) {
-      return T_ == T && w_ == w && wd_ == wdamp && a.c1 == A.c1 && a.c2 == A.c2 &&
-             a.gbest_from == A.gbest_from;
-    };
-    const bool cached = s->gexec && same(s->g_T, s->g_w, s->g_wdamp, s->g_args);
-    if (!cached) {
-      if (same(s->seen_T, s->seen_w, s->seen_wdamp, s->seen_args)) {
-        ++s->seen_runs;
-      } else {
-        s->seen_T = T;
-        s->seen_w = w;
-        s->seen_wdamp = wdamp;
-        s->seen_args = A;
-        s->seen_runs = 1;
-      }
-      if (s->seen_runs <= kCaptureAfter) graph = false;
-    }
-  }
+  // Graph launch: chunks of kChunk launches replayed from a captured graph.
+  // The automatic mode captures only for a run long enough to pay for it
+  // (or once captured).
+  bool graph = T >= kChunk && s->prm.mode == PPS_MODE_SYNC && s->launch == PPS_LAUNCH_GRAPH;
+  if (graph && s->prm.launch == PPS_LAUNCH_AUTO && !s->cexec && T < 4 * kChunk) graph = false;
   if (graph) {
-    int rc = prepare_sync_graph(s, T, w, wdamp, A);
+    int rc = upload_weights(s, T, w, wdamp);
+    if (rc != PPS_OK) return rc;
+    rc = prepare_chunk_graph(s, A);
     if (rc != PPS_OK) return rc;
   }
-  // Stream position of this run (read by the kernels, so a captured graph
+  // Stream position of this run (read by the kernels, so a captured chunk
   // replays after a reinit too); a queue-lock run starts with a clear
   // give-up flag.
-  HIPCHK(launch_set_u32x2((uint32_t*)s->S.tbase, t0, s->S.epoch, st));
+  HIPCHK(launch_set_tbase((uint32_t*)s->S.tbase, t0, s->S.epoch, st));
   if (s->prm.mode == PPS_MODE_QLOCK) HIPCHK(hipMemsetAsync(&s->S.gbrec->err, 0, sizeof(uint32_t), st));
   HIPCHK(hipEventRecord(s->ev0, st));
   if (T > 0) {
@@ -686,7 +710,12 @@ int pps_swarm_run(pps_swarm* s, int32_t maxiters, float w, float wdamp) {
         if (s->launch == PPS_LAUNCH_WORKGROUP) {
           HIPCHK(s->ks->sync_wg(s->S, s->P, A, w, wdamp, t0, T, st));
         } else if (graph) {
-          HIPCHK(hipGraphLaunch(s->gexec, st));
+          const int32_t chunks = T / kChunk;
+          for (int32_t c = 0; c < chunks; ++c) HIPCHK(hipGraphLaunch(s->cexec, st));
+          // the rest one by one: the chunks advanced the device's iteration base
+          for (int32_t k = 1; k <= T - chunks * kChunk; ++k)
+            HIPCHK(s->ks->sync_step(s->S, s->P, A, s->wtab_h[(size_t)(chunks * kChunk + k - 1)],
+                                    (uint32_t)k, st));
         } else {
           float wk = w;
           for (int32_t k = 1; k <= T; ++k) {
@@ -812,7 +841,8 @@ int pps_swarm_free(pps_swarm* s) {
   if (!s) return PPS_OK;
   if (s->ctx) hipSetDevice(s->ctx->device);
   if (s->ctx && s->ctx->stream) hipStreamSynchronize(s->ctx->stream);
-  if (s->gexec) hipGraphExecDestroy(s->gexec);
+  if (s->cexec) hipGraphExecDestroy(s->cexec);
+  if (s->wtab) hipFree(s->wtab);
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
   if (s->mem) hipFree(s->mem);
@@ -847,25 +877,49 @@ int pps_autotune_sync(pps_ctx* ctx, const pps_problem* prob, int64_t n, int32_t 
       prm.launch = PPS_LAUNCH_AUTO;  // as the swarms it is tuned for (n > 1024)
       prm.grid_per_cu = g;
       prm.particles_per_workgroup = b;
-      prm.seed = 0x7E57u;
-      pps_swarm* s = nullptr;
-      int rc = pps_swarm_init(ctx, prob, &prm, &s);
-      if (rc != PPS_OK) return rc;
+      // Two scratch swarms of the size, run in turn, as a workload holding
+      // more than one swarm: their states compete for the Infinity Cache,
+      // where workgroups of fewer waves hide memory latency worse (2^20 x
+      // 10-D in 1000-iteration runs of two swarms in turn, us per launch: 256
+      // particles at 1, 2, 3 per CU 44.2, 43.0, 41.9; 512 at 2 37.3), while a
+      // lone scratch swarm had timed 256 at 1 and 3 per CU at 35.5-35.7.  One
+      // swarm only if a second does not fit.
+      pps_swarm* sw[2] = {nullptr, nullptr};
+      int rc = PPS_OK;
+      for (int q = 0; q < 2 && rc == PPS_OK; ++q) {
+        prm.seed = 0x7E57u + (uint64_t)q;
+        rc = pps_swarm_init(ctx, prob, &prm, &sw[q]);
+        if (rc == PPS_E_NOMEM && q == 1) {
+          sw[1] = nullptr;
+          rc = PPS_OK;
+        }
+      }
+      if (rc != PPS_OK) {
+        pps_swarm_free(sw[0]);
+        return rc;
+      }
+      const int ns = sw[1] ? 2 : 1;
       float ms = 0.0f, best_ms = 1e30f;
       // Timed past the first thousand iterations, where a solve spends most of
       // its time: early on most particles improve and write their best
       // positions, and the fastest shape there is not always the fastest
       // later (2^20 x 10-D: 4 workgroups per CU lead 2 per CU by 3 us at
       // iteration 100 and trail or tie them from iteration 1000).  These
-      // runs launch one by one (the automatic mode captures a graph only
-      // after 8 runs with the same parameters); the best of three is kept.
-      rc = pps_swarm_run(s, std::max(iters, 1000), 0.7298f, 1.0f);
+      // runs launch as the automatic mode does at this size (replayed chunk
+      // graphs or one by one); the best of three is kept.
+      for (int q = 0; q < ns && rc == PPS_OK; ++q)
+        rc = pps_swarm_run(sw[q], std::max(iters, 1000), 0.7298f, 1.0f);
       for (int rep = 0; rep < 3 && rc == PPS_OK; ++rep) {
-        rc = pps_swarm_run(s, iters, 0.7298f, 1.0f);
-        if (rc == PPS_OK) rc = pps_swarm_elapsed_ms(s, &ms);
-        best_ms = std::min(best_ms, ms);
+        float sum = 0.0f;
+        for (int q = 0; q < ns && rc == PPS_OK; ++q) {
+          rc = pps_swarm_run(sw[q], iters, 0.7298f, 1.0f);
+          if (rc == PPS_OK) rc = pps_swarm_elapsed_ms(sw[q], &ms);
+          sum += ms;
+        }
+        best_ms = std::min(best_ms, sum / (float)ns);
       }
-      pps_swarm_free(s);
+      pps_swarm_free(sw[0]);
+      pps_swarm_free(sw[1]);
       if (rc != PPS_OK) return rc;
       const float us = best_ms * 1e3f / (float)iters;
       if (best_g == 0 || us < best_us) {

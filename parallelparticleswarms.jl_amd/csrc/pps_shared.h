@@ -126,7 +126,11 @@ struct SwarmDev {
   // executes iteration t = tbase[0] + k of epoch tbase[1], so a captured
   // iteration loop can be replayed from any starting iteration and after any
   // number of reinitialisations.  The host writes both words before each run.
+  // tbase[2] is the offset into wtab of the current chunk of a chunk graph.
   const uint32_t* tbase;
+  // Inertia weights of the current run's iterations (device; w_k as the
+  // host's iterated FP32 product), read by launches with StepArgs::w_table.
+  const float* wtab;
   uint32_t key0, key1;  // Philox key (seed)
   uint32_t epoch;       // epoch of the kernels launched outside a graph
   int32_t G;

@@ -65,43 +65,59 @@ def test_sync_multiblock_trajectory(launch, obj, d, n):
 
 @pytest.mark.parametrize("obj,d,n", [(O.RASTRIGIN, 10, 5000), (O.ROSENBROCK, 3, 70000)])
 def test_graph_replay_after_reinit(obj, d, n):
-    # init -> solve! -> reinit! -> solve! with the iteration loop in a captured
-    # graph (same T, so the graph is replayed, not recaptured): every run after
-    # a reinit must draw from the new epoch's random stream
+    # init -> solve! -> reinit! -> solve! with the iterations in replays of a
+    # captured chunk graph (40 = a chunk of 32 + 8 launched one by one):
+    # every run after a reinit must draw from the new epoch's random stream
     note(f"graph replay after reinit obj={obj} d={d} n={n}")
     g, o = make_pair(obj, d, n, seed=31 + n, launch=L.LAUNCH_GRAPH)
     for rnd in range(3):
-        g.run(12)
-        o.run_sync(12)
+        g.run(40)
+        o.run_sync(40)
         assert_same_state(g, o, f"round {rnd}")
         assert_same_gbest(g, o, f"round {rnd}")
         g.reinit()
         o.reinit()
         assert_same_state(g, o, f"reinit {rnd}")
-    g.run(12)
-    o.run_sync(12)
+    g.run(40)
+    o.run_sync(40)
     assert_same_state(g, o, "after the last reinit")
     assert_same_gbest(g, o, "after the last reinit")
 
 
-def test_automatic_launch_captures_on_repeat():
-    # the automatic launch mode launches iterations one by one and captures a
-    # graph once 8 runs in a row have used the same parameters; every run (one
-    # by one, the capture, replays, a new length, after a reinit) must equal
-    # the oracle
-    note("automatic launch: one by one, capture after 8 repeats, replay")
-    g, o = make_pair(O.RASTRIGIN, 10, 5000, seed=77, launch=L.LAUNCH_AUTO)
-    for rnd, T in enumerate([4] * 11 + [3, 3] + [4] * 10):
-        g.run(T)
-        o.run_sync(T)
+@pytest.mark.parametrize("launch", [L.LAUNCH_AUTO, L.LAUNCH_GRAPH])
+def test_chunk_graph_runs_match_oracle(launch):
+    # graph launches replay a captured chunk of 32 iterations whose inertia
+    # weights come from the run's table and launch the rest one by one; the
+    # automatic mode captures from 128 iterations on (or once captured).  Runs
+    # shorter than a chunk, exact multiples, remainders, new weights, a table
+    # that outgrows its 1024 entries (the chunk is captured again) and a
+    # reinit must all equal the oracle
+    note(f"chunk graph runs launch={launch}")
+    g, o = make_pair(O.RASTRIGIN, 10, 5000, seed=77, launch=launch)
+    runs = [(5, 0.7298, 1.0), (40, 0.7298, 0.99), (70, 0.9, 0.995), (32, 0.7298, 1.0),
+            (3, 0.5, 1.0), (130, 0.7298, 0.97), (33, 0.7298, 0.99), (1500, 0.8, 0.999),
+            (64, 0.7298, 1.0), (100, 0.6, 0.98)]
+    for rnd, (T, w, wd) in enumerate(runs):
+        g.run(T, w, wd)
+        o.run_sync(T, w, wd)
         assert_same_state(g, o, f"run {rnd} (T={T})")
         assert_same_gbest(g, o, f"run {rnd} (T={T})")
-        if rnd == 12:
+        if rnd == 5:
             g.reinit()
             o.reinit()
-    g.run(5)
-    o.run_sync(5)
-    assert_same_state(g, o, "last run")
+
+
+def test_chunk_graph_ode_pair_kernel():
+    # the ODE estimation's two-lanes-per-particle step in replayed chunks
+    rng = np.random.default_rng(4)
+    obs = (rng.uniform(0.5, 3, (101, 2))).astype(np.float32)
+    note("chunk graph LV pair kernel")
+    g, o = make_pair(O.LOTKA_VOLTERRA, 4, 300, seed=23, lb=0.0, ub=5.0, obs=obs,
+                     launch=L.LAUNCH_GRAPH)
+    g.run(40, 0.7298, 0.99)
+    o.run_sync(40, 0.7298, 0.99)
+    assert_same_state(g, o, "after 40 iterations")
+    assert_same_gbest(g, o)
 
 
 def test_sync_current_mode_multiblock():
