@@ -170,7 +170,7 @@ def c4_mix(P, L, ctx, count=4096, plain_sample=256, served_qps=None):
     wall = time.perf_counter() - t0
     # plain per-problem launch loop on a sample of the same mix
     t0 = time.perf_counter()
-    plain = [P.solve_alone(q) for q in reqs[:plain_sample]]
+    plain = [P.solve_alone(q, _launch(L, L.LAUNCH_AUTO)) for q in reqs[:plain_sample]]
     pwall = time.perf_counter() - t0
     mismatch = sum(1 for a, b in zip(res, plain)
                    if not (np.array_equal(a.u, b.u) and a.cost_f32 == b.cost_f32))
