@@ -696,22 +696,30 @@ __device__ __forceinline__ float ld_acq(const float* p) {
 
 // Queue-lock iteration (src/kernels.jl:38-125) with BLOCK = 1024 particles
 // per workgroup, one tile per workgroup.  Each workgroup snapshots the live
-// global best, steps, counts the particles whose best cost beats the live
-// global best, and, if at least min_improvers did, publishes its best under a
-// bounded spin lock when it still beats the global best.
+// global best (position and cost) at its start, steps, counts the particles
+// whose best cost beats the snapshot, and, if at least min_improvers did,
+// publishes its best under a bounded spin lock when it still beats the live
+// global best.  Workgroups race on the record as the reference's blocks do;
+// which publications a block sees depends on the schedule either way.
 template <int MAXD, bool EXACT, int OBJ, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArgs A, float w,
                                                  uint32_t k) {
   const uint32_t t = S.tbase[0] + k;
   const uint32_t epoch = S.tbase[1];
   w = step_w(S, A, w, k);
+  static_assert(BLOCK == 1024, "the slot reductions read 16 slots");
   __shared__ uint64_t s_red[BLOCK / 64];
   __shared__ int s_cnt[BLOCK / 64];
   __shared__ float s_gb[MAXD];
+  __shared__ float s_gcost;
   const int d = EXACT ? MAXD : P.d;
   GbRec* R = S.gbrec;
+  // The live global best as this block finds it at its start: position and
+  // cost in one round trip (the publication re-reads the cost under the lock).
   if ((int)threadIdx.x < d) s_gb[threadIdx.x] = ld_acq(&R->pos[threadIdx.x]);
+  if (threadIdx.x == 64) s_gcost = ld_acq(&R->cost);
   __syncthreads();
+  const float gcost = s_gcost;
   float gb[MAXD];
 #pragma unroll
   for (int j = 0; j < MAXD; ++j) gb[j] = j < d ? s_gb[j] : 0.0f;
@@ -743,11 +751,22 @@ __global__ __launch_bounds__(BLOCK) void k_qlock(SwarmDev S, ProbArgs P, StepArg
       S.pc[i] = pc;
     }
   }
-  const bool imp = act && (pc < ld_acq(&R->cost));
-  const int count = block_sum_i32<BLOCK>(imp ? 1 : 0, s_cnt);
-  const uint64_t mykey = imp ? make_key(pc, (uint32_t)i) : KEY_NONE;
-  const uint64_t bk = block_min_u64<BLOCK>(mykey, s_red);
-  if (count >= A.min_improvers && imp && mykey == bk) {
+  // count the particles whose best cost beats it and find the first lowest of
+  // them: per wavefront a ballot and a 32-bit DPP minimum, then 16 slots
+  const bool imp = act && (pc < gcost);
+  uint32_t om;
+  const int wl = wave_argmin_first(cost_order(pc), imp, om);
+  const uint64_t bal = __ballot(imp);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_red[wid] = wl < 0 ? KEY_NONE
+                        : ((uint64_t)om << 32) | (uint32_t)((int64_t)blockIdx.x * BLOCK + wid * 64 + wl);
+    s_cnt[wid] = (int)__popcll(bal);
+  }
+  __syncthreads();
+  const uint64_t bk = slots16_min(s_red);
+  const int count = slots16_sum(s_cnt);
+  if (count > 0 && count >= A.min_improvers && imp && (uint32_t)i == (uint32_t)bk) {
     // Bounded spin: a holder is a running wave that releases within a few
     // hundred cycles; give up (and flag it) rather than wait forever.
     bool got = false;
