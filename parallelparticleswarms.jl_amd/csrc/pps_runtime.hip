@@ -655,11 +655,17 @@ static int upload_weights(pps_swarm* s, int32_t T, float w, float wdamp) {
   }
   if (T > s->wtab_cap) {
     const int32_t cap = std::max(T, std::max(1024, 2 * s->wtab_cap));
+    // the captured chunk holds the old table's address: drop both first
+    if (s->cexec) {
+      hipGraphExecDestroy(s->cexec);
+      s->cexec = nullptr;
+    }
     if (s->wtab) {
       HIPCHK(hipStreamSynchronize(s->ctx->stream));
       HIPCHK(hipFree(s->wtab));
       s->wtab = nullptr;
       s->wtab_cap = 0;
+      s->S.wtab = nullptr;
     }
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, sizeof(float) * (size_t)cap);
@@ -669,10 +675,6 @@ static int upload_weights(pps_swarm* s, int32_t T, float w, float wdamp) {
     s->wtab = (float*)p;
     s->wtab_cap = cap;
     s->S.wtab = s->wtab;
-    if (s->cexec) {
-      hipGraphExecDestroy(s->cexec);
-      s->cexec = nullptr;
-    }
   }
   HIPCHK(hipMemcpyAsync(s->wtab, s->wtab_h.data(), sizeof(float) * (size_t)T,
                         hipMemcpyHostToDevice, s->ctx->stream));
