@@ -46,6 +46,21 @@ __device__ __forceinline__ uint64_t block_min_u64(uint64_t k, uint64_t* s_red) {
   return r;
 }
 
+// The same without the trailing barrier: for a kernel's last reduction, or
+// one followed by a barrier before its slots are written again.
+template <int BLOCK>
+__device__ __forceinline__ uint64_t block_min_u64_once(uint64_t k, uint64_t* s_red) {
+  k = wave_min_u64(k);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (BLOCK == 64) return k;
+  if (lane == 0) s_red[wid] = k;
+  __syncthreads();
+  uint64_t r = s_red[0];
+#pragma unroll
+  for (int i = 1; i < BLOCK / 64; ++i) r = umin64(r, s_red[i]);
+  return r;
+}
+
 // Minimum over a workgroup of nw wavefronts (nw = blockDim.x / 64, uniform);
 // with one wavefront no barrier is taken.  Every thread receives the result.
 __device__ __forceinline__ uint64_t block_min_u64_n(uint64_t k, uint64_t* s_red, int nw) {
@@ -114,7 +129,7 @@ template <int BLOCK>
 __device__ __forceinline__ void publish_block(const SwarmDev& S, uint32_t par, int d,
                                               uint64_t mykey, const float* src,
                                               uint64_t* s_red) {
-  const uint64_t bk = block_min_u64<BLOCK>(mykey, s_red);
+  const uint64_t bk = block_min_u64_once<BLOCK>(mykey, s_red);
   const int b = blockIdx.x;
   if (bk != KEY_NONE) {
     const int64_t wi = (int64_t)(uint32_t)bk;
@@ -305,7 +320,9 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
   __builtin_amdgcn_sched_barrier(0);
   if (PRE && i < S.n) step_draws<MAXD>(r1, r2, (uint32_t)i, t, epoch, S.key0, S.key1, d);
   for (int j = tid + BLOCK; j < G; j += BLOCK) gk = umin64(gk, keys[j]);
-  gk = block_min_u64<BLOCK>(gk, s_red);
+  // (the barrier after s_gb below orders these slot reads before the end's
+  // slot writes)
+  gk = block_min_u64_once<BLOCK>(gk, s_red);
   {
     const uint32_t gidx = (uint32_t)gk;
     const int gblk = (int)((gidx / (uint32_t)S.block) % (uint32_t)G);
@@ -376,7 +393,7 @@ __global__ __launch_bounds__(BLOCK) PPS_SYNC_ATTR void k_sync_step(SwarmDev S, P
     }
   }
   if constexpr (HOLD) {
-    const uint64_t bk = block_min_u64<BLOCK>(mykey, s_red);
+    const uint64_t bk = block_min_u64_once<BLOCK>(mykey, s_red);
     if (bk != KEY_NONE && mykey == bk) {
 #pragma unroll
       for (int j = 0; j < MAXD; ++j)
